@@ -1,0 +1,174 @@
+/*
+ * mppca.h -- C ABI of the MI355X (gfx950) patch-wise MP-PCA denoiser.
+ *
+ * The reference (paquiteau/lowrank-gpu-fastest) exposes one Python function,
+ * `denoise(vol_np) -> np.ndarray` (step8_tuned.py:116-146), whose device work
+ * is split into Triton kernels and vendor calls. Each entry point below
+ * replaces one of those stages; the Python host side
+ * (lowrank-gpu-fastest_amd/denoise.py) binds them with ctypes and keeps the
+ * reference's `denoise` signature.
+ *
+ * Conventions
+ *   - All array pointers are DEVICE pointers owned by the caller (hipMalloc or
+ *     torch tensors); the library never allocates device memory.
+ *   - `stream` is a hipStream_t passed as void*; NULL means the null stream.
+ *     Calls only enqueue work; they never synchronise.
+ *   - Return value: 0 = ok, < 0 = invalid argument (MPPCA_E*), > 0 = the
+ *     hipError_t reported by the runtime. mppca_last_error() gives a
+ *     thread-local message for the last failing call on this thread.
+ *   - Volumes are float32, C-contiguous (X, Y, Z, T): voxel v = (x*Y + y)*Z + z
+ *     owns the T floats at v*T.
+ *   - Patches follow the reference's corner rule (common.py:37-44): on each
+ *     axis the strided corners 0, s, 2s, ... that fit, plus one last corner
+ *     flush with the border when the stride misses it; patch index
+ *     b = (ix*ncy + iy)*ncz + iz (x outer, z inner).
+ *   - Limits: 2 <= T <= 128, px*py*pz <= 4096, patch <= volume on each axis.
+ *   - Thread safety: entry points keep no mutable global state; concurrent
+ *     calls on distinct streams with distinct buffers are safe.
+ */
+#ifndef MPPCA_H
+#define MPPCA_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MPPCA_OK 0
+#define MPPCA_EINVAL (-1)       /* bad shape, pointer or size */
+#define MPPCA_EUNSUPPORTED (-2) /* outside the supported limits */
+#define MPPCA_ENOSPACE (-3)     /* workspace too small */
+
+/* Arithmetic forms of the whole path (mppca_workspace_bytes,
+ * mppca_denoise_device). Both read and write float32 volumes.
+ *   PLAIN: Gram on FP64 MFMA (exact products of the float32 centred data),
+ *          FP64 eigensolver (Householder + bisection + twisted factorisation),
+ *          FP32 MFMA reconstruction. Matches step 0 to ~1e-7.
+ *   FP32 : the reference's all-FP32 arithmetic (FP32 MFMA Gram, FP32 Jacobi
+ *          eigensolver), as step8_tuned.py computes it. */
+#define MPPCA_FORM_PLAIN 0
+#define MPPCA_FORM_FP32 1
+
+/* Version of this ABI (major * 100 + minor). */
+int mppca_abi_version(void);
+
+/* Message of the last failing call on the calling thread ("" if none). */
+const char* mppca_last_error(void);
+
+/* Number of patches of a geometry (common.py:37-44 corner rule), or < 0. */
+int64_t mppca_num_patches(int X, int Y, int Z, int px, int py, int pz, int stride);
+
+/* ---------------------------------------------------------------------------
+ * Stage launchers
+ * ------------------------------------------------------------------------- */
+
+/* Temporal centring of every voxel: mean[v] = sum_t vol[v,t] / T and
+ * xc[v,t] = vol[v,t] - mean[v].
+ * Replaces the per-patch row mean of gram_kernel / recon_scatter_kernel
+ * (step8_tuned.py:51-52, 81-82; step0_numpy_loop.py:11), computed once per
+ * voxel instead of once per (patch, voxel). */
+int mppca_centre(const float* vol, float* xc, float* mean, int64_t nvox, int T,
+                 void* stream);
+
+/* Gram matrices of patches [b0, b0+nb): gram[b] = Xc_b^T Xc_b (T x T,
+ * row-major, FP32 MFMA accumulation), Xc_b the N x T centred patch.
+ * Replaces gram_kernel (step8_tuned.py:32-56, launch :132). */
+int mppca_gram(const float* xc, float* gram, int X, int Y, int Z, int T,
+               int px, int py, int pz, int stride, int64_t b0, int64_t nb,
+               void* stream);
+
+/* As mppca_gram with FP64 accumulation: every product of two float32 values
+ * is exact in FP64 (v_mfma_f64_16x16x4f64), gram[b] is T x T doubles.
+ * Replaces gram_kernel (step8_tuned.py:32-56) in the PLAIN form. */
+int mppca_gram64(const float* xc, double* gram, int X, int Y, int Z, int T,
+                 int px, int py, int pz, int stride, int64_t b0, int64_t nb,
+                 void* stream);
+
+/* Batched symmetric eigensolver, cyclic Jacobi (parallel round-robin order),
+ * one wavefront per T x T matrix held in LDS.
+ * lam[b, :] = eigenvalues ascending; vec[b] (row-major) holds the matching
+ * eigenvectors in COLUMNS (vec[b][r*T + c] = component r of vector c), the
+ * layout torch.linalg.eigh returns; info[b] = sweeps used, or -1 if
+ * max_sweeps did not converge.
+ * Replaces torch.linalg.eigh (step8_tuned.py:134) / XsyevBatched
+ * (step5_xsyev_fp32.py:26-74). */
+int mppca_eigh_jacobi(const float* gram, float* lam, float* vec, int32_t* info,
+                      int64_t nb, int T, int max_sweeps, void* stream);
+
+/* MP rank from ascending eigenvalues lam[b, 0..T-1] of N-voxel patches.
+ * rank[b] = p (count semantics), weight[b] = 1 / (2 + p),
+ * margin[b] = min_i |lhs_i - rhs_i| / (error bound of lhs_i - rhs_i)
+ * (< 1: the decision at some index is within the eigenvalue error bound).
+ * Arithmetic is step 0's: eigenvalues / n and suffix sums in float32, the two
+ * sides of the inequality in float64.
+ * Replaces mp_keep (step8_tuned.py:96-113; step0_numpy_loop.py:15-24). */
+int mppca_mp_threshold(const float* lam, int64_t nb, int T, int N,
+                       int32_t* rank, float* weight, float* margin, void* stream);
+
+/* Fused eigensolver + MP rank + rank-p projector for patches b in [0, nb):
+ * proj[b] = V_p V_p^T (T x T), plus rank, weight, margin, info as above and
+ * lam_desc[b, :] = eigenvalues in descending order (any of lam_desc,
+ * margin, info may be NULL).
+ * Replaces eigh + mp_keep + the projector bmm (step8_tuned.py:134-137). */
+int mppca_eigh_mp(const float* gram, int64_t nb, int T, int N, int max_sweeps,
+                  float* proj, int32_t* rank, float* weight, float* margin,
+                  float* lam_desc, int32_t* info, void* stream);
+
+/* FP64 eigensolver + MP rank + projector for patches b in [0, nb): FP64 Gram
+ * matrices in, proj[b] = V_p V_p^T as float32, rank / weight / margin /
+ * lam_desc (float32, descending; may be NULL) / info (0 ok, -2 non-finite
+ * input; may be NULL) out. Only the top-p eigenvectors are computed.
+ * Replaces eigh + mp_keep + the projector bmm (step8_tuned.py:134-137) in the
+ * PLAIN form. */
+int mppca_eig64_mp(const double* gram, int64_t nb, int T, int N, float* proj,
+                   int32_t* rank, float* weight, float* margin, float* lam_desc,
+                   int32_t* info, void* stream);
+
+/* Reconstruction + weighted overlap scatter for patches [b0, b0+nb):
+ * acc[row(v), :] += weight[b] * xc[row(v), :] @ proj[b] for every voxel v of
+ * patch b (float atomics into a zero-initialised acc of the volume's shape).
+ * proj and weight are indexed by b - b0.
+ * Replaces recon_scatter_kernel (step8_tuned.py:59-86, launch :139-143); the
+ * "+ m" term and the weight sum move to mppca_normalize. */
+int mppca_recon_scatter(const float* xc, const float* proj, const float* weight,
+                        float* acc, int X, int Y, int Z, int T, int px, int py,
+                        int pz, int stride, int64_t b0, int64_t nb, void* stream);
+
+/* out[v, :] = mean[v] + acc[v, :] / W_v with W_v = sum of weight[b] over the
+ * patches b that contain v (gathered, no atomics); 0 where W_v = 0.
+ * weight holds all P patch weights. out may alias acc.
+ * Replaces `out / wsum.clamp_min(1e-12)` and the wsum atomics
+ * (step8_tuned.py:86, 145; step0_numpy_loop.py:37-39). */
+int mppca_normalize(const float* acc, const float* mean, const float* weight,
+                    float* out, int X, int Y, int Z, int T, int px, int py,
+                    int pz, int stride, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Whole path on device buffers
+ * ------------------------------------------------------------------------- */
+
+/* Bytes of workspace mppca_denoise_device needs for a geometry and form
+ * (< 0: error). */
+int64_t mppca_workspace_bytes(int X, int Y, int Z, int T, int px, int py,
+                              int pz, int stride, int form);
+
+/* Denoise a device-resident volume: centre, Gram, eigensolver + MP rank,
+ * reconstruction + scatter, normalisation; all enqueued on `stream`.
+ * vol and out are (X, Y, Z, T) float32 device buffers (distinct);
+ * workspace is >= mppca_workspace_bytes(...) bytes of device memory, 256-byte
+ * aligned. rank / margin / info (each P entries, may be NULL) receive the
+ * per-patch rank, MP margin and eigensolver sweeps.
+ * Replaces step8_tuned.denoise between the upload and the download
+ * (step8_tuned.py:119-145). */
+int mppca_denoise_device(const float* vol, float* out, int X, int Y, int Z,
+                         int T, int px, int py, int pz, int stride, int form,
+                         void* workspace, int64_t workspace_bytes,
+                         int32_t* rank, float* margin, int32_t* info,
+                         void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MPPCA_H */

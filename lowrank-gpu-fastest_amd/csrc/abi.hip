@@ -1,0 +1,278 @@
+// abi.hip -- the extern "C" entry points of libmppca.so (include/mppca.h):
+// argument checks, workspace layout, stage sequencing. Host code only.
+#include <stdarg.h>
+#include <stdio.h>
+
+#include "common.h"
+
+namespace mppca {
+
+static thread_local char g_err[512] = "";
+
+int set_err(int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+int launch_status(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return set_err((int)e, "%s: %s", what, hipGetErrorString(e));
+  return MPPCA_OK;
+}
+
+int prepare_lds(const void* fn, size_t bytes) {
+  if (bytes <= 65536) return MPPCA_OK;
+  hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  if (e != hipSuccess)
+    return set_err((int)e, "cannot reserve %zu bytes of LDS: %s", bytes, hipGetErrorString(e));
+  return MPPCA_OK;
+}
+
+static bool axis_counts(int S, int p, int s, int* ns, int* nc) {
+  if (p < 1 || p > S || s < 1) return false;
+  *ns = (S - p) / s + 1;
+  *nc = *ns + (((S - p) % s) != 0 ? 1 : 0);
+  return true;
+}
+
+int make_geom(Geom* g, int X, int Y, int Z, int T, int px, int py, int pz, int stride) {
+  if (X < 1 || Y < 1 || Z < 1)
+    return set_err(MPPCA_EINVAL, "volume shape (%d,%d,%d) must be positive", X, Y, Z);
+  if (T < 2 || T > MAX_T)
+    return set_err(MPPCA_EUNSUPPORTED, "T=%d outside the supported [2, %d]", T, MAX_T);
+  if (stride < 1) return set_err(MPPCA_EINVAL, "stride %d must be >= 1", stride);
+  if (px < 1 || py < 1 || pz < 1)
+    return set_err(MPPCA_EINVAL, "patch (%d,%d,%d) must be positive", px, py, pz);
+  if (px > X || py > Y || pz > Z)
+    return set_err(MPPCA_EINVAL, "patch (%d,%d,%d) larger than volume (%d,%d,%d)", px, py, pz,
+                   X, Y, Z);
+  const long long N = (long long)px * py * pz;
+  if (N > MAX_N)
+    return set_err(MPPCA_EUNSUPPORTED, "patch of %lld voxels exceeds %d", N, MAX_N);
+  g->X = X;
+  g->Y = Y;
+  g->Z = Z;
+  g->T = T;
+  g->px = px;
+  g->py = py;
+  g->pz = pz;
+  g->stride = stride;
+  axis_counts(X, px, stride, &g->nsx, &g->ncx);
+  axis_counts(Y, py, stride, &g->nsy, &g->ncy);
+  axis_counts(Z, pz, stride, &g->nsz, &g->ncz);
+  g->N = (int)N;
+  g->P = (long long)g->ncx * g->ncy * g->ncz;
+  g->nvox = (long long)X * Y * Z;
+  if (g->nvox * T >= (1LL << 40)) return set_err(MPPCA_EUNSUPPORTED, "volume too large");
+  return MPPCA_OK;
+}
+
+namespace {
+
+struct Workspace {
+  float* xc;
+  float* mean;
+  void* gram;   // P*T*T doubles (plain form) or floats (fp32 form)
+  float* proj;  // P*T*T floats (plain form; the fp32 form projects in place of gram)
+  float* weight;
+  int* rank;
+  float* margin;
+  int* info;
+  size_t bytes;
+};
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+Workspace carve_workspace(char* base, const Geom& g, int form) {
+  Workspace w;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    char* p = base ? base + off : nullptr;
+    off += align256(bytes);
+    return (void*)p;
+  };
+  const size_t ptt = (size_t)g.P * g.T * g.T;
+  w.xc = (float*)take((size_t)g.nvox * g.T * sizeof(float));
+  w.mean = (float*)take((size_t)g.nvox * sizeof(float));
+  if (form == MPPCA_FORM_PLAIN) {
+    w.gram = take(ptt * sizeof(double));
+    w.proj = (float*)take(ptt * sizeof(float));
+  } else {
+    w.gram = take(ptt * sizeof(float));
+    w.proj = (float*)w.gram;
+  }
+  w.weight = (float*)take((size_t)g.P * sizeof(float));
+  w.rank = (int*)take((size_t)g.P * sizeof(int));
+  w.margin = (float*)take((size_t)g.P * sizeof(float));
+  w.info = (int*)take((size_t)g.P * sizeof(int));
+  w.bytes = off;
+  return w;
+}
+
+bool valid_form(int form) { return form == MPPCA_FORM_PLAIN || form == MPPCA_FORM_FP32; }
+
+}  // namespace
+}  // namespace mppca
+
+using namespace mppca;
+
+extern "C" {
+
+int mppca_abi_version(void) { return MPPCA_ABI_VERSION; }
+
+const char* mppca_last_error(void) { return g_err; }
+
+int64_t mppca_num_patches(int X, int Y, int Z, int px, int py, int pz, int stride) {
+  Geom g;
+  const int rc = make_geom(&g, X, Y, Z, 2, px, py, pz, stride);
+  if (rc) return rc;
+  return g.P;
+}
+
+int mppca_centre(const float* vol, float* xc, float* mean, int64_t nvox, int T, void* stream) {
+  if (!vol || !xc || !mean) return set_err(MPPCA_EINVAL, "centre: null pointer");
+  if (nvox < 0 || T < 1 || T > MAX_T) return set_err(MPPCA_EINVAL, "centre: bad sizes");
+  if (nvox == 0) return MPPCA_OK;
+  return launch_centre(vol, xc, mean, (long long)nvox, T, (hipStream_t)stream);
+}
+
+static int check_range(const Geom& g, int64_t b0, int64_t nb, const char* what) {
+  if (b0 < 0 || nb < 0 || b0 + nb > g.P)
+    return set_err(MPPCA_EINVAL, "%s: patch range [%lld, %lld) outside [0, %lld)", what,
+                   (long long)b0, (long long)(b0 + nb), g.P);
+  return MPPCA_OK;
+}
+
+int mppca_gram(const float* xc, float* gram, int X, int Y, int Z, int T, int px, int py, int pz,
+               int stride, int64_t b0, int64_t nb, void* stream) {
+  Geom g;
+  int rc = make_geom(&g, X, Y, Z, T, px, py, pz, stride);
+  if (rc) return rc;
+  if (!xc || !gram) return set_err(MPPCA_EINVAL, "gram: null pointer");
+  if ((rc = check_range(g, b0, nb, "gram"))) return rc;
+  if (nb == 0) return MPPCA_OK;
+  return launch_gram(xc, gram, g, b0, nb, (hipStream_t)stream);
+}
+
+int mppca_gram64(const float* xc, double* gram, int X, int Y, int Z, int T, int px, int py,
+                 int pz, int stride, int64_t b0, int64_t nb, void* stream) {
+  Geom g;
+  int rc = make_geom(&g, X, Y, Z, T, px, py, pz, stride);
+  if (rc) return rc;
+  if (!xc || !gram) return set_err(MPPCA_EINVAL, "gram64: null pointer");
+  if ((rc = check_range(g, b0, nb, "gram64"))) return rc;
+  if (nb == 0) return MPPCA_OK;
+  return launch_gram64(xc, gram, g, b0, nb, (hipStream_t)stream);
+}
+
+int mppca_eigh_jacobi(const float* gram, float* lam, float* vec, int32_t* info, int64_t nb, int T,
+                      int max_sweeps, void* stream) {
+  if (!gram || !lam || !vec) return set_err(MPPCA_EINVAL, "eigh: null pointer");
+  if (T < 2 || T > MAX_T) return set_err(MPPCA_EUNSUPPORTED, "eigh: T=%d outside [2, %d]", T, MAX_T);
+  if (nb < 0 || max_sweeps < 1) return set_err(MPPCA_EINVAL, "eigh: bad nb / max_sweeps");
+  if (nb == 0) return MPPCA_OK;
+  return launch_eigh_jacobi(gram, lam, vec, info, nb, T, max_sweeps, (hipStream_t)stream);
+}
+
+int mppca_mp_threshold(const float* lam, int64_t nb, int T, int N, int32_t* rank, float* weight,
+                       float* margin, void* stream) {
+  if (!lam || !rank || !weight) return set_err(MPPCA_EINVAL, "mp: null pointer");
+  if (T < 2 || T > MAX_T || N < 1) return set_err(MPPCA_EUNSUPPORTED, "mp: bad T=%d / N=%d", T, N);
+  if (nb < 0) return set_err(MPPCA_EINVAL, "mp: nb < 0");
+  if (nb == 0) return MPPCA_OK;
+  return launch_mp(lam, nb, T, N, rank, weight, margin, (hipStream_t)stream);
+}
+
+int mppca_eigh_mp(const float* gram, int64_t nb, int T, int N, int max_sweeps, float* proj,
+                  int32_t* rank, float* weight, float* margin, float* lam_desc, int32_t* info,
+                  void* stream) {
+  if (!gram || !proj || !rank || !weight) return set_err(MPPCA_EINVAL, "eigh_mp: null pointer");
+  if (T < 2 || T > MAX_T || N < 1)
+    return set_err(MPPCA_EUNSUPPORTED, "eigh_mp: bad T=%d / N=%d", T, N);
+  if (nb < 0 || max_sweeps < 1) return set_err(MPPCA_EINVAL, "eigh_mp: bad nb / max_sweeps");
+  if (nb == 0) return MPPCA_OK;
+  return launch_eigh_mp(gram, nb, T, N, max_sweeps, proj, rank, weight, margin, lam_desc, info,
+                        (hipStream_t)stream);
+}
+
+int mppca_eig64_mp(const double* gram, int64_t nb, int T, int N, float* proj, int32_t* rank,
+                   float* weight, float* margin, float* lam_desc, int32_t* info, void* stream) {
+  if (!gram || !proj || !rank || !weight) return set_err(MPPCA_EINVAL, "eig64_mp: null pointer");
+  if (T < 2 || T > MAX_T || N < 1)
+    return set_err(MPPCA_EUNSUPPORTED, "eig64_mp: bad T=%d / N=%d", T, N);
+  if (nb < 0) return set_err(MPPCA_EINVAL, "eig64_mp: nb < 0");
+  if (nb == 0) return MPPCA_OK;
+  return launch_eig64_mp(gram, nb, T, N, proj, rank, weight, margin, lam_desc, info,
+                         (hipStream_t)stream);
+}
+
+int mppca_recon_scatter(const float* xc, const float* proj, const float* weight, float* acc, int X,
+                        int Y, int Z, int T, int px, int py, int pz, int stride, int64_t b0,
+                        int64_t nb, void* stream) {
+  Geom g;
+  int rc = make_geom(&g, X, Y, Z, T, px, py, pz, stride);
+  if (rc) return rc;
+  if (!xc || !proj || !weight || !acc) return set_err(MPPCA_EINVAL, "recon: null pointer");
+  if ((rc = check_range(g, b0, nb, "recon"))) return rc;
+  if (nb == 0) return MPPCA_OK;
+  return launch_recon(xc, proj, weight, acc, g, b0, nb, (hipStream_t)stream);
+}
+
+int mppca_normalize(const float* acc, const float* mean, const float* weight, float* out, int X,
+                    int Y, int Z, int T, int px, int py, int pz, int stride, void* stream) {
+  Geom g;
+  const int rc = make_geom(&g, X, Y, Z, T, px, py, pz, stride);
+  if (rc) return rc;
+  if (!acc || !mean || !weight || !out) return set_err(MPPCA_EINVAL, "normalize: null pointer");
+  return launch_normalize(acc, mean, weight, out, g, (hipStream_t)stream);
+}
+
+int64_t mppca_workspace_bytes(int X, int Y, int Z, int T, int px, int py, int pz, int stride,
+                              int form) {
+  Geom g;
+  const int rc = make_geom(&g, X, Y, Z, T, px, py, pz, stride);
+  if (rc) return rc;
+  if (!valid_form(form)) return set_err(MPPCA_EINVAL, "unknown form %d", form);
+  return (int64_t)carve_workspace(nullptr, g, form).bytes;
+}
+
+int mppca_denoise_device(const float* vol, float* out, int X, int Y, int Z, int T, int px, int py,
+                         int pz, int stride, int form, void* workspace, int64_t workspace_bytes,
+                         int32_t* rank, float* margin, int32_t* info, void* stream) {
+  Geom g;
+  int rc = make_geom(&g, X, Y, Z, T, px, py, pz, stride);
+  if (rc) return rc;
+  if (!valid_form(form)) return set_err(MPPCA_EINVAL, "unknown form %d", form);
+  if (!vol || !out || !workspace) return set_err(MPPCA_EINVAL, "denoise: null pointer");
+  if (vol == out) return set_err(MPPCA_EINVAL, "denoise: vol and out must be distinct");
+  if ((uintptr_t)workspace % 256 != 0)
+    return set_err(MPPCA_EINVAL, "denoise: workspace must be 256-byte aligned");
+  Workspace w = carve_workspace((char*)workspace, g, form);
+  if ((int64_t)w.bytes > workspace_bytes)
+    return set_err(MPPCA_ENOSPACE, "denoise: workspace %lld < %zu bytes",
+                   (long long)workspace_bytes, w.bytes);
+  hipStream_t st = (hipStream_t)stream;
+  int* rk = rank ? rank : w.rank;
+  float* mg = margin ? margin : w.margin;
+  int* inf = info ? info : w.info;
+  if ((rc = launch_centre(vol, w.xc, w.mean, g.nvox, T, st))) return rc;
+  if (form == MPPCA_FORM_PLAIN) {
+    if ((rc = launch_gram64(w.xc, (double*)w.gram, g, 0, g.P, st))) return rc;
+    if ((rc = launch_eig64_mp((const double*)w.gram, g.P, T, g.N, w.proj, rk, w.weight, mg,
+                              nullptr, inf, st)))
+      return rc;
+  } else {
+    if ((rc = launch_gram(w.xc, (float*)w.gram, g, 0, g.P, st))) return rc;
+    if ((rc = launch_eigh_mp((const float*)w.gram, g.P, T, g.N, 20, w.proj, rk, w.weight, mg,
+                             nullptr, inf, st)))
+      return rc;
+  }
+  if ((rc = zero_fill(out, g.nvox * T, st))) return rc;
+  if ((rc = launch_recon(w.xc, w.proj, w.weight, out, g, 0, g.P, st))) return rc;
+  return launch_normalize(out, w.mean, w.weight, out, g, st);
+}
+
+}  // extern "C"

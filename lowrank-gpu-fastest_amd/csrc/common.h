@@ -1,0 +1,162 @@
+// common.h -- shared definitions of the MI355X (gfx950) MP-PCA kernels.
+//
+// Geometry follows the reference's corner rule (common.py:37-44): per axis
+// the strided corners 0, s, 2s, ... that fit plus one shifted last corner
+// flush with the border; patch b = (ix*ncy + iy)*ncz + iz.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdint.h>
+
+#include "mppca.h"
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+
+#define MPPCA_ABI_VERSION 101
+#define MAX_T 128
+#define MAX_N 4096
+
+namespace mppca {
+
+// ---- error state (abi.hip) --------------------------------------------------
+int set_err(int code, const char* fmt, ...);
+int launch_status(const char* what);
+int prepare_lds(const void* fn, size_t bytes);
+
+// ---- geometry ---------------------------------------------------------------
+struct Geom {
+  int X, Y, Z, T;
+  int px, py, pz, stride;
+  int nsx, nsy, nsz;  // strided corners per axis
+  int ncx, ncy, ncz;  // all corners per axis (strided + shifted last)
+  int N;              // voxels per patch
+  long long P;        // patches
+  long long nvox;     // X*Y*Z
+};
+
+int make_geom(Geom* g, int X, int Y, int Z, int T, int px, int py, int pz, int stride);
+
+inline int ntb_of(int T) { return (T + 15) / 16; }
+
+__device__ __forceinline__ int corner_of(int i, int ns, int S, int p, int s) {
+  return i < ns ? i * s : S - p;
+}
+
+// Voxel index of the first voxel of patch b.
+__device__ __forceinline__ long long patch_base(const Geom& g, long long b) {
+  const int iz = (int)(b % g.ncz);
+  const long long r = b / g.ncz;
+  const int iy = (int)(r % g.ncy);
+  const int ix = (int)(r / g.ncy);
+  const int cx = corner_of(ix, g.nsx, g.X, g.px, g.stride);
+  const int cy = corner_of(iy, g.nsy, g.Y, g.py, g.stride);
+  const int cz = corner_of(iz, g.nsz, g.Z, g.pz, g.stride);
+  return ((long long)cx * g.Y + cy) * g.Z + cz;
+}
+
+// Offsets (in voxels) of the N voxels of a patch from its first voxel,
+// x-major order (step2_torch_batched.py:24-27): n = (ox*py + oy)*pz + oz.
+__device__ __forceinline__ void fill_offsets(int* off, const Geom& g) {
+  for (int n = threadIdx.x; n < g.N; n += blockDim.x) {
+    const int oz = n % g.pz;
+    const int r = n / g.pz;
+    const int oy = r % g.py;
+    const int ox = r / g.py;
+    off[n] = (ox * g.Y + oy) * g.Z + oz;
+  }
+}
+
+// Corner indices covering coordinate x on one axis: strided [lo, hi] plus the
+// shifted last corner (index ns) when `extra`.
+__device__ __forceinline__ void cover_range(int x, int S, int p, int s, int ns, int nc, int* lo,
+                                            int* hi, bool* extra) {
+  const int a = x - p + 1;
+  *lo = a <= 0 ? 0 : (a + s - 1) / s;
+  const int h = x / s;
+  *hi = h > ns - 1 ? ns - 1 : h;
+  *extra = (nc > ns) && (x >= S - p);
+}
+
+// ---- MP rule (step0_numpy_loop.py:18-24) ---------------------------------------
+// Error model of the eigenvalues fed to the rule, for the margin output:
+//   |d lambda_j| <= u32 * (abs_k * lambda_max + rel_k * sqrt(lambda_j * lambda_max))
+// (u32 = 2^-24). The FP32 Gram route carries an absolute u*lambda_max term; the
+// FP64 route is limited by step 0's own float32 SVD, whose squared singular
+// values carry ~2u * s_j * s_max.
+struct MpErr {
+  double abs_k, rel_k;
+};
+
+// MP rank of descending eigenvalues lam[0..T-1] of an N-voxel patch, in
+// step 0's arithmetic: e_i = fl32(max(lam_i, 0) / n), rcum = float32 suffix
+// sums, the two sides of the inequality in float64; M = min(N, T-1) values,
+// n = max(N, T-1) (the transposed convention when N < T-1). Single lane.
+// margin: min over i of |lhs_i - rhs_i| / (error bound of lhs_i - rhs_i).
+__device__ inline void mp_rule(const float* lam, float* e, float* rc, int T, int N, MpErr err,
+                               int* p_out, float* margin_out) {
+  const int M = N < T - 1 ? N : T - 1;
+  const int n = N > T - 1 ? N : T - 1;
+  const float nf = (float)n;
+  for (int i = 0; i < M; ++i) e[i] = fmaxf(lam[i], 0.f) / nf;
+  float run = e[M - 1];
+  rc[M - 1] = run;
+  for (int i = M - 2; i >= 0; --i) {
+    run = run + e[i];
+    rc[i] = run;
+  }
+  const double u = 5.9604644775390625e-08;  // 2^-24
+  const double sq = sqrt((double)M * (double)n);
+  const double emax = (double)e[0];
+  // error bound of each e_j, and its suffix sums (for rcum), accumulated on the fly
+  double de_last = u * (err.abs_k * emax + err.rel_k * sqrt((double)e[M - 1] * emax));
+  double de_suffix = 0.0;
+  for (int i = M - 1; i >= 0; --i)
+    de_suffix += u * (err.abs_k * emax + err.rel_k * sqrt((double)e[i] * emax));
+  int p = 0;
+  double mg = INFINITY;
+  for (int i = 0; i < M; ++i) {
+    const float d = e[i] - e[M - 1];
+    const double lhs = (double)d * (double)(M - i) * (double)(n - i);
+    const double rhs = (double)(4.f * rc[i]) * sq;
+    p += lhs > rhs ? 1 : 0;
+    const double de_i = u * (err.abs_k * emax + err.rel_k * sqrt((double)e[i] * emax));
+    const double eb = (de_i + de_last) * (double)(M - i) * (double)(n - i) + 4.0 * de_suffix * sq;
+    const double diff = fabs(lhs - rhs);
+    const double r = eb > 0.0 ? diff / eb : INFINITY;
+    mg = r < mg ? r : mg;
+    de_suffix -= de_i;
+    if (de_suffix < 0.0) de_suffix = 0.0;
+  }
+  *p_out = p;
+  *margin_out = (float)mg;
+}
+
+// Error models of the two routes (see MpErr).
+constexpr MpErr kErrFp32Route = {16.0, 0.0};
+constexpr MpErr kErrFp64Route = {1.0, 32.0};
+
+// ---- launchers (defined per stage file) ----------------------------------------
+int zero_fill(float* p, long long n, hipStream_t st);
+int launch_centre(const float* vol, float* xc, float* mean, long long nvox, int T, hipStream_t st);
+int launch_normalize(const float* acc, const float* mean, const float* weight, float* out,
+                     const Geom& g, hipStream_t st);
+int launch_gram(const float* xc, float* gram, const Geom& g, long long b0, long long nb,
+                hipStream_t st);
+int launch_gram64(const float* xc, double* gram, const Geom& g, long long b0, long long nb,
+                  hipStream_t st);
+int launch_eigh_jacobi(const float* gram, float* lam, float* vec, int* info, long long nb, int T,
+                       int max_sweeps, hipStream_t st);
+int launch_eigh_mp(const float* gram, long long nb, int T, int N, int max_sweeps, float* proj,
+                   int* rank, float* weight, float* margin, float* lam_desc, int* info,
+                   hipStream_t st);
+int launch_mp(const float* lam, long long nb, int T, int N, int* rank, float* weight,
+              float* margin, hipStream_t st);
+int launch_eig64_mp(const double* gram, long long nb, int T, int N, float* proj, int* rank,
+                    float* weight, float* margin, float* lam_desc, int* info, hipStream_t st);
+int launch_recon(const float* xc, const float* proj, const float* weight, float* acc,
+                 const Geom& g, long long b0, long long nb, hipStream_t st);
+
+}  // namespace mppca

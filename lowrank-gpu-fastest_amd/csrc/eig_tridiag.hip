@@ -1,0 +1,395 @@
+// eig_tridiag.hip -- FP64 batched symmetric eigensolver fused with the MP
+// rank and the rank-p projector; the eigensolver of the "plain" form.
+// Replaces torch.linalg.eigh + mp_keep + the projector bmm
+// (step8_tuned.py:134-137). One wavefront per T x T matrix, all in LDS.
+//
+// 1. Householder tridiagonalisation, A = Q T Q^T (Golub & Van Loan 8.3.1;
+//    reflector H = I - tau v v^T, v_0 = 1, as LAPACK's dlarfg defines it).
+//    Lane i owns row k+1+i of the trailing block: p = tau A22 v,
+//    w = p - (tau/2)(p.v) v, A22 -= v w^T + w v^T.
+// 2. Every eigenvalue by bisection on Sturm counts of the LDL^T pivots
+//    (q_i = d_i - x - e_{i-1}^2 / q_{i-1}; count of q_i < 0), lane j owns the
+//    j-th largest, to ~4 ulps of ||T||.
+// 3. MP rank p from the eigenvalues (step 0's arithmetic, common.h).
+// 4. Eigenvectors of the top p only (U and the noise vectors are never
+//    formed): twisted factorisation of T - lambda I (Parlett & Dhillon 2000):
+//    forward LDL^T pivots D+, backward UDU^T pivots D-, twist r at
+//    min |D+_r + D-_r - (d_r - lambda)|, z_r = 1 and the two one-term
+//    recurrences outward; modified Gram-Schmidt inside each group of vectors
+//    guards near-equal eigenvalues.
+// 5. Back-transform x = H_0 H_1 ... H_{T-3} z and P_b = sum_j x_j x_j^T,
+//    stored as float32 for the FP32 reconstruction.
+#include "common.h"
+
+namespace mppca {
+
+// Eigenvectors computed concurrently (one lane each): 8, or 4 above T = 96 so
+// that the whole working set of a T = 128 matrix stays within 160 KB of LDS.
+__host__ __device__ inline int group_of(int T) { return T > 96 ? 4 : 8; }
+
+struct TriLds {
+  double* A;    // T x LDA; after step 1 column k below the diagonal holds v_k
+  double* d;    // T
+  double* e;    // T: e[i] couples i and i+1; e[T-1] = 0
+  double* e2;   // T: e[i]^2
+  double* tau;  // T
+  double* v;    // T: current reflector
+  double* w;    // T: current w
+  double* lam;  // T: eigenvalues, descending
+  double* dp;   // G x LDZ
+  double* dm;   // G x LDZ
+  double* z;    // G x LDZ
+  float* lamf;  // T
+  float* ef;    // T
+  float* rcf;   // T
+  int LDA, LDZ;
+};
+
+static size_t tri_lds_bytes(int T) {
+  const size_t LDA = T + 1, LDZ = T + 1;
+  const size_t nd = (size_t)T * LDA + 8 * (size_t)T + 3 * (size_t)group_of(T) * LDZ;
+  return nd * sizeof(double) + 3 * (size_t)T * sizeof(float);
+}
+
+__device__ inline TriLds tri_carve(double* sm, int T) {
+  TriLds L;
+  L.LDA = T + 1;
+  L.LDZ = T + 1;
+  double* p = sm;
+  L.A = p;
+  p += T * L.LDA;
+  L.d = p;
+  p += T;
+  L.e = p;
+  p += T;
+  L.e2 = p;
+  p += T;
+  L.tau = p;
+  p += T;
+  L.v = p;
+  p += T;
+  L.w = p;
+  p += T;
+  L.lam = p;
+  p += T;
+  p += T;  // spare (keeps the float tail 16-byte aligned for any T)
+  const int G = group_of(T);
+  L.dp = p;
+  p += G * L.LDZ;
+  L.dm = p;
+  p += G * L.LDZ;
+  L.z = p;
+  p += G * L.LDZ;
+  L.lamf = (float*)p;
+  L.ef = L.lamf + T;
+  L.rcf = L.ef + T;
+  return L;
+}
+
+__device__ __forceinline__ double wave_sum(double x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+  return x;
+}
+__device__ __forceinline__ double wave_min(double x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x = fmin(x, __shfl_xor(x, o));
+  return x;
+}
+__device__ __forceinline__ double wave_max(double x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x = fmax(x, __shfl_xor(x, o));
+  return x;
+}
+
+// Number of eigenvalues of the tridiagonal (d, e) smaller than x.
+__device__ __forceinline__ int sturm_count(const double* d, const double* e2, int T, double x,
+                                           double pivmin) {
+  double q = d[0] - x;
+  int c = q < 0.0 ? 1 : 0;
+  for (int i = 1; i < T; ++i) {
+    if (fabs(q) < pivmin) q = -pivmin;
+    q = (d[i] - x) - e2[i - 1] / q;
+    c += q < 0.0 ? 1 : 0;
+  }
+  return c;
+}
+
+// Returns false when the matrix holds a non-finite value.
+__device__ bool tridiagonalize(const TriLds& L, int T, int lane) {
+  double* A = L.A;
+  const int LDA = L.LDA;
+  bool finite = true;
+  for (int k = 0; k + 2 < T; ++k) {
+    const int len = T - k - 1;
+    const int r0 = k + 1 + lane, r1 = r0 + 64;
+    const bool h0 = lane < len, h1 = lane + 64 < len;
+    const double x0 = h0 ? A[r0 * LDA + k] : 0.0;
+    const double x1 = h1 ? A[r1 * LDA + k] : 0.0;
+    const double alpha = A[(k + 1) * LDA + k];
+    const double ss = wave_sum((lane >= 1 ? x0 * x0 : 0.0) + x1 * x1);
+    if (!isfinite(ss) || !isfinite(alpha)) finite = false;
+    double tk = 0.0, beta = alpha;
+    if (ss > 0.0 && finite) {
+      const double mu = sqrt(alpha * alpha + ss);
+      beta = alpha >= 0.0 ? -mu : mu;
+      tk = (beta - alpha) / beta;
+      const double sc = 1.0 / (alpha - beta);
+      if (h0) L.v[lane] = lane == 0 ? 1.0 : x0 * sc;
+      if (h1) L.v[lane + 64] = x1 * sc;
+    } else {
+      if (h0) L.v[lane] = lane == 0 ? 1.0 : 0.0;
+      if (h1) L.v[lane + 64] = 0.0;
+    }
+    if (lane == 0) {
+      L.d[k] = A[k * LDA + k];
+      L.e[k] = beta;
+      L.tau[k] = tk;
+    }
+    __syncthreads();
+    // keep v_k in column k for the back-transform
+    if (h0) A[r0 * LDA + k] = L.v[lane];
+    if (h1) A[r1 * LDA + k] = L.v[lane + 64];
+    if (tk != 0.0) {
+      const double* vv = L.v;
+      double p0 = 0.0, p1 = 0.0;
+      if (h0) {
+        const double* row = A + r0 * LDA + k + 1;
+        for (int j = 0; j < len; ++j) p0 += row[j] * vv[j];
+        p0 *= tk;
+      }
+      if (h1) {
+        const double* row = A + r1 * LDA + k + 1;
+        for (int j = 0; j < len; ++j) p1 += row[j] * vv[j];
+        p1 *= tk;
+      }
+      const double v0 = h0 ? vv[lane] : 0.0, v1 = h1 ? vv[lane + 64] : 0.0;
+      const double K = 0.5 * tk * wave_sum(p0 * v0 + p1 * v1);
+      const double w0 = p0 - K * v0, w1 = p1 - K * v1;
+      if (h0) L.w[lane] = w0;
+      if (h1) L.w[lane + 64] = w1;
+      __syncthreads();
+      const double* ww = L.w;
+      if (h0) {
+        double* row = A + r0 * LDA + k + 1;
+        for (int j = 0; j < len; ++j) row[j] -= v0 * ww[j] + w0 * vv[j];
+      }
+      if (h1) {
+        double* row = A + r1 * LDA + k + 1;
+        for (int j = 0; j < len; ++j) row[j] -= v1 * ww[j] + w1 * vv[j];
+      }
+    }
+    __syncthreads();
+  }
+  if (lane == 0) {
+    if (T >= 2) {
+      L.d[T - 2] = A[(T - 2) * LDA + T - 2];
+      L.e[T - 2] = A[(T - 1) * LDA + T - 2];
+    }
+    L.d[T - 1] = A[(T - 1) * LDA + T - 1];
+    L.e[T - 1] = 0.0;
+  }
+  __syncthreads();
+  for (int i = lane; i < T; i += 64) {
+    L.e2[i] = L.e[i] * L.e[i];
+    if (!isfinite(L.d[i]) || !isfinite(L.e[i])) finite = false;
+  }
+  __syncthreads();
+  return __any(!finite) == 0;
+}
+
+// All eigenvalues, descending, into L.lam. Returns ||T||-scale bound.
+__device__ double bisect_all(const TriLds& L, int T, int lane) {
+  double lo_g = INFINITY, hi_g = -INFINITY, emax2 = 0.0;
+  for (int i = lane; i < T; i += 64) {
+    const double r = (i > 0 ? fabs(L.e[i - 1]) : 0.0) + fabs(L.e[i]);
+    lo_g = fmin(lo_g, L.d[i] - r);
+    hi_g = fmax(hi_g, L.d[i] + r);
+    emax2 = fmax(emax2, L.e2[i]);
+  }
+  lo_g = wave_min(lo_g);
+  hi_g = wave_max(hi_g);
+  emax2 = wave_max(emax2);
+  const double u = 1.1102230246251565e-16;  // 2^-53
+  const double bnorm = fmax(fabs(lo_g), fabs(hi_g));
+  const double pivmin = 2.2250738585072014e-308 * fmax(1.0, emax2);
+  const double slack = 2.0 * u * bnorm * T + pivmin;
+  lo_g -= slack;
+  hi_g += slack;
+  const double tol = 4.0 * u * bnorm + pivmin;
+  for (int j = lane; j < T; j += 64) {
+    const int k = T - 1 - j;  // k-th smallest
+    double lo = lo_g, hi = hi_g;
+    for (int it = 0; it < 128 && hi - lo > tol; ++it) {
+      const double mid = 0.5 * (lo + hi);
+      if (sturm_count(L.d, L.e2, T, mid, pivmin) > k)
+        hi = mid;
+      else
+        lo = mid;
+    }
+    L.lam[j] = 0.5 * (lo + hi);
+  }
+  __syncthreads();
+  return bnorm;
+}
+
+// Eigenvector of the tridiagonal for eigenvalue lambda (twisted factorisation)
+// into z[0..T-1], normalised; dp / dm are this lane's scratch rows.
+__device__ void twisted_vector(const TriLds& L, int T, double lambda, double pivmin, double* dp,
+                               double* dm, double* z) {
+  const double* d = L.d;
+  const double* e = L.e;
+  dp[0] = d[0] - lambda;
+  for (int i = 0; i + 1 < T; ++i) {
+    double q = dp[i];
+    if (fabs(q) < pivmin) q = q < 0.0 ? -pivmin : pivmin;
+    dp[i] = q;
+    dp[i + 1] = (d[i + 1] - lambda) - e[i] * e[i] / q;
+  }
+  dm[T - 1] = d[T - 1] - lambda;
+  for (int i = T - 2; i >= 0; --i) {
+    double q = dm[i + 1];
+    if (fabs(q) < pivmin) q = q < 0.0 ? -pivmin : pivmin;
+    dm[i + 1] = q;
+    dm[i] = (d[i] - lambda) - e[i] * e[i] / q;
+  }
+  int r = 0;
+  double best = INFINITY;
+  for (int i = 0; i < T; ++i) {
+    const double g = fabs(dp[i] + dm[i] - (d[i] - lambda));
+    if (g < best) {
+      best = g;
+      r = i;
+    }
+  }
+  z[r] = 1.0;
+  double nrm = 1.0;
+  for (int i = r - 1; i >= 0; --i) {
+    const double zi = -(e[i] / dp[i]) * z[i + 1];
+    z[i] = zi;
+    nrm += zi * zi;
+  }
+  for (int i = r; i + 1 < T; ++i) {
+    const double zi = -(e[i] / dm[i + 1]) * z[i];
+    z[i + 1] = zi;
+    nrm += zi * zi;
+  }
+  const double s = 1.0 / sqrt(nrm);
+  for (int i = 0; i < T; ++i) z[i] *= s;
+}
+
+__global__ __launch_bounds__(64) void k_eig64_mp(const double* __restrict__ gram, int T, int N,
+                                                 long long nb, float* __restrict__ proj,
+                                                 int* __restrict__ rank,
+                                                 float* __restrict__ weight,
+                                                 float* __restrict__ margin,
+                                                 float* __restrict__ lam_desc,
+                                                 int* __restrict__ info) {
+  extern __shared__ double smd[];
+  __shared__ int s_p;
+  const long long b = blockIdx.x;
+  if (b >= nb) return;
+  const int lane = threadIdx.x;
+  TriLds L = tri_carve(smd, T);
+  const double* Gb = gram + b * (long long)T * T;
+  for (int idx = lane; idx < T * T; idx += 64) {
+    const int r = idx / T, c = idx - r * T;
+    L.A[r * L.LDA + c] = Gb[idx];
+  }
+  __syncthreads();
+  const bool finite = tridiagonalize(L, T, lane);
+  float* Pb = proj + b * (long long)T * T;
+  if (!finite) {  // non-finite input: no rank, no projector, flagged
+    for (int idx = lane; idx < T * T; idx += 64) Pb[idx] = 0.f;
+    if (lane == 0) {
+      rank[b] = 0;
+      weight[b] = 0.5f;
+      if (margin) margin[b] = 0.f;
+      if (info) info[b] = -2;
+    }
+    if (lam_desc)
+      for (int c = lane; c < T; c += 64) lam_desc[b * T + c] = NAN;
+    return;
+  }
+  const double bnorm = bisect_all(L, T, lane);
+  for (int c = lane; c < T; c += 64) {
+    L.lamf[c] = (float)L.lam[c];
+    if (lam_desc) lam_desc[b * T + c] = L.lamf[c];
+  }
+  __syncthreads();
+  if (lane == 0) {
+    int p;
+    float mg;
+    mp_rule(L.lamf, L.ef, L.rcf, T, N, kErrFp64Route, &p, &mg);
+    s_p = p;
+    rank[b] = p;
+    weight[b] = (float)(1.0 / (2.0 + (double)p));
+    if (margin) margin[b] = mg;
+    if (info) info[b] = 0;
+  }
+  __syncthreads();
+  const int p = s_p;
+  // pivot floor of the twisted factorisation: a backward perturbation of
+  // u * ||T||, which bounds the growth of the one-term recurrences
+  const double pivmin = fmax(1.1102230246251565e-16 * bnorm, 2.2250738585072014e-308);
+  const int LDZ = L.LDZ;
+  const int G = group_of(T);
+  for (int g0 = 0; g0 < p || g0 == 0; g0 += G) {
+    const int gn = p - g0 < G ? p - g0 : G;
+    if (lane < gn) {
+      double* z = L.z + lane * LDZ;
+      twisted_vector(L, T, L.lam[g0 + lane], pivmin, L.dp + lane * LDZ, L.dm + lane * LDZ, z);
+    }
+    __syncthreads();
+    // modified Gram-Schmidt inside the group (all lanes over components)
+    for (int a = 1; a < gn; ++a) {
+      for (int c = 0; c < a; ++c) {
+        double part = 0.0;
+        for (int i = lane; i < T; i += 64) part += L.z[a * LDZ + i] * L.z[c * LDZ + i];
+        const double dot = wave_sum(part);
+        for (int i = lane; i < T; i += 64) L.z[a * LDZ + i] -= dot * L.z[c * LDZ + i];
+        __syncthreads();
+      }
+      double part = 0.0;
+      for (int i = lane; i < T; i += 64) part += L.z[a * LDZ + i] * L.z[a * LDZ + i];
+      const double s = 1.0 / sqrt(wave_sum(part));
+      for (int i = lane; i < T; i += 64) L.z[a * LDZ + i] *= s;
+      __syncthreads();
+    }
+    // back-transform: x = H_0 (H_1 (... H_{T-3} z)), one lane per vector
+    if (lane < gn) {
+      double* x = L.z + lane * LDZ;
+      for (int k = T - 3; k >= 0; --k) {
+        const double tk = L.tau[k];
+        if (tk == 0.0) continue;
+        const double* vk = L.A + k;  // v_k[i] = A[(k+1+i)*LDA + k]
+        double s = 0.0;
+        for (int i = 0; i < T - k - 1; ++i) s += vk[(k + 1 + i) * L.LDA] * x[k + 1 + i];
+        s *= tk;
+        for (int i = 0; i < T - k - 1; ++i) x[k + 1 + i] -= s * vk[(k + 1 + i) * L.LDA];
+      }
+    }
+    __syncthreads();
+    // P += X X^T over this group
+    for (int idx = lane; idx < T * T; idx += 64) {
+      const int r = idx / T, c = idx - r * T;
+      double s = 0.0;
+      for (int g = 0; g < gn; ++g) s += L.z[g * LDZ + r] * L.z[g * LDZ + c];
+      Pb[idx] = g0 == 0 ? (float)s : (float)((double)Pb[idx] + s);
+    }
+    __syncthreads();
+    if (p == 0) break;
+  }
+}
+
+int launch_eig64_mp(const double* gram, long long nb, int T, int N, float* proj, int* rank,
+                    float* weight, float* margin, float* lam_desc, int* info, hipStream_t st) {
+  const size_t lds = tri_lds_bytes(T);
+  int rc = prepare_lds((const void*)k_eig64_mp, lds);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_eig64_mp, dim3((unsigned)nb), dim3(64), lds, st, gram, T, N, nb, proj,
+                     rank, weight, margin, lam_desc, info);
+  return launch_status("eig64+mp kernel");
+}
+
+}  // namespace mppca

@@ -1,0 +1,135 @@
+// gram.hip -- fused gather + Gram: G_b = Xc_b^T Xc_b straight from the
+// centred volume, on MFMA. Replaces gram_kernel (step8_tuned.py:32-56).
+//
+// One wavefront per patch, 4 patches per 256-thread block. MFMA 16x16x4
+// operand layout (f32 and f64 alike): lane l supplies A[l%16][l/16] and
+// B[l/16][l%16]; the 16x16 result sits in 4 registers, D[4*(l/16)+r][l%16].
+// For G = Xc^T Xc both operands are the same 4-voxel x 16-frame slice, so a
+// lane loads Xc[n0 + l/16][16*tb + l%16] once per frame block tb and feeds it
+// to every (tb1 <= tb2) product; only the upper block triangle is formed and
+// mirrored on the store.
+//
+// S = float : v_mfma_f32_16x16x4f32, FP32 accumulation (the reference's
+//             arithmetic class).
+// S = double: v_mfma_f64_16x16x4f64 on exact float->double operands: every
+//             product is exact and sums carry 53 bits, so G matches the exact
+//             Gram of the float32 centred patch to ~1e-16 relative.
+#include "common.h"
+
+namespace mppca {
+
+__device__ __forceinline__ floatx4 mfma_16x16x4(float a, float b, floatx4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ doublex4 mfma_16x16x4(double a, double b, doublex4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+template <typename S>
+struct Vec4;
+template <>
+struct Vec4<float> {
+  typedef floatx4 type;
+};
+template <>
+struct Vec4<double> {
+  typedef doublex4 type;
+};
+
+template <int NTB, typename S>
+__global__ __launch_bounds__(256) void k_gram(const float* __restrict__ xc, S* __restrict__ gram,
+                                              Geom g, long long b0, long long nb) {
+  typedef typename Vec4<S>::type V4;
+  extern __shared__ int s_off[];
+  fill_offsets(s_off, g);
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const long long bl = (long long)blockIdx.x * 4 + wave;
+  if (bl >= nb) return;
+  const int T = g.T, N = g.N;
+  const int i = lane & 15, kq = lane >> 4;
+  const float* xb = xc + patch_base(g, b0 + bl) * T;
+  constexpr int NACC = NTB * (NTB + 1) / 2;
+  V4 acc[NACC];
+#pragma unroll
+  for (int k = 0; k < NACC; ++k) acc[k] = V4{0, 0, 0, 0};
+
+  float a[NTB], an[NTB];
+  auto load = [&](int n, float* dst) {
+    if (n < N) {
+      const float* row = xb + (long long)s_off[n] * T;
+#pragma unroll
+      for (int tb = 0; tb < NTB; ++tb) {
+        const int t = tb * 16 + i;
+        dst[tb] = t < T ? row[t] : 0.f;
+      }
+    } else {
+#pragma unroll
+      for (int tb = 0; tb < NTB; ++tb) dst[tb] = 0.f;
+    }
+  };
+  load(kq, a);
+  for (int n0 = 0; n0 < N; n0 += 4) {
+    load(n0 + 4 + kq, an);  // next slice in flight while this one multiplies
+    int k = 0;
+#pragma unroll
+    for (int tb1 = 0; tb1 < NTB; ++tb1)
+#pragma unroll
+      for (int tb2 = tb1; tb2 < NTB; ++tb2, ++k)
+        acc[k] = mfma_16x16x4((S)a[tb1], (S)a[tb2], acc[k]);
+#pragma unroll
+    for (int tb = 0; tb < NTB; ++tb) a[tb] = an[tb];
+  }
+
+  S* Gb = gram + bl * (long long)T * T;
+  int k = 0;
+#pragma unroll
+  for (int tb1 = 0; tb1 < NTB; ++tb1)
+#pragma unroll
+    for (int tb2 = tb1; tb2 < NTB; ++tb2, ++k)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int ti = tb1 * 16 + 4 * kq + r, tj = tb2 * 16 + i;
+        if (ti < T && tj < T) {
+          const S v = acc[k][r];
+          Gb[ti * T + tj] = v;
+          if (tb1 != tb2) Gb[tj * T + ti] = v;
+        }
+      }
+}
+
+template <int NTB, typename S>
+static int launch_gram_t(const float* xc, S* gram, const Geom& g, long long b0, long long nb,
+                         hipStream_t st) {
+  const unsigned blocks = (unsigned)((nb + 3) / 4);
+  hipLaunchKernelGGL((k_gram<NTB, S>), dim3(blocks), dim3(256), g.N * sizeof(int), st, xc, gram,
+                     g, b0, nb);
+  return launch_status("gram kernel");
+}
+
+template <typename S>
+static int launch_gram_any(const float* xc, S* gram, const Geom& g, long long b0, long long nb,
+                           hipStream_t st) {
+  switch (ntb_of(g.T)) {
+    case 1: return launch_gram_t<1, S>(xc, gram, g, b0, nb, st);
+    case 2: return launch_gram_t<2, S>(xc, gram, g, b0, nb, st);
+    case 3: return launch_gram_t<3, S>(xc, gram, g, b0, nb, st);
+    case 4: return launch_gram_t<4, S>(xc, gram, g, b0, nb, st);
+    case 5: return launch_gram_t<5, S>(xc, gram, g, b0, nb, st);
+    case 6: return launch_gram_t<6, S>(xc, gram, g, b0, nb, st);
+    case 7: return launch_gram_t<7, S>(xc, gram, g, b0, nb, st);
+    default: return launch_gram_t<8, S>(xc, gram, g, b0, nb, st);
+  }
+}
+
+int launch_gram(const float* xc, float* gram, const Geom& g, long long b0, long long nb,
+                hipStream_t st) {
+  return launch_gram_any<float>(xc, gram, g, b0, nb, st);
+}
+
+int launch_gram64(const float* xc, double* gram, const Geom& g, long long b0, long long nb,
+                  hipStream_t st) {
+  return launch_gram_any<double>(xc, gram, g, b0, nb, st);
+}
+
+}  // namespace mppca

@@ -1,0 +1,228 @@
+"""Host driver of the HIP denoiser: the reference's ``denoise`` entry point.
+
+``denoise(vol, patch, stride)`` takes a float32 (X, Y, Z, T) volume and
+returns the denoised volume, like the reference's ``stepN.denoise(vol_np)``
+(step8_tuned.py:116-146, step0_numpy_loop.py:28-39) with the patch shape and
+stride as arguments instead of module globals (common.py:27-29).
+
+* NumPy (or CPU tensor) in -> NumPy out: host -> host, the timed form of the
+  reference's harness (common.py:129-143).
+* CUDA tensor in -> CUDA tensor out: device -> device, no host copies.
+
+All device work is enqueued on torch's current stream through the C ABI of
+``libmppca.so``; device buffers are torch tensors held by a per-geometry
+:class:`Denoiser` and reused across calls.
+"""
+
+from __future__ import annotations
+
+import threading
+
+import numpy as np
+
+from . import _lib
+from .geometry import check_geometry, num_patches
+
+# "plain": FP64-accumulated MFMA Gram + FP64 eigensolver, FP32 reconstruction
+#          (matches step 0 to ~1e-7 on any input); the default.
+# "fp32":  the reference's all-FP32 arithmetic (FP32 MFMA Gram, FP32 Jacobi).
+FORMS = ("plain", "fp32")
+
+
+class Denoiser:
+    """Device-resident state for one (shape, patch, stride): workspace, buffers.
+
+    Not thread-safe: one Denoiser serves one host thread at a time (the
+    module-level :func:`denoise` keeps one per thread and geometry).
+    """
+
+    def __init__(self, shape, patch=(11, 11, 11), stride=3, device=None, form="plain"):
+        import torch
+
+        if form not in FORMS:
+            raise ValueError(f"unknown form {form!r}; available: {FORMS}")
+        self.torch = torch
+        self.shape = tuple(int(s) for s in shape)
+        self.patch = tuple(int(p) for p in patch)
+        self.stride = int(stride)
+        self.form = form
+        check_geometry(self.shape, self.patch, self.stride)
+        if not torch.cuda.is_available():
+            raise RuntimeError("no GPU visible: the HIP denoiser has no CPU fallback")
+        self.device = torch.device(device if device is not None else "cuda")
+        self.lib = _lib.load()
+        self.n_patches = num_patches(self.shape, self.patch, self.stride)
+        self.form_code = _lib.FORM_CODES[form]
+        ws = int(self.lib.mppca_workspace_bytes(*self._geom_args(), self.form_code))
+        if ws < 0:
+            _lib.check(ws, "mppca_workspace_bytes")
+        dev = self.device
+        self._ws = torch.empty(ws + 256, dtype=torch.uint8, device=dev)
+        self._ws_ptr = (self._ws.data_ptr() + 255) // 256 * 256
+        self._ws_bytes = ws
+        self.vol_d = torch.empty(self.shape, dtype=torch.float32, device=dev)
+        self.out_d = torch.empty(self.shape, dtype=torch.float32, device=dev)
+        self.rank = torch.empty(self.n_patches, dtype=torch.int32, device=dev)
+        self.margin = torch.empty(self.n_patches, dtype=torch.float32, device=dev)
+        self.sweeps = torch.empty(self.n_patches, dtype=torch.int32, device=dev)
+        self._pin_in = None
+
+    def _geom_args(self):
+        x, y, z, t = self.shape
+        px, py, pz = self.patch
+        return (x, y, z, t, px, py, pz, self.stride)
+
+    # -- device -> device ---------------------------------------------------
+    def run_device(self, vol_d, out_d, stream=None):
+        """Enqueue the whole path on ``stream`` (default: torch's current)."""
+        torch = self.torch
+        if vol_d.dtype != torch.float32 or tuple(vol_d.shape) != self.shape:
+            raise ValueError(f"expected float32 {self.shape}, got {vol_d.dtype} {tuple(vol_d.shape)}")
+        if not vol_d.is_contiguous() or not out_d.is_contiguous():
+            raise ValueError("device volumes must be contiguous")
+        if vol_d.data_ptr() == out_d.data_ptr():
+            raise ValueError("input and output device buffers must be distinct")
+        if stream is None:
+            stream = torch.cuda.current_stream(self.device)
+        rc = self.lib.mppca_denoise_device(
+            vol_d.data_ptr(), out_d.data_ptr(), *self._geom_args(), self.form_code,
+            self._ws_ptr, self._ws_bytes,
+            self.rank.data_ptr(), self.margin.data_ptr(), self.sweeps.data_ptr(),
+            stream.cuda_stream,
+        )
+        _lib.check(rc, "mppca_denoise_device")
+        return out_d
+
+    # -- device -> device, one C-ABI call per stage ---------------------------
+    STAGES = ("centre", "gram", "eigh_mp", "zero", "recon", "normalize")
+
+    def run_staged(self, vol_d, out_d, mark=None, max_sweeps=20):
+        """Same kernels as :meth:`run_device`, launched stage by stage.
+
+        ``mark(name)`` (optional) is called before each stage and once more
+        with ``None`` after the last, so a caller can record events between
+        stages on the current stream. Intermediates stay available in
+        ``self.stage_buffers`` for inspection.
+        """
+        torch = self.torch
+        x, y, z, t = self.shape
+        nvox = x * y * z
+        P = self.n_patches
+        if not hasattr(self, "stage_buffers"):
+            dev = self.device
+            gdt = torch.float64 if self.form == "plain" else torch.float32
+            self.stage_buffers = {
+                "xc": torch.empty(self.shape, dtype=torch.float32, device=dev),
+                "mean": torch.empty(self.shape[:3], dtype=torch.float32, device=dev),
+                "gram": torch.empty((P, t, t), dtype=gdt, device=dev),
+                "proj": torch.empty((P, t, t), dtype=torch.float32, device=dev),
+                "weight": torch.empty(P, dtype=torch.float32, device=dev),
+            }
+        sb = self.stage_buffers
+        st = torch.cuda.current_stream(self.device).cuda_stream
+        L, chk, g = self.lib, _lib.check, self._geom_args()
+        n_vox = int(np.prod(self.patch))
+        mark = mark or (lambda name: None)
+        mark("centre")
+        chk(L.mppca_centre(vol_d.data_ptr(), sb["xc"].data_ptr(), sb["mean"].data_ptr(),
+                           nvox, t, st), "mppca_centre")
+        mark("gram")
+        if self.form == "plain":
+            chk(L.mppca_gram64(sb["xc"].data_ptr(), sb["gram"].data_ptr(), *g, 0, P, st),
+                "mppca_gram64")
+            mark("eigh_mp")
+            chk(L.mppca_eig64_mp(sb["gram"].data_ptr(), P, t, n_vox, sb["proj"].data_ptr(),
+                                 self.rank.data_ptr(), sb["weight"].data_ptr(),
+                                 self.margin.data_ptr(), None, self.sweeps.data_ptr(), st),
+                "mppca_eig64_mp")
+        else:
+            chk(L.mppca_gram(sb["xc"].data_ptr(), sb["gram"].data_ptr(), *g, 0, P, st),
+                "mppca_gram")
+            mark("eigh_mp")
+            chk(L.mppca_eigh_mp(sb["gram"].data_ptr(), P, t, n_vox, max_sweeps,
+                                sb["proj"].data_ptr(), self.rank.data_ptr(),
+                                sb["weight"].data_ptr(), self.margin.data_ptr(), None,
+                                self.sweeps.data_ptr(), st), "mppca_eigh_mp")
+        mark("zero")
+        out_d.zero_()
+        mark("recon")
+        chk(L.mppca_recon_scatter(sb["xc"].data_ptr(), sb["proj"].data_ptr(),
+                                  sb["weight"].data_ptr(), out_d.data_ptr(), *g, 0, P, st),
+            "mppca_recon_scatter")
+        mark("normalize")
+        chk(L.mppca_normalize(out_d.data_ptr(), sb["mean"].data_ptr(), sb["weight"].data_ptr(),
+                              out_d.data_ptr(), *g, st), "mppca_normalize")
+        mark(None)
+        return out_d
+
+    # -- host -> host -------------------------------------------------------
+    def __call__(self, vol) -> np.ndarray:
+        torch = self.torch
+        a = vol.numpy() if isinstance(vol, torch.Tensor) else np.asarray(vol)
+        if a.dtype != np.float32:
+            raise TypeError(f"volume must be float32, got {a.dtype}")
+        if a.shape != self.shape:
+            raise ValueError(f"volume shape {a.shape} != {self.shape}")
+        if self._pin_in is None:
+            self._pin_in = torch.empty(self.shape, dtype=torch.float32, pin_memory=True)
+        np.copyto(self._pin_in.numpy(), a)
+        stream = torch.cuda.current_stream(self.device)
+        self.vol_d.copy_(self._pin_in, non_blocking=True)
+        self.run_device(self.vol_d, self.out_d, stream)
+        host = torch.empty(self.shape, dtype=torch.float32, pin_memory=True)
+        host.copy_(self.out_d, non_blocking=True)
+        stream.synchronize()
+        return host.numpy()
+
+    def info(self) -> dict:
+        """Per-patch rank, MP margin (units of the error bound) and solver status
+        ("sweeps": Jacobi sweeps in the fp32 form, 0 in the plain form; -1 no
+        convergence, -2 non-finite input)."""
+        self.torch.cuda.current_stream(self.device).synchronize()
+        return {
+            "rank": self.rank.cpu().numpy().astype(np.int64),
+            "margin": self.margin.cpu().numpy(),
+            "sweeps": self.sweeps.cpu().numpy(),
+        }
+
+
+_CACHE = threading.local()
+
+
+def get_denoiser(shape, patch=(11, 11, 11), stride=3, device=None, form="plain") -> Denoiser:
+    """The calling thread's cached :class:`Denoiser` for a geometry."""
+    if not hasattr(_CACHE, "d"):
+        _CACHE.d = {}
+    key = (tuple(int(s) for s in shape), tuple(int(p) for p in patch), int(stride), str(device), form)
+    d = _CACHE.d.get(key)
+    if d is None:
+        d = Denoiser(shape, patch, stride, device=device, form=form)
+        _CACHE.d[key] = d
+    return d
+
+
+def denoise(vol, patch=(11, 11, 11), stride=3, *, form="plain", return_info=False):
+    """MP-PCA denoise a float32 (X, Y, Z, T) volume on the GPU.
+
+    NumPy in -> NumPy out (host -> host); a CUDA tensor in -> a new CUDA
+    tensor out. ``form`` is one of :data:`FORMS`. ``return_info`` adds a dict
+    with the per-patch rank, MP margin and solver status. Raises on invalid
+    input; never falls back to a CPU path. A patch holding NaN or Inf gets
+    rank 0 and status -2; the NaN reaches the output, as in any float path.
+    """
+    try:
+        import torch
+    except ImportError as e:  # pragma: no cover
+        raise ImportError("PyTorch (ROCm) is required by the HIP denoiser") from e
+    if isinstance(vol, torch.Tensor) and vol.is_cuda:
+        d = get_denoiser(tuple(vol.shape), patch, stride, device=vol.device, form=form)
+        out = torch.empty_like(vol)
+        d.run_device(vol.contiguous(), out)
+        res = out
+    else:
+        shape = tuple(vol.shape)
+        d = get_denoiser(shape, patch, stride, form=form)
+        res = d(vol)
+    if return_info:
+        return res, d.info()
+    return res

@@ -1,0 +1,87 @@
+"""CPU tests of the C ABI: the library loads, exports every declared symbol,
+and validates arguments on the host (no GPU work is launched here)."""
+
+import re
+
+import pytest
+
+from conftest import ROOT
+
+HEADER = ROOT / "include" / "mppca.h"
+
+
+def declared_functions():
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(mppca_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_header_declares_the_stages():
+    names = declared_functions()
+    for stage in ("mppca_centre", "mppca_gram", "mppca_gram64", "mppca_eig64_mp",
+                  "mppca_eigh_jacobi", "mppca_mp_threshold",
+                  "mppca_eigh_mp", "mppca_recon_scatter", "mppca_normalize",
+                  "mppca_denoise_device", "mppca_workspace_bytes", "mppca_last_error"):
+        assert stage in names
+
+
+def test_library_exports_every_declared_symbol(lib, pkg):
+    names = declared_functions()
+    for n in names:
+        assert hasattr(lib, n), f"{n} declared in mppca.h but not exported"
+    assert set(names) == set(pkg._lib.SIGNATURES), "ctypes signatures out of sync with mppca.h"
+
+
+def test_abi_version(lib, pkg):
+    assert lib.mppca_abi_version() == pkg._lib.ABI_VERSION
+
+
+def test_num_patches_and_errors(lib):
+    assert lib.mppca_num_patches(96, 96, 48, 11, 11, 11, 3) == 12600
+    assert lib.mppca_num_patches(32, 32, 24, 5, 5, 5, 2) == 2475
+    assert lib.mppca_num_patches(5, 5, 5, 11, 11, 11, 3) < 0
+    assert b"larger than volume" in lib.mppca_last_error()
+    assert lib.mppca_num_patches(16, 16, 16, 4, 4, 4, 0) < 0
+
+
+PLAIN, FP32 = 0, 1
+
+
+def test_workspace_bytes(lib):
+    x, y, z, t = 96, 96, 48, 48
+    nvox, p = x * y * z, 12600
+    ws = lib.mppca_workspace_bytes(x, y, z, t, 11, 11, 11, 3, FP32)
+    assert 4 * (nvox * t + nvox + p * t * t + 4 * p) <= ws
+    assert ws < 4 * (nvox * t + nvox + p * t * t + 4 * p) + 256 * 16
+    ws = lib.mppca_workspace_bytes(x, y, z, t, 11, 11, 11, 3, PLAIN)  # + FP64 Gram
+    assert 4 * (nvox * t + nvox + 3 * p * t * t + 4 * p) <= ws
+    assert ws < 4 * (nvox * t + nvox + 3 * p * t * t + 4 * p) + 256 * 16
+    assert lib.mppca_workspace_bytes(x, y, z, t, 11, 11, 11, 3, 7) < 0  # unknown form
+    assert lib.mppca_workspace_bytes(x, y, z, 129, 11, 11, 11, 3, 0) < 0  # T > 128
+    assert lib.mppca_workspace_bytes(x, y, z, 1, 11, 11, 11, 3, 0) < 0   # T < 2
+    assert lib.mppca_workspace_bytes(40, 40, 40, 48, 17, 17, 17, 3, 0) < 0  # N > 4096
+
+
+def test_host_side_validation_launches_nothing(lib):
+    # null pointers and bad ranges are rejected before any launch
+    assert lib.mppca_gram(None, None, 16, 16, 16, 8, 5, 5, 5, 2, 0, 1, None) < 0
+    assert lib.mppca_gram(1, 1, 16, 16, 16, 8, 5, 5, 5, 2, 0, 10**6, None) < 0
+    assert b"outside" in lib.mppca_last_error()
+    assert lib.mppca_eigh_jacobi(None, None, None, None, 4, 8, 10, None) < 0
+    assert lib.mppca_eigh_mp(1, 4, 200, 100, 10, 1, 1, 1, None, None, None, None) < 0
+    assert lib.mppca_eig64_mp(None, 4, 8, 100, None, None, None, None, None, None, None) < 0
+    assert lib.mppca_gram64(1, 1, 16, 16, 16, 8, 5, 5, 5, 2, -1, 1, None) < 0
+    dd = lib.mppca_denoise_device
+    assert dd(1, 1, 16, 16, 16, 8, 5, 5, 5, 2, 0, 1, 10, None, None, None, None) < 0
+    assert dd(256, 512, 16, 16, 16, 8, 5, 5, 5, 2, 0, 256, 10, None, None, None,
+              None) == -3  # workspace too small
+    assert dd(256, 256, 16, 16, 16, 8, 5, 5, 5, 2, 0, 256, 10, None, None, None,
+              None) == -1  # in place is not allowed
+    assert dd(256, 512, 16, 16, 16, 8, 5, 5, 5, 2, 5, 256, 1 << 30, None, None, None,
+              None) == -1  # unknown form
+
+
+def test_check_raises(pkg, lib):
+    lib.mppca_num_patches(5, 5, 5, 11, 11, 11, 3)
+    with pytest.raises(pkg._lib.MppcaError):
+        pkg._lib.check(-1, "probe")

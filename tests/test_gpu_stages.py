@@ -1,0 +1,327 @@
+"""GPU tests of each kernel through the C ABI, against float64 NumPy and
+closed forms. Tolerances are stated per test (FP32 arithmetic)."""
+
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+EPS32 = float(np.finfo(np.float32).eps)
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+def _stream():
+    return _torch().cuda.current_stream().cuda_stream
+
+
+def _dev(a, dtype=None):
+    torch = _torch()
+    t = torch.from_numpy(np.ascontiguousarray(a))
+    if dtype is not None:
+        t = t.to(dtype)
+    return t.cuda()
+
+
+def _sync():
+    _torch().cuda.synchronize()
+
+
+def _gather(vol, patch, stride):
+    """(P, N, T) patches of a volume in the reference's voxel order."""
+    import oracle as o
+
+    px, py, pz = patch
+    c = o.patch_corners(vol.shape, patch, stride)
+    return np.stack([vol[i:i + px, j:j + py, k:k + pz].reshape(-1, vol.shape[-1])
+                     for i, j, k in c])
+
+
+def test_centre(pkg, lib):
+    torch = _torch()
+    rng = np.random.default_rng(0)
+    for t in (2, 17, 48, 128):
+        vol = (rng.standard_normal((7, 6, 5, t)) * 3 + 100).astype(np.float32)
+        d = _dev(vol)
+        xc = torch.empty_like(d)
+        mean = torch.empty(vol.shape[:3], dtype=torch.float32, device="cuda")
+        pkg._lib.check(lib.mppca_centre(d.data_ptr(), xc.data_ptr(), mean.data_ptr(),
+                                        7 * 6 * 5, t, _stream()), "centre")
+        _sync()
+        m64 = vol.astype(np.float64).mean(-1)
+        # float32 sum of T terms, then one division: |err| <= T*eps*max|x|
+        tol = t * EPS32 * np.abs(vol).max()
+        assert np.abs(mean.cpu().numpy() - m64).max() <= tol
+        np.testing.assert_allclose(xc.cpu().numpy(), vol - mean.cpu().numpy()[..., None],
+                                   rtol=0, atol=0)
+
+
+@pytest.mark.parametrize("shape,patch,stride", [
+    ((12, 11, 10, 10), (5, 5, 5), 2),
+    ((14, 12, 12, 16), (3, 4, 5), 3),
+    ((16, 16, 14, 30), (7, 7, 7), 3),
+    ((13, 13, 13, 48), (11, 11, 11), 2),
+    ((10, 9, 8, 64), (5, 5, 5), 2),
+    ((9, 8, 8, 100), (4, 4, 4), 2),
+    ((8, 8, 8, 128), (5, 5, 5), 3),
+    ((6, 6, 6, 2), (3, 3, 3), 1),
+])
+def test_gram_vs_fp64(pkg, lib, shape, patch, stride):
+    torch = _torch()
+    rng = np.random.default_rng(1)
+    vol = (rng.random(shape[:3] + (3,)) @ rng.standard_normal((3, shape[3]))
+           + 0.05 * rng.standard_normal(shape)).astype(np.float32)
+    xc = (vol - vol.mean(-1, keepdims=True, dtype=np.float64)).astype(np.float32)
+    P = pkg.num_patches(shape, patch, stride)
+    t = shape[3]
+    g = torch.empty((P, t, t), dtype=torch.float32, device="cuda")
+    # two launches over a split patch range exercise b0 / nb
+    half = P // 2
+    dx = _dev(xc)
+    for b0, nb in ((0, half), (half, P - half)):
+        gv = g[b0:]
+        pkg._lib.check(lib.mppca_gram(dx.data_ptr(), gv.data_ptr(), *shape, *patch, stride,
+                                      b0, nb, _stream()), "gram")
+    _sync()
+    X = _gather(xc, patch, stride).astype(np.float64)
+    G64 = np.einsum("bnt,bns->bts", X, X)
+    A = np.abs(X)
+    bound = np.einsum("bnt,bns->bts", A, A) * (np.sqrt(X.shape[1]) + 2) * EPS32
+    Gg = g.cpu().numpy().astype(np.float64)
+    assert np.all(np.abs(Gg - G64) <= bound + 1e-30)
+    np.testing.assert_array_equal(Gg, np.swapaxes(Gg, 1, 2))  # exactly symmetric
+
+
+def _random_spd(rng, b, t, cond=1e4, null=True):
+    q, _ = np.linalg.qr(rng.standard_normal((b, t, t)))
+    lam = np.exp(rng.uniform(0, np.log(cond), (b, t)))
+    if null:
+        lam[:, 0] = 0.0
+    return np.einsum("bij,bj,bkj->bik", q, lam, q), lam
+
+
+@pytest.mark.parametrize("t", [2, 3, 16, 17, 48, 64, 96, 128])
+def test_eigh_jacobi(pkg, lib, t):
+    torch = _torch()
+    rng = np.random.default_rng(2)
+    b = 24
+    G64, _ = _random_spd(rng, b, t)
+    G = G64.astype(np.float32)
+    dg = _dev(G)
+    lam = torch.empty((b, t), dtype=torch.float32, device="cuda")
+    vec = torch.empty((b, t, t), dtype=torch.float32, device="cuda")
+    info = torch.empty(b, dtype=torch.int32, device="cuda")
+    pkg._lib.check(lib.mppca_eigh_jacobi(dg.data_ptr(), lam.data_ptr(), vec.data_ptr(),
+                                         info.data_ptr(), b, t, 30, _stream()), "eigh")
+    _sync()
+    L, V, inf = (lam.cpu().numpy().astype(np.float64), vec.cpu().numpy().astype(np.float64),
+                 info.cpu().numpy())
+    assert np.all(inf >= 0), f"no convergence: {inf}"
+    ref = np.linalg.eigvalsh(G.astype(np.float64))
+    nrm = np.linalg.norm(G.astype(np.float64), axis=(1, 2), ord=2)[:, None]
+    assert np.all(np.diff(L, axis=1) >= 0)  # ascending
+    assert np.all(np.abs(L - ref).max(axis=1) <= 8 * t * EPS32 * nrm[:, 0])
+    eye = np.eye(t)
+    assert np.abs(np.swapaxes(V, 1, 2) @ V - eye).max() <= 8 * t * EPS32
+    resid = np.linalg.norm(G.astype(np.float64) @ V - V * L[:, None, :], axis=(1, 2))
+    assert np.all(resid <= 8 * t * EPS32 * nrm[:, 0])
+
+
+def test_eigh_closed_forms(pkg, lib):
+    torch = _torch()
+    t = 6
+    mats = np.zeros((4, t, t), dtype=np.float32)
+    mats[1] = np.diag([5, -1, 3, 0, 2, 7]).astype(np.float32)  # diagonal: no rotation
+    a, b, c = 2.0, 3.0, -1.0
+    mats[2, 0, 0], mats[2, 1, 1], mats[2, 0, 1], mats[2, 1, 0] = a, c, b, b  # 2x2 block
+    mats[3] = np.full((t, t), 1.5, dtype=np.float32)  # rank one: eigenvalues 0 x5, 9
+    dg = _dev(mats)
+    lam = torch.empty((4, t), dtype=torch.float32, device="cuda")
+    vec = torch.empty((4, t, t), dtype=torch.float32, device="cuda")
+    info = torch.empty(4, dtype=torch.int32, device="cuda")
+    pkg._lib.check(lib.mppca_eigh_jacobi(dg.data_ptr(), lam.data_ptr(), vec.data_ptr(),
+                                         info.data_ptr(), 4, t, 30, _stream()), "eigh")
+    _sync()
+    L = lam.cpu().numpy()
+    inf = info.cpu().numpy()
+    np.testing.assert_array_equal(L[0], np.zeros(t))
+    assert inf[0] == 0 and inf[1] == 0
+    np.testing.assert_array_equal(L[1], np.sort([5, -1, 3, 0, 2, 7]).astype(np.float32))
+    disc = np.sqrt(((a - c) / 2) ** 2 + b * b)
+    two = np.sort([(a + c) / 2 - disc, (a + c) / 2 + disc, 0, 0, 0, 0])
+    np.testing.assert_allclose(L[2], two, rtol=0, atol=4 * EPS32 * 4)
+    np.testing.assert_allclose(L[3], [0, 0, 0, 0, 0, 9.0], rtol=0, atol=16 * EPS32 * 9)
+
+
+def _mp_reference(lam_asc, n_vox, t):
+    """Oracle MP rank in step 0's dtypes from ascending eigenvalues."""
+    m, n = min(n_vox, t - 1), max(n_vox, t - 1)
+    desc = lam_asc[::-1][:m]
+    eigs = np.maximum(desc, np.float32(0)) / np.float32(n)
+    assert eigs.dtype == np.float32
+    return oracle.mp_rank(eigs, n, m)[0]
+
+
+@pytest.mark.parametrize("n_vox,t", [(1331, 48), (125, 16), (343, 30), (729, 100),
+                                     (125, 128), (64, 64), (27, 2)])
+def test_mp_threshold_bit_exact(pkg, lib, n_vox, t):
+    """Ranks equal the oracle's exactly on the same eigenvalues (integer result)."""
+    torch = _torch()
+    rng = np.random.default_rng(3)
+    b = 400
+    m = min(n_vox, t - 1)
+    lam = np.zeros((b, t), dtype=np.float32)
+    for i in range(b):
+        r = rng.integers(0, min(m, 10) + 1)
+        noise = np.sort(rng.gamma(4.0, 0.25, m))[::-1]
+        spikes = rng.uniform(1.5, 30.0, r) * noise.max()
+        vals = np.sort(np.concatenate([spikes, noise[r:]]))[::-1] * n_vox * 1e-3
+        full = np.zeros(t)
+        full[:m] = vals
+        full[m:] = rng.uniform(-1e-6, 1e-6, t - m)  # null / rank-deficient tail
+        lam[i] = np.sort(full).astype(np.float32)
+    dl = _dev(lam)
+    rank = torch.empty(b, dtype=torch.int32, device="cuda")
+    w = torch.empty(b, dtype=torch.float32, device="cuda")
+    mg = torch.empty(b, dtype=torch.float32, device="cuda")
+    pkg._lib.check(lib.mppca_mp_threshold(dl.data_ptr(), b, t, n_vox, rank.data_ptr(),
+                                          w.data_ptr(), mg.data_ptr(), _stream()), "mp")
+    _sync()
+    got = rank.cpu().numpy()
+    want = np.array([_mp_reference(lam[i], n_vox, t) for i in range(b)])
+    np.testing.assert_array_equal(got, want)
+    if m >= 8:
+        assert len(np.unique(want)) > 3
+    np.testing.assert_array_equal(w.cpu().numpy(),
+                                  (1.0 / (2.0 + want.astype(np.float64))).astype(np.float32))
+
+
+def test_recon_and_normalize_vs_fp64(pkg, lib):
+    torch = _torch()
+    rng = np.random.default_rng(4)
+    shape, patch, stride = (13, 12, 11, 24), (5, 4, 6), 2
+    t = shape[3]
+    vol = rng.standard_normal(shape).astype(np.float32)
+    mean = vol.mean(-1, dtype=np.float64).astype(np.float32)
+    xc = vol - mean[..., None]
+    P = pkg.num_patches(shape, patch, stride)
+    q = rng.standard_normal((P, t, 3))
+    proj = (q @ np.swapaxes(q, 1, 2)).astype(np.float32)
+    wts = (1.0 / (2.0 + rng.integers(0, 8, P))).astype(np.float32)
+    acc = torch.zeros(shape, dtype=torch.float32, device="cuda")
+    dxc, dp, dw, dm = _dev(xc), _dev(proj), _dev(wts), _dev(mean)
+    pkg._lib.check(lib.mppca_recon_scatter(dxc.data_ptr(), dp.data_ptr(), dw.data_ptr(),
+                                           acc.data_ptr(), *shape, *patch, stride, 0, P,
+                                           _stream()), "recon")
+    out = torch.empty_like(acc)
+    pkg._lib.check(lib.mppca_normalize(acc.data_ptr(), dm.data_ptr(), dw.data_ptr(),
+                                       out.data_ptr(), *shape, *patch, stride, _stream()),
+                   "normalize")
+    _sync()
+    ref_acc = np.zeros(shape)
+    wsum = np.zeros(shape[:3])
+    px, py, pz = patch
+    for b, (i, j, k) in enumerate(pkg.patch_corners(shape, patch, stride)):
+        sl = np.s_[i:i + px, j:j + py, k:k + pz]
+        ref_acc[sl] += wts[b] * (xc[sl].astype(np.float64) @ proj[b].astype(np.float64))
+        wsum[sl] += wts[b]
+    scale = np.abs(xc).max() * np.abs(proj).max() * t * 40
+    np.testing.assert_allclose(acc.cpu().numpy(), ref_acc, rtol=0, atol=scale * 4 * EPS32)
+    ref_out = mean[..., None] + ref_acc / wsum[..., None]
+    np.testing.assert_allclose(out.cpu().numpy(), ref_out, rtol=0,
+                               atol=4 * EPS32 * (np.abs(ref_out).max() + scale))
+
+
+def test_gram64_exact(pkg, lib):
+    """FP64 MFMA Gram: products of float32 values are exact, sums carry 53 bits."""
+    torch = _torch()
+    rng = np.random.default_rng(8)
+    for shape, patch, stride in [((13, 13, 13, 48), (11, 11, 11), 2), ((9, 8, 8, 100), (4, 4, 4), 2),
+                                 ((10, 9, 8, 17), (5, 3, 4), 2), ((8, 8, 8, 128), (5, 5, 5), 3)]:
+        vol = (rng.random(shape[:3] + (3,)) @ rng.standard_normal((3, shape[3]))
+               + 0.05 * rng.standard_normal(shape)).astype(np.float32)
+        xc = (vol - vol.mean(-1, keepdims=True, dtype=np.float64)).astype(np.float32)
+        P, t = pkg.num_patches(shape, patch, stride), shape[3]
+        g = torch.empty((P, t, t), dtype=torch.float64, device="cuda")
+        dx = _dev(xc)
+        pkg._lib.check(lib.mppca_gram64(dx.data_ptr(), g.data_ptr(), *shape, *patch, stride, 0, P,
+                                        _stream()), "gram64")
+        _sync()
+        X = _gather(xc, patch, stride).astype(np.float64)
+        G64 = np.einsum("bnt,bns->bts", X, X)
+        A = np.abs(X)
+        bound = np.einsum("bnt,bns->bts", A, A) * X.shape[1] * 2.0 ** -52
+        assert np.all(np.abs(g.cpu().numpy() - G64) <= bound)
+
+
+def _eig64(pkg, lib, G, n_vox):
+    torch = _torch()
+    b, t, _ = G.shape
+    dg = _dev(G.astype(np.float64))
+    proj = torch.empty((b, t, t), dtype=torch.float32, device="cuda")
+    rank = torch.empty(b, dtype=torch.int32, device="cuda")
+    w = torch.empty(b, dtype=torch.float32, device="cuda")
+    mg = torch.empty(b, dtype=torch.float32, device="cuda")
+    lam = torch.empty((b, t), dtype=torch.float32, device="cuda")
+    info = torch.empty(b, dtype=torch.int32, device="cuda")
+    pkg._lib.check(lib.mppca_eig64_mp(dg.data_ptr(), b, t, n_vox, proj.data_ptr(),
+                                      rank.data_ptr(), w.data_ptr(), mg.data_ptr(),
+                                      lam.data_ptr(), info.data_ptr(), _stream()), "eig64_mp")
+    _sync()
+    return (proj.cpu().numpy(), rank.cpu().numpy(), w.cpu().numpy(), mg.cpu().numpy(),
+            lam.cpu().numpy(), info.cpu().numpy())
+
+
+@pytest.mark.parametrize("n_vox,t", [(1331, 48), (125, 16), (343, 30), (729, 96), (1331, 128),
+                                     (64, 2), (125, 3), (60, 17), (40, 64)])
+def test_eig64_mp_vs_numpy(pkg, lib, n_vox, t):
+    """Eigenvalues, MP rank and rank-p projector of FP64 Gram matrices of
+    low-rank + noise patches against numpy.linalg.eigh in float64."""
+    rng = np.random.default_rng(9)
+    b = 32
+    X = np.empty((b, n_vox, t))
+    for i in range(b):
+        r = int(rng.integers(0, min(8, t - 1) + 1))
+        sig = rng.random((n_vox, r)) @ rng.standard_normal((r, t)) * rng.uniform(0.2, 3.0)
+        X[i] = sig + rng.uniform(0.01, 0.1) * rng.standard_normal((n_vox, t))
+    X -= X.mean(axis=2, keepdims=True)
+    G = np.einsum("bnt,bns->bts", X, X)
+    proj, rank, w, mg, lam, info = _eig64(pkg, lib, G, n_vox)
+    assert np.all(info == 0)
+    ref_l, ref_v = np.linalg.eigh(G)
+    ref_l, ref_v = ref_l[:, ::-1], ref_v[:, :, ::-1]
+    nrm = ref_l[:, :1]
+    assert np.all(np.abs(lam - ref_l) <= 4 * EPS32 * np.abs(ref_l) + 1e-12 * nrm)
+    m, n = min(n_vox, t - 1), max(n_vox, t - 1)
+    want = np.array([oracle.mp_rank(np.maximum(lam[i, :m], np.float32(0)) / np.float32(n), n, m)[0]
+                     for i in range(b)])
+    np.testing.assert_array_equal(rank, want)
+    assert len(np.unique(rank)) > 1 or t <= 3
+    np.testing.assert_array_equal(w, (1.0 / (2.0 + rank)).astype(np.float32))
+    for i in range(b):
+        vp = ref_v[i, :, :rank[i]]
+        np.testing.assert_allclose(proj[i], vp @ vp.T, rtol=0, atol=1e-6)
+
+
+def test_eig64_degenerate(pkg, lib):
+    """Zero matrix, a repeated signal eigenvalue and a non-finite entry."""
+    t, n_vox = 12, 200
+    G = np.zeros((3, t, t))
+    q, _ = np.linalg.qr(np.random.default_rng(10).standard_normal((t, t)))
+    lam = np.full(t, 0.01)
+    lam[:3] = 50.0  # triple eigenvalue, clearly signal
+    lam[-1] = 0.0
+    G[1] = (q * lam) @ q.T
+    G[2] = G[1]
+    G[2, 3, 4] = G[2, 4, 3] = np.nan
+    proj, rank, w, mg, lamo, info = _eig64(pkg, lib, G, n_vox)
+    assert rank[0] == 0 and info[0] == 0 and np.all(proj[0] == 0)
+    assert rank[1] == 3 and info[1] == 0
+    np.testing.assert_allclose(proj[1], q[:, :3] @ q[:, :3].T, rtol=0, atol=1e-6)
+    assert info[2] == -2 and rank[2] == 0
