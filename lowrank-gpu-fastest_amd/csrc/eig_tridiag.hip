@@ -105,11 +105,13 @@ __device__ __forceinline__ double wave_max(double x) {
 // Number of eigenvalues of the tridiagonal (d, e) smaller than x.
 __device__ __forceinline__ int sturm_count(const double* d, const double* e2, int T, double x,
                                            double pivmin) {
+  // a pivot below pivmin (x an eigenvalue of a leading block) counts as -pivmin
   double q = d[0] - x;
+  if (fabs(q) < pivmin) q = -pivmin;
   int c = q < 0.0 ? 1 : 0;
   for (int i = 1; i < T; ++i) {
-    if (fabs(q) < pivmin) q = -pivmin;
     q = (d[i] - x) - e2[i - 1] / q;
+    if (fabs(q) < pivmin) q = -pivmin;
     c += q < 0.0 ? 1 : 0;
   }
   return c;

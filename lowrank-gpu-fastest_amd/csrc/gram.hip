@@ -3,7 +3,8 @@
 //
 // One wavefront per patch, 4 patches per 256-thread block. MFMA 16x16x4
 // operand layout (f32 and f64 alike): lane l supplies A[l%16][l/16] and
-// B[l/16][l%16]; the 16x16 result sits in 4 registers, D[4*(l/16)+r][l%16].
+// B[l/16][l%16]. The 16x16 result sits in 4 registers per lane, column l%16;
+// register r holds row 4*(l/16)+r for f32 but row l/16+4*r for f64.
 // For G = Xc^T Xc both operands are the same 4-voxel x 16-frame slice, so a
 // lane loads Xc[n0 + l/16][16*tb + l%16] once per frame block tb and feeds it
 // to every (tb1 <= tb2) product; only the upper block triangle is formed and
@@ -30,10 +31,13 @@ struct Vec4;
 template <>
 struct Vec4<float> {
   typedef floatx4 type;
+  // row of the 16x16 result held by register r of lane group kq = lane/16
+  static __device__ __forceinline__ int row(int kq, int r) { return 4 * kq + r; }
 };
 template <>
 struct Vec4<double> {
   typedef doublex4 type;
+  static __device__ __forceinline__ int row(int kq, int r) { return kq + 4 * r; }
 };
 
 template <int NTB, typename S>
@@ -89,7 +93,7 @@ __global__ __launch_bounds__(256) void k_gram(const float* __restrict__ xc, S* _
     for (int tb2 = tb1; tb2 < NTB; ++tb2, ++k)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int ti = tb1 * 16 + 4 * kq + r, tj = tb2 * 16 + i;
+        const int ti = tb1 * 16 + Vec4<S>::row(kq, r), tj = tb2 * 16 + i;
         if (ti < T && tj < T) {
           const S v = acc[k][r];
           Gb[ti * T + tj] = v;
