@@ -118,10 +118,11 @@ int mppca_eigh_mp(const float* gram, int64_t nb, int T, int N, int max_sweeps,
 /* FP64 eigensolver + MP rank + projector for patches b in [0, nb): FP64 Gram
  * matrices in, proj[b] = V_p V_p^T as float32, rank / weight / margin /
  * lam_desc (float32, descending; may be NULL) / info (0 ok, -2 non-finite
- * input; may be NULL) out. Only the top-p eigenvectors are computed.
+ * input; may be NULL) out. Only the top-p eigenvectors are computed. gram is
+ * used as scratch and clobbered (it is consumed before being overwritten).
  * Replaces eigh + mp_keep + the projector bmm (step8_tuned.py:134-137) in the
  * PLAIN form. */
-int mppca_eig64_mp(const double* gram, int64_t nb, int T, int N, float* proj,
+int mppca_eig64_mp(double* gram, int64_t nb, int T, int N, float* proj,
                    int32_t* rank, float* weight, float* margin, float* lam_desc,
                    int32_t* info, void* stream);
 

@@ -198,7 +198,7 @@ int mppca_eigh_mp(const float* gram, int64_t nb, int T, int N, int max_sweeps, f
                         (hipStream_t)stream);
 }
 
-int mppca_eig64_mp(const double* gram, int64_t nb, int T, int N, float* proj, int32_t* rank,
+int mppca_eig64_mp(double* gram, int64_t nb, int T, int N, float* proj, int32_t* rank,
                    float* weight, float* margin, float* lam_desc, int32_t* info, void* stream) {
   if (!gram || !proj || !rank || !weight) return set_err(MPPCA_EINVAL, "eig64_mp: null pointer");
   if (T < 2 || T > MAX_T || N < 1)
@@ -261,7 +261,7 @@ int mppca_denoise_device(const float* vol, float* out, int X, int Y, int Z, int 
   if ((rc = launch_centre(vol, w.xc, w.mean, g.nvox, T, st))) return rc;
   if (form == MPPCA_FORM_PLAIN) {
     if ((rc = launch_gram64(w.xc, (double*)w.gram, g, 0, g.P, st))) return rc;
-    if ((rc = launch_eig64_mp((const double*)w.gram, g.P, T, g.N, w.proj, rk, w.weight, mg,
+    if ((rc = launch_eig64_mp((double*)w.gram, g.P, T, g.N, w.proj, rk, w.weight, mg,
                               nullptr, inf, st)))
       return rc;
   } else {

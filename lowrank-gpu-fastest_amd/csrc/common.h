@@ -154,7 +154,7 @@ int launch_eigh_mp(const float* gram, long long nb, int T, int N, int max_sweeps
                    hipStream_t st);
 int launch_mp(const float* lam, long long nb, int T, int N, int* rank, float* weight,
               float* margin, hipStream_t st);
-int launch_eig64_mp(const double* gram, long long nb, int T, int N, float* proj, int* rank,
+int launch_eig64_mp(double* gram, long long nb, int T, int N, float* proj, int* rank,
                     float* weight, float* margin, float* lam_desc, int* info, hipStream_t st);
 int launch_recon(const float* xc, const float* proj, const float* weight, float* acc,
                  const Geom& g, long long b0, long long nb, hipStream_t st);

@@ -102,6 +102,14 @@ __device__ __forceinline__ double wave_max(double x) {
   return x;
 }
 
+// 1/q to full precision: hardware estimate plus two Newton steps.
+__device__ __forceinline__ double rcp64(double q) {
+  double r = __builtin_amdgcn_rcp(q);
+  r = fma(fma(-q, r, 1.0), r, r);
+  r = fma(fma(-q, r, 1.0), r, r);
+  return r;
+}
+
 // Number of eigenvalues of the tridiagonal (d, e) smaller than x.
 __device__ __forceinline__ int sturm_count(const double* d, const double* e2, int T, double x,
                                            double pivmin) {
@@ -110,7 +118,7 @@ __device__ __forceinline__ int sturm_count(const double* d, const double* e2, in
   if (fabs(q) < pivmin) q = -pivmin;
   int c = q < 0.0 ? 1 : 0;
   for (int i = 1; i < T; ++i) {
-    q = (d[i] - x) - e2[i - 1] / q;
+    q = (d[i] - x) - e2[i - 1] * rcp64(q);
     if (fabs(q) < pivmin) q = -pivmin;
     c += q < 0.0 ? 1 : 0;
   }
@@ -384,8 +392,309 @@ __global__ __launch_bounds__(64) void k_eig64_mp(const double* __restrict__ gram
   }
 }
 
-int launch_eig64_mp(const double* gram, long long nb, int T, int N, float* proj, int* rank,
+// ---------------------------------------------------------------------------
+// Register-resident form for T <= 48 (TP = T rounded up to 16, 32 or 48).
+// Lane l holds row l of A in VGPRs; every loop over a row is unrolled over TP
+// with zero padding, so no register is indexed at run time. Because A stays
+// symmetric, column k of the trailing block is row k, which lane k publishes
+// to LDS once per step. The reflector v_k overwrites row k of the patch's
+// Gram matrix in global memory (the matrix itself is in registers by then):
+// that buffer is clobbered. The top-p eigenvectors are back-transformed at
+// the end, x = H_0 H_1 ... H_{T-3} z, one lane per vector.
+// ---------------------------------------------------------------------------
+
+constexpr int kRowGroup = 8;  // eigenvectors per pass (one lane each)
+
+template <int TP>
+__global__ __launch_bounds__(64) void k_eig64_rows(double* __restrict__ gram, int T, int N,
+                                                   long long nb, float* __restrict__ proj,
+                                                   int* __restrict__ rank,
+                                                   float* __restrict__ weight,
+                                                   float* __restrict__ margin,
+                                                   float* __restrict__ lam_desc,
+                                                   int* __restrict__ info) {
+  constexpr int LD = TP + 1;
+  __shared__ double s_bc[TP];  // published row
+  __shared__ double s_v[TP], s_w[TP];
+  __shared__ double s_d[TP], s_e[TP], s_e2[TP], s_tau[TP];
+  __shared__ double s_lam[TP];
+  __shared__ double s_dm[kRowGroup * LD];
+  __shared__ double s_z[kRowGroup * LD];  // D+ pivots, then the vector
+  __shared__ float s_lamf[TP], s_ef[TP], s_rcf[TP];
+  __shared__ int s_p;
+
+  const long long b = blockIdx.x;
+  if (b >= nb) return;
+  const int lane = threadIdx.x;
+  const bool row_ok = lane < T;
+  double* Gb = gram + b * (long long)T * T;
+  double a[TP];
+  bool finite = true;
+#pragma unroll
+  for (int j = 0; j < TP; ++j) {
+    a[j] = (row_ok && j < T) ? Gb[lane * T + j] : 0.0;
+    finite = finite && isfinite(a[j]);
+  }
+  const bool all_finite = __any(!finite) == 0;  // also: every row is in registers
+  float* Pb = proj + b * (long long)T * T;
+  if (!all_finite) {
+    for (int idx = lane; idx < T * T; idx += 64) Pb[idx] = 0.f;
+    if (lane == 0) {
+      rank[b] = 0;
+      weight[b] = 0.5f;
+      if (margin) margin[b] = 0.f;
+      if (info) info[b] = -2;
+    }
+    if (lam_desc)
+      for (int c = lane; c < T; c += 64) lam_desc[b * T + c] = NAN;
+    return;
+  }
+
+  // ---- 1. tridiagonalisation ------------------------------------------------
+  for (int k = 0; k + 2 < T; ++k) {
+    if (lane == k) {
+#pragma unroll
+      for (int j = 0; j < TP; ++j) s_bc[j] = a[j];
+    }
+    __syncthreads();
+    const double x = (lane > k && lane < TP) ? s_bc[lane] : 0.0;
+    const double alpha = s_bc[k + 1];
+    const double ss = wave_sum(lane > k + 1 ? x * x : 0.0);
+    double tk = 0.0, beta = alpha, sc = 0.0;
+    if (ss > 0.0) {
+      const double mu = sqrt(alpha * alpha + ss);
+      beta = alpha >= 0.0 ? -mu : mu;
+      tk = (beta - alpha) / beta;
+      sc = 1.0 / (alpha - beta);
+    }
+    const double v = lane == k + 1 ? 1.0 : (lane > k + 1 ? x * sc : 0.0);
+    if (lane < TP) s_v[lane] = v;
+    if (row_ok) Gb[k * T + lane] = v;  // reflector k, zero for indices <= k
+    if (lane == 0) {
+      s_d[k] = s_bc[k];
+      s_e[k] = beta;
+      s_tau[k] = tk;
+    }
+    __syncthreads();
+    if (tk != 0.0) {
+      double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
+#pragma unroll
+      for (int j = 0; j < TP; ++j) {
+        const double vj = s_v[j];
+        if ((j & 3) == 0) p0 = fma(a[j], vj, p0);
+        if ((j & 3) == 1) p1 = fma(a[j], vj, p1);
+        if ((j & 3) == 2) p2 = fma(a[j], vj, p2);
+        if ((j & 3) == 3) p3 = fma(a[j], vj, p3);
+      }
+      const double pl = lane > k ? tk * ((p0 + p1) + (p2 + p3)) : 0.0;
+      const double K = 0.5 * tk * wave_sum(pl * v);
+      const double w = pl - K * v;
+      if (lane < TP) s_w[lane] = w;
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < TP; ++j) a[j] = fma(-v, s_w[j], fma(-w, s_v[j], a[j]));
+    }
+    __syncthreads();
+  }
+  // last 2x2 block: rows T-2 and T-1 (T is runtime: select out of the row)
+  if (lane == T - 2 || lane == T - 1) {
+    double dd = 0.0, ee = 0.0;
+#pragma unroll
+    for (int j = 0; j < TP; ++j) {
+      if (j == lane) dd = a[j];
+      if (lane == T - 1 && j == T - 2) ee = a[j];
+    }
+    s_d[lane] = dd;
+    if (lane == T - 1) s_e[T - 2] = ee;
+  }
+  if (lane == 0) s_e[T - 1] = 0.0;
+  __syncthreads();
+
+  // ---- 2. eigenvalues by bisection --------------------------------------------
+  double lo_g = INFINITY, hi_g = -INFINITY, emax2 = 0.0;
+  if (lane < T) {
+    const double el = lane > 0 ? fabs(s_e[lane - 1]) : 0.0;
+    const double er = fabs(s_e[lane]);
+    lo_g = s_d[lane] - el - er;
+    hi_g = s_d[lane] + el + er;
+    emax2 = s_e[lane] * s_e[lane];
+    s_e2[lane] = emax2;
+  }
+  lo_g = wave_min(lo_g);
+  hi_g = wave_max(hi_g);
+  emax2 = wave_max(emax2);
+  const double u = 1.1102230246251565e-16;
+  const double bnorm = fmax(fabs(lo_g), fabs(hi_g));
+  const double pivmin = 2.2250738585072014e-308 * fmax(1.0, emax2);
+  const double slack = 2.0 * u * bnorm * T + pivmin;
+  lo_g -= slack;
+  hi_g += slack;
+  __syncthreads();
+  if (lane < T) {
+    const int kk = T - 1 - lane;  // kk-th smallest
+    const double tol = 4.0 * u * bnorm + pivmin;
+    double lo = lo_g, hi = hi_g;
+    for (int it = 0; it < 96 && hi - lo > tol; ++it) {
+      const double x = 0.5 * (lo + hi);
+      double qq = s_d[0] - x;
+      if (fabs(qq) < pivmin) qq = -pivmin;
+      int c = qq < 0.0 ? 1 : 0;
+      for (int i = 1; i < T; ++i) {
+        qq = (s_d[i] - x) - s_e2[i - 1] * rcp64(qq);
+        if (fabs(qq) < pivmin) qq = -pivmin;
+        c += qq < 0.0 ? 1 : 0;
+      }
+      if (c > kk)
+        hi = x;
+      else
+        lo = x;
+    }
+    s_lam[lane] = 0.5 * (lo + hi);
+    s_lamf[lane] = (float)(0.5 * (lo + hi));
+  }
+  __syncthreads();
+  if (lam_desc)
+    for (int c = lane; c < T; c += 64) lam_desc[b * T + c] = s_lamf[c];
+
+  // ---- 3. MP rank ----------------------------------------------------------------
+  if (lane == 0) {
+    int p;
+    float mg;
+    mp_rule(s_lamf, s_ef, s_rcf, T, N, kErrFp64Route, &p, &mg);
+    s_p = p;
+    rank[b] = p;
+    weight[b] = (float)(1.0 / (2.0 + (double)p));
+    if (margin) margin[b] = mg;
+    if (info) info[b] = 0;
+  }
+  __syncthreads();
+  const int p = s_p;
+
+  // ---- 4./5. top-p eigenvectors, back-transform, projector rows -----------------
+  const double pvfloor = fmax(u * bnorm, 2.2250738585072014e-308);
+  float* dst = Pb + lane * T;  // row `lane` of P_b
+  if (p == 0 && row_ok)
+    for (int c = 0; c < T; ++c) dst[c] = 0.f;
+  for (int g0 = 0; g0 < p; g0 += kRowGroup) {
+    const int gn = p - g0 < kRowGroup ? p - g0 : kRowGroup;
+    if (lane < gn) {
+      // twisted factorisation for lambda = s_lam[g0 + lane]; z holds D+ first
+      const double lam = s_lam[g0 + lane];
+      double* z = s_z + lane * LD;
+      double* dm = s_dm + lane * LD;
+      double dq = s_d[0] - lam;
+      for (int i = 0; i + 1 < T; ++i) {
+        if (fabs(dq) < pvfloor) dq = dq < 0.0 ? -pvfloor : pvfloor;
+        z[i] = dq;
+        dq = (s_d[i + 1] - lam) - s_e2[i] * rcp64(dq);
+      }
+      z[T - 1] = dq;
+      // backward pivots D-, twist index r = argmin |D+_i + D-_i - (d_i - lambda)|
+      dq = s_d[T - 1] - lam;
+      int r = T - 1;
+      double best = fabs(z[T - 1] + dq - (s_d[T - 1] - lam));
+      for (int i = T - 2; i >= 0; --i) {
+        if (fabs(dq) < pvfloor) dq = dq < 0.0 ? -pvfloor : pvfloor;
+        dm[i + 1] = dq;
+        dq = (s_d[i] - lam) - s_e2[i] * rcp64(dq);
+        const double gam = fabs(z[i] + dq - (s_d[i] - lam));
+        if (gam < best) {
+          best = gam;
+          r = i;
+        }
+      }
+      // z_r = 1, then outward (z[i] overwrites D+_i after reading it)
+      double nrm = 1.0;
+      double zn = 1.0;
+      for (int i = r - 1; i >= 0; --i) {
+        zn = -(s_e[i] * rcp64(z[i])) * zn;
+        z[i] = zn;
+        nrm = fma(zn, zn, nrm);
+      }
+      z[r] = 1.0;
+      zn = 1.0;
+      for (int i = r; i + 1 < T; ++i) {
+        zn = -(s_e[i] * rcp64(dm[i + 1])) * zn;
+        z[i + 1] = zn;
+        nrm = fma(zn, zn, nrm);
+      }
+      const double sn = 1.0 / sqrt(nrm);
+      for (int i = 0; i < T; ++i) z[i] *= sn;
+    }
+    __syncthreads();
+    // modified Gram-Schmidt inside the group (guards near-equal eigenvalues)
+    for (int a2 = 1; a2 < gn; ++a2) {
+      for (int c2 = 0; c2 < a2; ++c2) {
+        const double part = lane < T ? s_z[a2 * LD + lane] * s_z[c2 * LD + lane] : 0.0;
+        const double dot = wave_sum(part);
+        if (lane < T) s_z[a2 * LD + lane] -= dot * s_z[c2 * LD + lane];
+        __syncthreads();
+      }
+      const double part = lane < T ? s_z[a2 * LD + lane] * s_z[a2 * LD + lane] : 0.0;
+      const double sc2 = 1.0 / sqrt(wave_sum(part));
+      if (lane < T) s_z[a2 * LD + lane] *= sc2;
+      __syncthreads();
+    }
+    // back-transform, lane g: x = H_0 (H_1 (... H_{T-3} z))
+    if (lane < gn) {
+      double zr[TP];
+#pragma unroll
+      for (int j = 0; j < TP; ++j) zr[j] = j < T ? s_z[lane * LD + j] : 0.0;
+      for (int k = T - 3; k >= 0; --k) {
+        const double tk = s_tau[k];
+        if (tk == 0.0) continue;
+        const double* hk = Gb + k * T;
+        double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+        for (int j = 0; j < TP; ++j) {
+          if (j < T) {
+            if (j & 1)
+              s1 = fma(hk[j], zr[j], s1);
+            else
+              s0 = fma(hk[j], zr[j], s0);
+          }
+        }
+        const double sk = tk * (s0 + s1);
+#pragma unroll
+        for (int j = 0; j < TP; ++j)
+          if (j < T) zr[j] = fma(-sk, hk[j], zr[j]);
+      }
+#pragma unroll
+      for (int j = 0; j < TP; ++j)
+        if (j < T) s_z[lane * LD + j] = zr[j];
+    }
+    __syncthreads();
+    // P[l][c] (+)= sum_g x_g[l] x_g[c]; rank > 8 accumulates over passes in
+    // global memory (each lane owns its row, so no other lane touches it)
+    if (row_ok) {
+      double xl[kRowGroup];
+#pragma unroll
+      for (int g = 0; g < kRowGroup; ++g) xl[g] = g < gn ? s_z[g * LD + lane] : 0.0;
+      for (int c = 0; c < T; ++c) {
+        double acc = g0 == 0 ? 0.0 : (double)dst[c];
+#pragma unroll
+        for (int g = 0; g < kRowGroup; ++g)
+          if (g < gn) acc = fma(xl[g], s_z[g * LD + c], acc);
+        dst[c] = (float)acc;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <int TP>
+static int launch_rows(double* gram, long long nb, int T, int N, float* proj, int* rank,
+                       float* weight, float* margin, float* lam_desc, int* info, hipStream_t st) {
+  hipLaunchKernelGGL((k_eig64_rows<TP>), dim3((unsigned)nb), dim3(64), 0, st, gram, T, N, nb,
+                     proj, rank, weight, margin, lam_desc, info);
+  return launch_status("eig64 (register) kernel");
+}
+
+int launch_eig64_mp(double* gram, long long nb, int T, int N, float* proj, int* rank,
                     float* weight, float* margin, float* lam_desc, int* info, hipStream_t st) {
+  if (T <= 16) return launch_rows<16>(gram, nb, T, N, proj, rank, weight, margin, lam_desc, info, st);
+  if (T <= 32) return launch_rows<32>(gram, nb, T, N, proj, rank, weight, margin, lam_desc, info, st);
+  if (T <= 48) return launch_rows<48>(gram, nb, T, N, proj, rank, weight, margin, lam_desc, info, st);
   const size_t lds = tri_lds_bytes(T);
   int rc = prepare_lds((const void*)k_eig64_mp, lds);
   if (rc) return rc;
