@@ -86,20 +86,46 @@ __device__ inline TriLds tri_carve(double* sm, int T) {
   return L;
 }
 
+// Wave-wide reductions of doubles on DPP lane moves (no LDS round trip):
+// quad xor 1, quad xor 2, half-row mirror and row mirror leave every lane of
+// a 16-lane row holding the row's result; the four rows are then combined
+// from lanes 0, 16, 32, 48 through scalar reads. Every lane gets the result.
+template <int CTRL>
+__device__ __forceinline__ double dpp64(double x) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(x), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(x), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double lane_read64(double x, int l) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(x), l);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(x), l);
+  return __hiloint2double(hi, lo);
+}
+constexpr int kDppQuadXor1 = 0xB1, kDppQuadXor2 = 0x4E, kDppRowHalfMirror = 0x141,
+              kDppRowMirror = 0x140;
+
 __device__ __forceinline__ double wave_sum(double x) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
-  return x;
+  x += dpp64<kDppQuadXor1>(x);
+  x += dpp64<kDppQuadXor2>(x);
+  x += dpp64<kDppRowHalfMirror>(x);
+  x += dpp64<kDppRowMirror>(x);
+  return (lane_read64(x, 0) + lane_read64(x, 16)) + (lane_read64(x, 32) + lane_read64(x, 48));
 }
 __device__ __forceinline__ double wave_min(double x) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) x = fmin(x, __shfl_xor(x, o));
-  return x;
+  x = fmin(x, dpp64<kDppQuadXor1>(x));
+  x = fmin(x, dpp64<kDppQuadXor2>(x));
+  x = fmin(x, dpp64<kDppRowHalfMirror>(x));
+  x = fmin(x, dpp64<kDppRowMirror>(x));
+  return fmin(fmin(lane_read64(x, 0), lane_read64(x, 16)),
+              fmin(lane_read64(x, 32), lane_read64(x, 48)));
 }
 __device__ __forceinline__ double wave_max(double x) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) x = fmax(x, __shfl_xor(x, o));
-  return x;
+  x = fmax(x, dpp64<kDppQuadXor1>(x));
+  x = fmax(x, dpp64<kDppQuadXor2>(x));
+  x = fmax(x, dpp64<kDppRowHalfMirror>(x));
+  x = fmax(x, dpp64<kDppRowMirror>(x));
+  return fmax(fmax(lane_read64(x, 0), lane_read64(x, 16)),
+              fmax(lane_read64(x, 32), lane_read64(x, 48)));
 }
 
 // 1/q to full precision: hardware estimate plus two Newton steps.
@@ -404,6 +430,82 @@ __global__ __launch_bounds__(64) void k_eig64_mp(const double* __restrict__ gram
 // ---------------------------------------------------------------------------
 
 constexpr int kRowGroup = 8;  // eigenvectors per pass (one lane each)
+constexpr int kHChunk = 8;    // reflectors staged through LDS per back-transform pass
+
+// Number of eigenvalues smaller than x of the tridiagonal with diagonal ds and
+// squared off-diagonal e2s, both scaled so that |ds|, |e| <= 1. Division-free
+// form of the LDL^T pivot count: the leading principal minors
+// p_i = (d_i - x) p_{i-1} - e_{i-1}^2 p_{i-2} (p_{-1} = 1) change sign exactly
+// where a pivot q_i = p_i / p_{i-1} is negative. An exactly zero minor counts
+// as a negative pivot (the -pivmin substitution of the pivot form), and the
+// pair (p_i, p_{i-1}) is rescaled by a power of two every 8 steps.
+__device__ __forceinline__ int sturm_count3(const double* ds, const double* e2s, int T, double x) {
+  constexpr double kTiny = 1e-290;
+  double pm = 1.0;
+  double pc = ds[0] - x;
+  if (pc == 0.0) pc = -kTiny;
+  int c = pc < 0.0 ? 1 : 0;
+  for (int i = 1; i < T; ++i) {
+    double pn = fma(ds[i] - x, pc, -e2s[i - 1] * pm);
+    if (pn == 0.0) pn = pc < 0.0 ? kTiny : -kTiny;
+    c += (pn < 0.0) != (pc < 0.0) ? 1 : 0;
+    pm = pc;
+    pc = pn;
+    if ((i & 7) == 0) {
+      const int ex = ilogb(fmax(fabs(pc), fabs(pm)));
+      pc = ldexp(pc, -ex);
+      pm = ldexp(pm, -ex);
+    }
+  }
+  return c;
+}
+
+// MP rule of mp_rule (common.h) evaluated with every lane: the float32
+// eigenvalue scaling and the float32 suffix sums keep step 0's sequential order
+// (lane 0), then lane i evaluates inequality i. Same results as mp_rule.
+__device__ void mp_rule_wave(const float* lam, float* e, float* rc, double* dsfx, int T, int N,
+                             MpErr err, int lane, int* p_out, float* margin_out) {
+  const int M = N < T - 1 ? N : T - 1;
+  const int n = N > T - 1 ? N : T - 1;
+  const float nf = (float)n;
+  const double u = 5.9604644775390625e-08;  // 2^-24
+  for (int i = lane; i < M; i += 64) e[i] = fmaxf(lam[i], 0.f) / nf;
+  __syncthreads();
+  const double emax = (double)e[0];
+  for (int i = lane; i < M; i += 64)
+    dsfx[i] = u * (err.abs_k * emax + err.rel_k * sqrt((double)e[i] * emax));
+  __syncthreads();
+  if (lane == 0) {
+    float run = e[M - 1];
+    rc[M - 1] = run;
+    double drun = dsfx[M - 1];
+    for (int i = M - 2; i >= 0; --i) {
+      run = run + e[i];
+      rc[i] = run;
+      drun += dsfx[i];
+      dsfx[i] = drun;  // suffix sum of the error bounds
+    }
+  }
+  __syncthreads();
+  const double sq = sqrt((double)M * (double)n);
+  const double de_last = u * (err.abs_k * emax + err.rel_k * sqrt((double)e[M - 1] * emax));
+  int cnt = 0;
+  double mg = INFINITY;
+  for (int i = lane; i < M; i += 64) {
+    const float d = e[i] - e[M - 1];
+    const double lhs = (double)d * (double)(M - i) * (double)(n - i);
+    const double rhs = (double)(4.f * rc[i]) * sq;
+    cnt += lhs > rhs ? 1 : 0;
+    const double de_i = u * (err.abs_k * emax + err.rel_k * sqrt((double)e[i] * emax));
+    const double eb = (de_i + de_last) * (double)(M - i) * (double)(n - i) + 4.0 * dsfx[i] * sq;
+    const double diff = fabs(lhs - rhs);
+    const double r = eb > 0.0 ? diff / eb : INFINITY;
+    mg = r < mg ? r : mg;
+  }
+  // integer count across lanes through the double reduction (exact below 2^53)
+  *p_out = (int)wave_sum((double)cnt);
+  *margin_out = (float)wave_min(mg);
+}
 
 template <int TP>
 __global__ __launch_bounds__(64) void k_eig64_rows(double* __restrict__ gram, int T, int N,
@@ -420,8 +522,9 @@ __global__ __launch_bounds__(64) void k_eig64_rows(double* __restrict__ gram, in
   __shared__ double s_lam[TP];
   __shared__ double s_dm[kRowGroup * LD];
   __shared__ double s_z[kRowGroup * LD];  // D+ pivots, then the vector
+  __shared__ double s_h[kHChunk * TP];    // staged reflectors
+  __shared__ double s_ds[TP], s_e2s[TP], s_dsfx[TP];
   __shared__ float s_lamf[TP], s_ef[TP], s_rcf[TP];
-  __shared__ int s_p;
 
   const long long b = blockIdx.x;
   if (b >= nb) return;
@@ -509,6 +612,14 @@ __global__ __launch_bounds__(64) void k_eig64_rows(double* __restrict__ gram, in
   }
   if (lane == 0) s_e[T - 1] = 0.0;
   __syncthreads();
+  // split the tridiagonal where |e_i| <= 8u (|d_i| + |d_{i+1}|): a backward
+  // perturbation of the order of the reduction's own rounding, after which an
+  // exactly repeated eigenvalue lives in separate blocks
+  if (lane + 1 < T) {
+    const double thr = 8.8817841970012523e-16 * (fabs(s_d[lane]) + fabs(s_d[lane + 1]));
+    if (fabs(s_e[lane]) <= thr) s_e[lane] = 0.0;
+  }
+  __syncthreads();
 
   // ---- 2. eigenvalues by bisection --------------------------------------------
   double lo_g = INFINITY, hi_g = -INFINITY, emax2 = 0.0;
@@ -529,46 +640,44 @@ __global__ __launch_bounds__(64) void k_eig64_rows(double* __restrict__ gram, in
   const double slack = 2.0 * u * bnorm * T + pivmin;
   lo_g -= slack;
   hi_g += slack;
+  const double bsc = bnorm > 0.0 ? bnorm : 1.0;
+  const double inv = 1.0 / bsc;
+  if (lane < T) {
+    s_ds[lane] = s_d[lane] * inv;
+    s_e2s[lane] = s_e2[lane] * (inv * inv);
+  }
   __syncthreads();
   if (lane < T) {
     const int kk = T - 1 - lane;  // kk-th smallest
-    const double tol = 4.0 * u * bnorm + pivmin;
-    double lo = lo_g, hi = hi_g;
+    const double tol = 4.0 * u;   // of the scaled problem, |T| <= 1
+    double lo = lo_g * inv, hi = hi_g * inv;
     for (int it = 0; it < 96 && hi - lo > tol; ++it) {
       const double x = 0.5 * (lo + hi);
-      double qq = s_d[0] - x;
-      if (fabs(qq) < pivmin) qq = -pivmin;
-      int c = qq < 0.0 ? 1 : 0;
-      for (int i = 1; i < T; ++i) {
-        qq = (s_d[i] - x) - s_e2[i - 1] * rcp64(qq);
-        if (fabs(qq) < pivmin) qq = -pivmin;
-        c += qq < 0.0 ? 1 : 0;
-      }
-      if (c > kk)
+      if (sturm_count3(s_ds, s_e2s, T, x) > kk)
         hi = x;
       else
         lo = x;
     }
-    s_lam[lane] = 0.5 * (lo + hi);
-    s_lamf[lane] = (float)(0.5 * (lo + hi));
+    const double lam = 0.5 * (lo + hi) * bsc;
+    s_lam[lane] = lam;
+    s_lamf[lane] = (float)lam;
   }
   __syncthreads();
   if (lam_desc)
     for (int c = lane; c < T; c += 64) lam_desc[b * T + c] = s_lamf[c];
 
   // ---- 3. MP rank ----------------------------------------------------------------
-  if (lane == 0) {
-    int p;
+  int p;
+  {
     float mg;
-    mp_rule(s_lamf, s_ef, s_rcf, T, N, kErrFp64Route, &p, &mg);
-    s_p = p;
-    rank[b] = p;
-    weight[b] = (float)(1.0 / (2.0 + (double)p));
-    if (margin) margin[b] = mg;
-    if (info) info[b] = 0;
+    mp_rule_wave(s_lamf, s_ef, s_rcf, s_dsfx, T, N, kErrFp64Route, lane, &p, &mg);
+    if (lane == 0) {
+      rank[b] = p;
+      weight[b] = (float)(1.0 / (2.0 + (double)p));
+      if (margin) margin[b] = mg;
+      if (info) info[b] = 0;
+    }
   }
-  __syncthreads();
-  const int p = s_p;
 
   // ---- 4./5. top-p eigenvectors, back-transform, projector rows -----------------
   const double pvfloor = fmax(u * bnorm, 2.2250738585072014e-308);
@@ -578,10 +687,19 @@ __global__ __launch_bounds__(64) void k_eig64_rows(double* __restrict__ gram, in
   for (int g0 = 0; g0 < p; g0 += kRowGroup) {
     const int gn = p - g0 < kRowGroup ? p - g0 : kRowGroup;
     if (lane < gn) {
-      // twisted factorisation for lambda = s_lam[g0 + lane]; z holds D+ first
-      const double lam = s_lam[g0 + lane];
+      // Twisted factorisation of T - lambda I (z holds D+ first). Near a
+      // cluster the twisted vectors of different twist indices r are columns
+      // of the resolvent (T - lambda I)^-1, which span the cluster: the m-th
+      // copy of a (numerically) repeated eigenvalue takes the twist with the
+      // m-th smallest |gamma_r|, and Gram-Schmidt below separates the copies.
+      const int j = g0 + lane;
+      const double lam = s_lam[j];
       double* z = s_z + lane * LD;
       double* dm = s_dm + lane * LD;
+      double* gam = s_h + lane * TP;  // free until the back-transform
+      const double ctol = 16.0 * u * bnorm + pivmin;
+      int m = 0;
+      for (int i = 0; i < j; ++i) m += fabs(s_lam[i] - lam) <= ctol ? 1 : 0;
       double dq = s_d[0] - lam;
       for (int i = 0; i + 1 < T; ++i) {
         if (fabs(dq) < pvfloor) dq = dq < 0.0 ? -pvfloor : pvfloor;
@@ -589,21 +707,24 @@ __global__ __launch_bounds__(64) void k_eig64_rows(double* __restrict__ gram, in
         dq = (s_d[i + 1] - lam) - s_e2[i] * rcp64(dq);
       }
       z[T - 1] = dq;
-      // backward pivots D-, twist index r = argmin |D+_i + D-_i - (d_i - lambda)|
       dq = s_d[T - 1] - lam;
-      int r = T - 1;
-      double best = fabs(z[T - 1] + dq - (s_d[T - 1] - lam));
+      gam[T - 1] = fabs(z[T - 1] + dq - (s_d[T - 1] - lam));
       for (int i = T - 2; i >= 0; --i) {
         if (fabs(dq) < pvfloor) dq = dq < 0.0 ? -pvfloor : pvfloor;
         dm[i + 1] = dq;
         dq = (s_d[i] - lam) - s_e2[i] * rcp64(dq);
-        const double gam = fabs(z[i] + dq - (s_d[i] - lam));
-        if (gam < best) {
-          best = gam;
-          r = i;
-        }
+        gam[i] = fabs(z[i] + dq - (s_d[i] - lam));
       }
-      // z_r = 1, then outward (z[i] overwrites D+_i after reading it)
+      int r = 0;
+      for (int sel = 0; sel <= m; ++sel) {  // m-th smallest |gamma| (ties: lowest index)
+        double best = INFINITY;
+        for (int i = 0; i < T; ++i)
+          if (gam[i] < best) {
+            best = gam[i];
+            r = i;
+          }
+        gam[r] = INFINITY;
+      }
       double nrm = 1.0;
       double zn = 1.0;
       for (int i = r - 1; i >= 0; --i) {
@@ -635,30 +756,43 @@ __global__ __launch_bounds__(64) void k_eig64_rows(double* __restrict__ gram, in
       if (lane < T) s_z[a2 * LD + lane] *= sc2;
       __syncthreads();
     }
-    // back-transform, lane g: x = H_0 (H_1 (... H_{T-3} z))
-    if (lane < gn) {
-      double zr[TP];
+    // back-transform x = H_0 (H_1 (... H_{T-3} z)), one lane per vector; the
+    // reflectors come back from global memory kHChunk rows at a time, loaded
+    // by the whole wave (coalesced) into LDS, zero-padded to TP
+    double zr[TP];
 #pragma unroll
-      for (int j = 0; j < TP; ++j) zr[j] = j < T ? s_z[lane * LD + j] : 0.0;
-      for (int k = T - 3; k >= 0; --k) {
-        const double tk = s_tau[k];
-        if (tk == 0.0) continue;
-        const double* hk = Gb + k * T;
-        double s0 = 0.0, s1 = 0.0;
+    for (int j = 0; j < TP; ++j) zr[j] = (lane < gn && j < T) ? s_z[lane * LD + j] : 0.0;
+    for (int kc = T - 3; kc >= 0; kc -= kHChunk) {
+      const int klo = kc - kHChunk + 1 > 0 ? kc - kHChunk + 1 : 0;
+      for (int r = 0; r <= kc - klo; ++r)
+        for (int c = lane; c < TP; c += 64)
+          s_h[r * TP + c] = c < T ? Gb[(klo + r) * T + c] : 0.0;
+      __syncthreads();
+      if (lane < gn) {
+        for (int k = kc; k >= klo; --k) {
+          const double tk = s_tau[k];
+          if (tk == 0.0) continue;
+          const double* hk = s_h + (k - klo) * TP;
+          double s0 = 0.0, s1 = 0.0;
 #pragma unroll
-        for (int j = 0; j < TP; ++j) {
-          if (j < T) {
+          for (int j = 0; j < TP; ++j) {
             if (j & 1)
               s1 = fma(hk[j], zr[j], s1);
             else
               s0 = fma(hk[j], zr[j], s0);
+            if ((j & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+          }
+          const double sk = tk * (s0 + s1);
+#pragma unroll
+          for (int j = 0; j < TP; ++j) {
+            zr[j] = fma(-sk, hk[j], zr[j]);
+            if ((j & 7) == 7) __builtin_amdgcn_sched_barrier(0);
           }
         }
-        const double sk = tk * (s0 + s1);
-#pragma unroll
-        for (int j = 0; j < TP; ++j)
-          if (j < T) zr[j] = fma(-sk, hk[j], zr[j]);
       }
+      __syncthreads();
+    }
+    if (lane < gn) {
 #pragma unroll
       for (int j = 0; j < TP; ++j)
         if (j < T) s_z[lane * LD + j] = zr[j];
