@@ -44,7 +44,8 @@ extern "C" {
  * mppca_denoise_device). Both read and write float32 volumes.
  *   PLAIN: Gram on FP64 MFMA (exact products of the float32 centred data),
  *          FP64 eigensolver (Householder + bisection + twisted factorisation),
- *          FP32 MFMA reconstruction. Matches step 0 to ~1e-7.
+ *          FP32 MFMA reconstruction from the rank-p factors (T <= 64; above
+ *          that from the projector). Matches step 0 to ~1e-7.
  *   FP32 : the reference's all-FP32 arithmetic (FP32 MFMA Gram, FP32 Jacobi
  *          eigensolver), as step8_tuned.py computes it. */
 #define MPPCA_FORM_PLAIN 0
@@ -146,6 +147,44 @@ int mppca_normalize(const float* acc, const float* mean, const float* weight,
                     int pz, int stride, void* stream);
 
 /* ---------------------------------------------------------------------------
+ * Rank-p factor route (T <= 64)
+ *
+ * Instead of the T x T projector P_b = V_p V_p^T, the eigensolver can write
+ * the top-p eigenvectors themselves, which is all the reconstruction needs:
+ * xc @ P_b = (xc @ V_p) @ V_p^T. Per patch, mppca_factor_floats(T) floats
+ * (TF = 16*ceil(T/16) frames; tile nc holds eigenvectors 16nc .. 16nc+15):
+ *   Ut[nc][c][t] at nc*16*TF + c*TF + t           eigenvectors as rows
+ *   Uw[nc][t][c] at TF*TF + nc*16*TF + t*16 + c   weight[b] * eigenvectors
+ * Inside a tile of q = min(16, p - 16nc) vectors, S2 = ceil(q/4), vector g
+ * sits in slot c = 4*(g / S2) + g % S2; other slots and frames t >= T are 0.
+ * Only tiles nc < ceil(p/16) are written.
+ * ------------------------------------------------------------------------- */
+
+/* Floats of one patch's factor block for T frames (< 0 if T > 64). */
+int64_t mppca_factor_floats(int T);
+
+/* As mppca_eig64_mp, writing the rank-p factors of every patch into
+ * factors + b * mppca_factor_floats(T) (16-byte aligned) instead of the
+ * projector. gram is clobbered. T <= 64.
+ * Replaces eigh + mp_keep + the projector bmm (step8_tuned.py:134-137). */
+int mppca_eig64_factors(double* gram, int64_t nb, int T, int N, float* factors,
+                        int32_t* rank, float* weight, float* margin, float* lam_desc,
+                        int32_t* info, void* stream);
+
+/* Reconstruction fused with normalisation from the factors of all P patches,
+ * written once per voxel:
+ * out[v, :] = mean[v] + (sum_{b contains v} weight[b] * xc[v, :] @ P_b)
+ *             / (sum_{b contains v} weight[b]),  0 where no patch covers v,
+ * with W summed in increasing patch index (step 0's order). A wavefront owns
+ * a 2 x 4 x 4 voxel tile and keeps its sums in registers: no atomics, no
+ * zero-initialised accumulator. xc and factors 16-byte aligned; T <= 64.
+ * Replaces recon_scatter_kernel + the final normalisation
+ * (step8_tuned.py:59-86, 139-145; step0_numpy_loop.py:36-39). */
+int mppca_recon_factors(const float* xc, const float* factors, const int32_t* rank,
+                        const float* weight, const float* mean, float* out, int X, int Y,
+                        int Z, int T, int px, int py, int pz, int stride, void* stream);
+
+/* ---------------------------------------------------------------------------
  * Whole path on device buffers
  * ------------------------------------------------------------------------- */
 
@@ -155,7 +194,7 @@ int64_t mppca_workspace_bytes(int X, int Y, int Z, int T, int px, int py,
                               int pz, int stride, int form);
 
 /* Denoise a device-resident volume: centre, Gram, eigensolver + MP rank,
- * reconstruction + scatter, normalisation; all enqueued on `stream`.
+ * reconstruction + normalisation; all enqueued on `stream`.
  * vol and out are (X, Y, Z, T) float32 device buffers (distinct);
  * workspace is >= mppca_workspace_bytes(...) bytes of device memory, 256-byte
  * aligned. rank / margin / info (each P entries, may be NULL) receive the

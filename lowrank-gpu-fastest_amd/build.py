@@ -29,6 +29,9 @@ HIPCC_FLAGS = [
     # packed FP32 VALU forms are kept out of the generated code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-munsafe-fp-atomics",
+    # MFMA accumulators in VGPRs (gfx950 allows it): no AGPR<->VGPR copies of
+    # loop-carried accumulators around every MFMA group
+    "-mllvm", "-amdgpu-mfma-vgpr-form",
     "-Wall",
     "-Wno-unused-function",
 ]

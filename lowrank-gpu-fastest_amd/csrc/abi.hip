@@ -76,7 +76,8 @@ struct Workspace {
   float* xc;
   float* mean;
   void* gram;   // P*T*T doubles (plain form) or floats (fp32 form)
-  float* proj;  // P*T*T floats (plain form; the fp32 form projects in place of gram)
+  float* proj;  // P*T*T floats: the fp32 form (in place of gram) and the plain form above T = 64
+  float* fac;   // P*factor_floats(T) floats: rank-p factors, plain form with T <= 64
   float* weight;
   int* rank;
   float* margin;
@@ -97,9 +98,14 @@ Workspace carve_workspace(char* base, const Geom& g, int form) {
   const size_t ptt = (size_t)g.P * g.T * g.T;
   w.xc = (float*)take((size_t)g.nvox * g.T * sizeof(float));
   w.mean = (float*)take((size_t)g.nvox * sizeof(float));
+  w.proj = nullptr;
+  w.fac = nullptr;
   if (form == MPPCA_FORM_PLAIN) {
     w.gram = take(ptt * sizeof(double));
-    w.proj = (float*)take(ptt * sizeof(float));
+    if (g.T <= kFactorMaxT)
+      w.fac = (float*)take((size_t)g.P * factor_floats(g.T) * sizeof(float));
+    else
+      w.proj = (float*)take(ptt * sizeof(float));
   } else {
     w.gram = take(ptt * sizeof(float));
     w.proj = (float*)w.gram;
@@ -205,7 +211,28 @@ int mppca_eig64_mp(double* gram, int64_t nb, int T, int N, float* proj, int32_t*
     return set_err(MPPCA_EUNSUPPORTED, "eig64_mp: bad T=%d / N=%d", T, N);
   if (nb < 0) return set_err(MPPCA_EINVAL, "eig64_mp: nb < 0");
   if (nb == 0) return MPPCA_OK;
-  return launch_eig64_mp(gram, nb, T, N, proj, rank, weight, margin, lam_desc, info,
+  return launch_eig64_mp(gram, nb, T, N, proj, nullptr, rank, weight, margin, lam_desc, info,
+                         (hipStream_t)stream);
+}
+
+int64_t mppca_factor_floats(int T) {
+  if (T < 2 || T > kFactorMaxT)
+    return set_err(MPPCA_EUNSUPPORTED, "factors: T = %d outside [2, %d]", T, kFactorMaxT);
+  return factor_floats(T);
+}
+
+int mppca_eig64_factors(double* gram, int64_t nb, int T, int N, float* factors, int32_t* rank,
+                        float* weight, float* margin, float* lam_desc, int32_t* info,
+                        void* stream) {
+  if (!gram || !factors || !rank || !weight)
+    return set_err(MPPCA_EINVAL, "eig64_factors: null pointer");
+  if (T < 2 || T > kFactorMaxT || N < 1)
+    return set_err(MPPCA_EUNSUPPORTED, "eig64_factors: bad T=%d / N=%d", T, N);
+  if ((uintptr_t)factors % 16 != 0)
+    return set_err(MPPCA_EINVAL, "eig64_factors: factors must be 16-byte aligned");
+  if (nb < 0) return set_err(MPPCA_EINVAL, "eig64_factors: nb < 0");
+  if (nb == 0) return MPPCA_OK;
+  return launch_eig64_mp(gram, nb, T, N, nullptr, factors, rank, weight, margin, lam_desc, info,
                          (hipStream_t)stream);
 }
 
@@ -228,6 +255,22 @@ int mppca_normalize(const float* acc, const float* mean, const float* weight, fl
   if (rc) return rc;
   if (!acc || !mean || !weight || !out) return set_err(MPPCA_EINVAL, "normalize: null pointer");
   return launch_normalize(acc, mean, weight, out, g, (hipStream_t)stream);
+}
+
+int mppca_recon_factors(const float* xc, const float* factors, const int32_t* rank,
+                        const float* weight, const float* mean, float* out, int X, int Y, int Z,
+                        int T, int px, int py, int pz, int stride, void* stream) {
+  Geom g;
+  const int rc = make_geom(&g, X, Y, Z, T, px, py, pz, stride);
+  if (rc) return rc;
+  if (!xc || !factors || !rank || !weight || !mean || !out)
+    return set_err(MPPCA_EINVAL, "recon_factors: null pointer");
+  if (T > kFactorMaxT)
+    return set_err(MPPCA_EUNSUPPORTED, "recon_factors: T = %d above %d", T, kFactorMaxT);
+  if ((uintptr_t)factors % 16 != 0 || (uintptr_t)xc % 16 != 0)
+    return set_err(MPPCA_EINVAL, "recon_factors: xc and factors must be 16-byte aligned");
+  if (out == xc) return set_err(MPPCA_EINVAL, "recon_factors: out must not alias xc");
+  return launch_recon_factors(xc, factors, rank, weight, mean, out, g, (hipStream_t)stream);
 }
 
 int64_t mppca_workspace_bytes(int X, int Y, int Z, int T, int px, int py, int pz, int stride,
@@ -261,16 +304,18 @@ int mppca_denoise_device(const float* vol, float* out, int X, int Y, int Z, int 
   if ((rc = launch_centre(vol, w.xc, w.mean, g.nvox, T, st))) return rc;
   if (form == MPPCA_FORM_PLAIN) {
     if ((rc = launch_gram64(w.xc, (double*)w.gram, g, 0, g.P, st))) return rc;
-    if ((rc = launch_eig64_mp((double*)w.gram, g.P, T, g.N, w.proj, rk, w.weight, mg,
+    if ((rc = launch_eig64_mp((double*)w.gram, g.P, T, g.N, w.proj, w.fac, rk, w.weight, mg,
                               nullptr, inf, st)))
       return rc;
+    if (w.fac) return launch_recon_factors(w.xc, w.fac, rk, w.weight, w.mean, out, g, st);
   } else {
     if ((rc = launch_gram(w.xc, (float*)w.gram, g, 0, g.P, st))) return rc;
     if ((rc = launch_eigh_mp((const float*)w.gram, g.P, T, g.N, 20, w.proj, rk, w.weight, mg,
                              nullptr, inf, st)))
       return rc;
   }
-  if ((rc = zero_fill(out, g.nvox * T, st))) return rc;
+  // projector route: atomic scatter into the zeroed output, then normalise
+  if ((rc = zero_fill(out, g.nvox * g.T, st))) return rc;
   if ((rc = launch_recon(w.xc, w.proj, w.weight, out, g, 0, g.P, st))) return rc;
   return launch_normalize(out, w.mean, w.weight, out, g, st);
 }

@@ -15,7 +15,7 @@
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef double doublex4 __attribute__((ext_vector_type(4)));
 
-#define MPPCA_ABI_VERSION 101
+#define MPPCA_ABI_VERSION 102
 #define MAX_T 128
 #define MAX_N 4096
 
@@ -59,13 +59,16 @@ __device__ __forceinline__ long long patch_base(const Geom& g, long long b) {
 
 // Offsets (in voxels) of the N voxels of a patch from its first voxel,
 // x-major order (step2_torch_batched.py:24-27): n = (ox*py + oy)*pz + oz.
+// The table carries kOffPad trailing zeros (voxel 0 of the patch) so that
+// loads past the last voxel stay in bounds without a branch.
+constexpr int kOffPad = 16;
 __device__ __forceinline__ void fill_offsets(int* off, const Geom& g) {
-  for (int n = threadIdx.x; n < g.N; n += blockDim.x) {
+  for (int n = threadIdx.x; n < g.N + kOffPad; n += blockDim.x) {
     const int oz = n % g.pz;
     const int r = n / g.pz;
     const int oy = r % g.py;
     const int ox = r / g.py;
-    off[n] = (ox * g.Y + oy) * g.Z + oz;
+    off[n] = n < g.N ? (ox * g.Y + oy) * g.Z + oz : 0;
   }
 }
 
@@ -134,6 +137,54 @@ __device__ inline void mp_rule(const float* lam, float* e, float* rc, int T, int
   *margin_out = (float)mg;
 }
 
+// ---- rank-p factors of a patch -----------------------------------------------
+// TF = 16*ceil(T/16) frames; per patch 2*TF*TF floats:
+//   Ut[nc][c][t] at nc*16*TF + c*TF + t            the eigenvectors as rows
+//   Uw[nc][t][c] at TF*TF + nc*16*TF + t*16 + c    w_b * the eigenvectors as columns
+// Slot tile nc holds eigenvectors 16nc .. 16nc+15 (descending eigenvalue);
+// only tiles nc < ceil(p/16) are written. Inside a tile of q = min(16, p-16nc)
+// vectors with S2 = ceil(q/4), vector g16 sits in slot c = 4*(g16/S2) + g16%S2,
+// so every filled slot has c%4 < S2; empty slots and frames t >= T hold zeros.
+__host__ __device__ inline int factor_tf(int T) { return 16 * ((T + 15) / 16); }
+__host__ __device__ inline long long factor_floats(int T) { return 2LL * factor_tf(T) * factor_tf(T); }
+constexpr int kFactorMaxT = 64;
+
+__device__ __forceinline__ int factor_slot(int g16, int q) {
+  const int S2 = (q + 3) >> 2;
+  return 4 * (g16 / S2) + g16 % S2;
+}
+
+// Writes vectors g0 .. g0+gn-1 (unit, x_g[t] = z[g*ldz + t]) of a patch of rank
+// p and weight w into its factor block F. Lane t writes frame t (TF <= 64).
+__device__ inline void write_factors(float* F, const double* z, int ldz, int g0, int gn, int p,
+                                     int T, int TF, double w, int lane) {
+  if (lane >= TF) return;
+  for (int g = 0; g < gn; ++g) {
+    const int G = g0 + g, nc = G >> 4;
+    const int q = p - 16 * nc < 16 ? p - 16 * nc : 16;
+    const int c = factor_slot(G & 15, q);
+    const double v = lane < T ? z[g * ldz + lane] : 0.0;
+    F[nc * 16 * TF + c * TF + lane] = (float)v;
+    F[TF * TF + nc * 16 * TF + lane * 16 + c] = (float)(w * v);
+  }
+}
+
+// Zeros the slots of the written tiles that no vector fills.
+__device__ inline void zero_factor_slots(float* F, int p, int TF, int lane) {
+  if (lane >= TF) return;
+  const int nct = (p + 15) >> 4;
+  for (int nc = 0; nc < nct; ++nc) {
+    const int q = p - 16 * nc < 16 ? p - 16 * nc : 16;
+    const int S2 = (q + 3) >> 2;
+    for (int c = 0; c < 16; ++c) {
+      const int a = c >> 2, s = c & 3;
+      if (s < S2 && a * S2 + s < q) continue;  // slot of vector a*S2 + s
+      F[nc * 16 * TF + c * TF + lane] = 0.f;
+      F[TF * TF + nc * 16 * TF + lane * 16 + c] = 0.f;
+    }
+  }
+}
+
 // Error models of the two routes (see MpErr).
 constexpr MpErr kErrFp32Route = {16.0, 0.0};
 constexpr MpErr kErrFp64Route = {1.0, 32.0};
@@ -154,9 +205,13 @@ int launch_eigh_mp(const float* gram, long long nb, int T, int N, int max_sweeps
                    hipStream_t st);
 int launch_mp(const float* lam, long long nb, int T, int N, int* rank, float* weight,
               float* margin, hipStream_t st);
-int launch_eig64_mp(double* gram, long long nb, int T, int N, float* proj, int* rank,
+// proj (T x T projectors) and/or fac (rank-p factors, T <= kFactorMaxT) may be
+// null; each non-null one is written.
+int launch_eig64_mp(double* gram, long long nb, int T, int N, float* proj, float* fac, int* rank,
                     float* weight, float* margin, float* lam_desc, int* info, hipStream_t st);
 int launch_recon(const float* xc, const float* proj, const float* weight, float* acc,
                  const Geom& g, long long b0, long long nb, hipStream_t st);
+int launch_recon_factors(const float* xc, const float* fac, const int* rank, const float* weight,
+                         const float* mean, float* out, const Geom& g, hipStream_t st);
 
 }  // namespace mppca

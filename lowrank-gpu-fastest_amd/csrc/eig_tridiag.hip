@@ -316,6 +316,7 @@ __device__ void twisted_vector(const TriLds& L, int T, double lambda, double piv
 
 __global__ __launch_bounds__(64) void k_eig64_mp(const double* __restrict__ gram, int T, int N,
                                                  long long nb, float* __restrict__ proj,
+                                                 float* __restrict__ fac,
                                                  int* __restrict__ rank,
                                                  float* __restrict__ weight,
                                                  float* __restrict__ margin,
@@ -334,9 +335,11 @@ __global__ __launch_bounds__(64) void k_eig64_mp(const double* __restrict__ gram
   }
   __syncthreads();
   const bool finite = tridiagonalize(L, T, lane);
-  float* Pb = proj + b * (long long)T * T;
+  float* Pb = proj ? proj + b * (long long)T * T : nullptr;
+  float* Fb = fac ? fac + b * factor_floats(T) : nullptr;
   if (!finite) {  // non-finite input: no rank, no projector, flagged
-    for (int idx = lane; idx < T * T; idx += 64) Pb[idx] = 0.f;
+    if (Pb)
+      for (int idx = lane; idx < T * T; idx += 64) Pb[idx] = 0.f;
     if (lane == 0) {
       rank[b] = 0;
       weight[b] = 0.5f;
@@ -406,16 +409,19 @@ __global__ __launch_bounds__(64) void k_eig64_mp(const double* __restrict__ gram
       }
     }
     __syncthreads();
+    if (Fb) write_factors(Fb, L.z, LDZ, g0, gn, p, T, factor_tf(T), 1.0 / (2.0 + (double)p), lane);
     // P += X X^T over this group
-    for (int idx = lane; idx < T * T; idx += 64) {
-      const int r = idx / T, c = idx - r * T;
-      double s = 0.0;
-      for (int g = 0; g < gn; ++g) s += L.z[g * LDZ + r] * L.z[g * LDZ + c];
-      Pb[idx] = g0 == 0 ? (float)s : (float)((double)Pb[idx] + s);
-    }
+    if (Pb)
+      for (int idx = lane; idx < T * T; idx += 64) {
+        const int r = idx / T, c = idx - r * T;
+        double s = 0.0;
+        for (int g = 0; g < gn; ++g) s += L.z[g * LDZ + r] * L.z[g * LDZ + c];
+        Pb[idx] = g0 == 0 ? (float)s : (float)((double)Pb[idx] + s);
+      }
     __syncthreads();
     if (p == 0) break;
   }
+  if (Fb) zero_factor_slots(Fb, p, factor_tf(T), lane);
 }
 
 // ---------------------------------------------------------------------------
@@ -510,6 +516,7 @@ __device__ void mp_rule_wave(const float* lam, float* e, float* rc, double* dsfx
 template <int TP>
 __global__ __launch_bounds__(64) void k_eig64_rows(double* __restrict__ gram, int T, int N,
                                                    long long nb, float* __restrict__ proj,
+                                                   float* __restrict__ fac,
                                                    int* __restrict__ rank,
                                                    float* __restrict__ weight,
                                                    float* __restrict__ margin,
@@ -539,9 +546,11 @@ __global__ __launch_bounds__(64) void k_eig64_rows(double* __restrict__ gram, in
     finite = finite && isfinite(a[j]);
   }
   const bool all_finite = __any(!finite) == 0;  // also: every row is in registers
-  float* Pb = proj + b * (long long)T * T;
+  float* Pb = proj ? proj + b * (long long)T * T : nullptr;
+  float* Fb = fac ? fac + b * (2LL * TP * TP) : nullptr;  // TP = 16*ceil(T/16)
   if (!all_finite) {
-    for (int idx = lane; idx < T * T; idx += 64) Pb[idx] = 0.f;
+    if (Pb)
+      for (int idx = lane; idx < T * T; idx += 64) Pb[idx] = 0.f;
     if (lane == 0) {
       rank[b] = 0;
       weight[b] = 0.5f;
@@ -681,8 +690,8 @@ __global__ __launch_bounds__(64) void k_eig64_rows(double* __restrict__ gram, in
 
   // ---- 4./5. top-p eigenvectors, back-transform, projector rows -----------------
   const double pvfloor = fmax(u * bnorm, 2.2250738585072014e-308);
-  float* dst = Pb + lane * T;  // row `lane` of P_b
-  if (p == 0 && row_ok)
+  float* dst = Pb ? Pb + lane * T : nullptr;  // row `lane` of P_b
+  if (p == 0 && row_ok && dst)
     for (int c = 0; c < T; ++c) dst[c] = 0.f;
   for (int g0 = 0; g0 < p; g0 += kRowGroup) {
     const int gn = p - g0 < kRowGroup ? p - g0 : kRowGroup;
@@ -798,9 +807,10 @@ __global__ __launch_bounds__(64) void k_eig64_rows(double* __restrict__ gram, in
         if (j < T) s_z[lane * LD + j] = zr[j];
     }
     __syncthreads();
+    if (Fb) write_factors(Fb, s_z, LD, g0, gn, p, T, TP, 1.0 / (2.0 + (double)p), lane);
     // P[l][c] (+)= sum_g x_g[l] x_g[c]; rank > 8 accumulates over passes in
     // global memory (each lane owns its row, so no other lane touches it)
-    if (row_ok) {
+    if (row_ok && dst) {
       double xl[kRowGroup];
 #pragma unroll
       for (int g = 0; g < kRowGroup; ++g) xl[g] = g < gn ? s_z[g * LD + lane] : 0.0;
@@ -814,26 +824,33 @@ __global__ __launch_bounds__(64) void k_eig64_rows(double* __restrict__ gram, in
     }
     __syncthreads();
   }
+  if (Fb) zero_factor_slots(Fb, p, TP, lane);
 }
 
 template <int TP>
-static int launch_rows(double* gram, long long nb, int T, int N, float* proj, int* rank,
-                       float* weight, float* margin, float* lam_desc, int* info, hipStream_t st) {
+static int launch_rows(double* gram, long long nb, int T, int N, float* proj, float* fac,
+                       int* rank, float* weight, float* margin, float* lam_desc, int* info,
+                       hipStream_t st) {
   hipLaunchKernelGGL((k_eig64_rows<TP>), dim3((unsigned)nb), dim3(64), 0, st, gram, T, N, nb,
-                     proj, rank, weight, margin, lam_desc, info);
+                     proj, fac, rank, weight, margin, lam_desc, info);
   return launch_status("eig64 (register) kernel");
 }
 
-int launch_eig64_mp(double* gram, long long nb, int T, int N, float* proj, int* rank,
+int launch_eig64_mp(double* gram, long long nb, int T, int N, float* proj, float* fac, int* rank,
                     float* weight, float* margin, float* lam_desc, int* info, hipStream_t st) {
-  if (T <= 16) return launch_rows<16>(gram, nb, T, N, proj, rank, weight, margin, lam_desc, info, st);
-  if (T <= 32) return launch_rows<32>(gram, nb, T, N, proj, rank, weight, margin, lam_desc, info, st);
-  if (T <= 48) return launch_rows<48>(gram, nb, T, N, proj, rank, weight, margin, lam_desc, info, st);
+  if (fac && T > kFactorMaxT)
+    return set_err(MPPCA_EUNSUPPORTED, "eig64: factors need T <= %d (T = %d)", kFactorMaxT, T);
+  if (T <= 16)
+    return launch_rows<16>(gram, nb, T, N, proj, fac, rank, weight, margin, lam_desc, info, st);
+  if (T <= 32)
+    return launch_rows<32>(gram, nb, T, N, proj, fac, rank, weight, margin, lam_desc, info, st);
+  if (T <= 48)
+    return launch_rows<48>(gram, nb, T, N, proj, fac, rank, weight, margin, lam_desc, info, st);
   const size_t lds = tri_lds_bytes(T);
   int rc = prepare_lds((const void*)k_eig64_mp, lds);
   if (rc) return rc;
   hipLaunchKernelGGL(k_eig64_mp, dim3((unsigned)nb), dim3(64), lds, st, gram, T, N, nb, proj,
-                     rank, weight, margin, lam_desc, info);
+                     fac, rank, weight, margin, lam_desc, info);
   return launch_status("eig64+mp kernel");
 }
 

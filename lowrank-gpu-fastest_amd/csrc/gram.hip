@@ -58,31 +58,34 @@ __global__ __launch_bounds__(256) void k_gram(const float* __restrict__ xc, S* _
 #pragma unroll
   for (int k = 0; k < NACC; ++k) acc[k] = V4{0, 0, 0, 0};
 
+  // Branch-free loads: every lane reads an in-bounds element (padded voxel
+  // table, frame clamped to T-1); the zero mask for voxels past N and frames
+  // past T is applied when the slice is consumed, one iteration after its
+  // load, so the next slice's loads stay in flight under the MFMAs.
   float a[NTB], an[NTB];
-  auto load = [&](int n, float* dst) {
-    if (n < N) {
-      const float* row = xb + (long long)s_off[n] * T;
+  auto load_raw = [&](int n, float* dst) {  // n < N + 8 <= N + kOffPad
+    const float* row = xb + (long long)s_off[n] * T;
 #pragma unroll
-      for (int tb = 0; tb < NTB; ++tb) {
-        const int t = tb * 16 + i;
-        dst[tb] = t < T ? row[t] : 0.f;
-      }
-    } else {
-#pragma unroll
-      for (int tb = 0; tb < NTB; ++tb) dst[tb] = 0.f;
+    for (int tb = 0; tb < NTB; ++tb) {
+      const int t = tb * 16 + i;
+      dst[tb] = row[t < T ? t : T - 1];
     }
   };
-  load(kq, a);
+  float tmask[NTB];
+#pragma unroll
+  for (int tb = 0; tb < NTB; ++tb) tmask[tb] = tb * 16 + i < T ? 1.f : 0.f;
+  load_raw(kq, an);
   for (int n0 = 0; n0 < N; n0 += 4) {
-    load(n0 + 4 + kq, an);  // next slice in flight while this one multiplies
+    const float nmask = n0 + kq < N ? 1.f : 0.f;
+#pragma unroll
+    for (int tb = 0; tb < NTB; ++tb) a[tb] = an[tb] * (nmask * tmask[tb]);
+    load_raw(n0 + 4 + kq, an);  // next slice in flight while this one multiplies
     int k = 0;
 #pragma unroll
     for (int tb1 = 0; tb1 < NTB; ++tb1)
 #pragma unroll
       for (int tb2 = tb1; tb2 < NTB; ++tb2, ++k)
         acc[k] = mfma_16x16x4((S)a[tb1], (S)a[tb2], acc[k]);
-#pragma unroll
-    for (int tb = 0; tb < NTB; ++tb) a[tb] = an[tb];
   }
 
   S* Gb = gram + bl * (long long)T * T;
@@ -106,7 +109,8 @@ template <int NTB, typename S>
 static int launch_gram_t(const float* xc, S* gram, const Geom& g, long long b0, long long nb,
                          hipStream_t st) {
   const unsigned blocks = (unsigned)((nb + 3) / 4);
-  hipLaunchKernelGGL((k_gram<NTB, S>), dim3(blocks), dim3(256), g.N * sizeof(int), st, xc, gram,
+  hipLaunchKernelGGL((k_gram<NTB, S>), dim3(blocks), dim3(256), (g.N + kOffPad) * sizeof(int), st,
+                     xc, gram,
                      g, b0, nb);
   return launch_status("gram kernel");
 }

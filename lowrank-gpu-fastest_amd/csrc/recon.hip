@@ -44,13 +44,14 @@ __global__ __launch_bounds__(256) void k_recon(const float* __restrict__ xc,
   for (int rb = wave; rb < nrb; rb += 4) {
     const int n = rb * 16 + i;
     float a[KQ];
-    if (n < g.N) {
-      const float* row = xc + (base + s_off[n]) * T + kq * KQ;
+    {  // branch-free: clamped in-bounds loads, zeroed by select
+      const bool nok = n < g.N;  // n < N + 16: the padded table stays in bounds
+      const float* row = xc + (base + s_off[n]) * T;
 #pragma unroll
-      for (int s = 0; s < KQ; ++s) a[s] = (kq * KQ + s < T) ? row[s] : 0.f;
-    } else {
-#pragma unroll
-      for (int s = 0; s < KQ; ++s) a[s] = 0.f;
+      for (int s = 0; s < KQ; ++s) {
+        const int t = kq * KQ + s;
+        a[s] = row[t < T ? t : T - 1] * ((nok && t < T) ? 1.f : 0.f);
+      }
     }
     floatx4 d[NTB];
 #pragma unroll
@@ -78,7 +79,8 @@ __global__ __launch_bounds__(256) void k_recon(const float* __restrict__ xc,
 template <int NTB>
 static int launch_recon_t(const float* xc, const float* proj, const float* weight, float* acc,
                    const Geom& g, long long b0, long long nb, hipStream_t st) {
-  const size_t lds = (size_t)NTB * 16 * NTB * 16 * sizeof(float) + (size_t)g.N * sizeof(int);
+  const size_t lds =
+      (size_t)NTB * 16 * NTB * 16 * sizeof(float) + (size_t)(g.N + kOffPad) * sizeof(int);
   hipLaunchKernelGGL((k_recon<NTB>), dim3((unsigned)nb), dim3(256), lds, st, xc, proj, weight, acc,
                      g, b0, nb);
   return launch_status("recon kernel");
@@ -95,6 +97,248 @@ int launch_recon(const float* xc, const float* proj, const float* weight, float*
     case 6: return launch_recon_t<6>(xc, proj, weight, acc, g, b0, nb, st);
     case 7: return launch_recon_t<7>(xc, proj, weight, acc, g, b0, nb, st);
     default: return launch_recon_t<8>(xc, proj, weight, acc, g, b0, nb, st);
+  }
+}
+
+// Corner indices along one axis whose patch [c, c+p) meets [a0, a0+ext):
+// strided [lo, hi] plus the shifted last corner (index ns) when `extra`.
+__device__ __forceinline__ void touch_range(int a0, int ext, int S, int p, int s, int ns, int nc,
+                                            int* lo, int* hi, bool* extra) {
+  const int a = a0 - p + 1;  // first coordinate a corner may start at
+  *lo = a <= 0 ? 0 : (a + s - 1) / s;
+  const int h = (a0 + ext - 1) / s;
+  *hi = h > ns - 1 ? ns - 1 : h;
+  const int last = S - p;
+  *extra = (nc > ns) && last <= a0 + ext - 1 && last + p - 1 >= a0;
+}
+
+// ---------------------------------------------------------------------------
+// Low-rank reconstruction fused with normalisation: no atomics, no zeroed
+// accumulator, one write of the output.
+//
+// Factors of patch b (written by the eigensolver; layout in common.h): Ut holds
+// the top-p eigenvectors as rows, Uw = w_b * the same vectors as columns, both
+// zero-padded to TF = 16*ceil(T/16) frames and to whole 16-slot tiles.
+//
+// A wave owns SX x 4 x 4 voxels (x, y, z): SX slabs of 16 voxels, row i of a
+// slab being voxel (y0 + i/4, z0 + i%4). The centred rows of its voxels stay in
+// registers. For each patch b that meets the tile and each slab inside b's x
+// range:
+//   C^T = Ut * Xc^T  (KQ MFMAs over frames)  -> lane (i, q) holds
+//                    C[voxel i][slot 4q + r] in register r
+//   C  *= [voxel i in b]
+//   D  += C * Uw^T   (K = slots, taken in the order slot 4q + s at step s, so
+//                    the registers of C are the A operand as they stand; steps
+//                    with s >= ceil(p/4) only meet empty slots and are skipped)
+// D (16 voxels x TF frames per slab) and W (weight sum per voxel) live in
+// registers; the wave writes out = mean + D / W once. Patches are visited in
+// increasing index, so W adds the weights in the order step 0 accumulates wsum
+// (step0_numpy_loop.py:36-39).
+// ---------------------------------------------------------------------------
+template <int NTB, int SX>
+__global__ __launch_bounds__(256) void k_recon_lr(const float* __restrict__ xc,
+                                                  const float* __restrict__ fac,
+                                                  const int* __restrict__ rank,
+                                                  const float* __restrict__ weight,
+                                                  const float* __restrict__ mean,
+                                                  float* __restrict__ out, Geom g, int nbx,
+                                                  int nby, int nbz, int per_xcd) {
+  constexpr int TF = NTB * 16, KQ = TF / 4, KV = KQ / 4;
+  const int T = g.T;
+  // Workgroups are dealt round-robin to the 8 XCDs; XCD k takes the contiguous
+  // run [k*per_xcd, (k+1)*per_xcd) of block tiles, so neighbouring tiles, which
+  // share most of their patches, share one L2.
+  const int bt = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
+  if (bt >= nbx * nby * nbz) return;
+  const int bz = bt % nbz, byx = bt / nbz, by = byx % nby, bx = byx / nby;
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int x0 = bx * SX, y0 = by * 4, z0 = (bz * 4 + wave) * 4;
+  if (z0 >= g.Z) return;
+  const int lane = threadIdx.x & 63, i = lane & 15, kq = lane >> 4;
+  const int ex = min(SX, g.X - x0), ey = min(4, g.Y - y0), ez = min(4, g.Z - z0);
+
+  // centred rows of the tile, B operand of C^T: xv[j][s] = Xc[voxel i of slab j]
+  // [frame kq*KQ + s]; voxels and frames past the volume are clamped in bounds
+  // (their slots and factor rows are zero, so they add nothing)
+  const int yi = y0 + (i >> 2), zi = z0 + (i & 3);
+  float xv[SX][KQ];
+#pragma unroll
+  for (int j = 0; j < SX; ++j) {
+    const int xj = min(x0 + j, g.X - 1);
+    const float* row = xc + (((long long)xj * g.Y + min(yi, g.Y - 1)) * g.Z + min(zi, g.Z - 1)) * T;
+    if ((T & 3) == 0) {
+#pragma unroll
+      for (int v = 0; v < KV; ++v) {
+        int f = kq * KQ + 4 * v;
+        f = f < T ? f : T - 4;
+        const float4 q = *(const float4*)(row + f);
+        xv[j][4 * v] = q.x;
+        xv[j][4 * v + 1] = q.y;
+        xv[j][4 * v + 2] = q.z;
+        xv[j][4 * v + 3] = q.w;
+      }
+    } else {
+#pragma unroll
+      for (int s = 0; s < KQ; ++s) {
+        const int f = kq * KQ + s;
+        xv[j][s] = row[f < T ? f : T - 1];
+      }
+    }
+  }
+
+  floatx4 D[SX][NTB];
+  float Wv[SX][4];
+#pragma unroll
+  for (int j = 0; j < SX; ++j) {
+#pragma unroll
+    for (int tb = 0; tb < NTB; ++tb) D[j][tb] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int r = 0; r < 4; ++r) Wv[j][r] = 0.f;
+  }
+
+  // patches meeting the tile, visited in increasing index b
+  int xl, xh, yl, yh, zl, zh;
+  bool xe, ye, ze;
+  touch_range(x0, ex, g.X, g.px, g.stride, g.nsx, g.ncx, &xl, &xh, &xe);
+  touch_range(y0, ey, g.Y, g.py, g.stride, g.nsy, g.ncy, &yl, &yh, &ye);
+  touch_range(z0, ez, g.Z, g.pz, g.stride, g.nsz, g.ncz, &zl, &zh, &ze);
+  const int nxs = xh >= xl ? xh - xl + 1 : 0, nys = yh >= yl ? yh - yl + 1 : 0,
+            nzs = zh >= zl ? zh - zl + 1 : 0;
+  const int nyl = nys + (ye ? 1 : 0), nzl = nzs + (ze ? 1 : 0);
+  const int npatch = (nxs + (xe ? 1 : 0)) * nyl * nzl;
+  auto patch_of = [&](int q, long long* b, int* cx, int* cy, int* cz) {
+    const int qz = q % nzl, qy = (q / nzl) % nyl, qx = q / (nzl * nyl);
+    const int ix = qx < nxs ? xl + qx : g.nsx;
+    const int iy = qy < nys ? yl + qy : g.nsy;
+    const int iz = qz < nzs ? zl + qz : g.nsz;
+    *b = ((long long)ix * g.ncy + iy) * g.ncz + iz;
+    *cx = corner_of(ix, g.nsx, g.X, g.px, g.stride);
+    *cy = corner_of(iy, g.nsy, g.Y, g.py, g.stride);
+    *cz = corner_of(iz, g.nsz, g.Z, g.pz, g.stride);
+  };
+  // factor fragments of slot tile nc: ut[s] = Ut[slot i][frame kq*KQ + s],
+  // uw[tb*4 + s] = Uw[frame tb*16 + i][slot 4*kq + s]
+  auto load_fac = [&](long long b, int nc, float* ut, float* uw) {
+    const float* F = fac + b * (2LL * TF * TF);
+    const float* U = F + nc * 16 * TF + i * TF + kq * KQ;
+#pragma unroll
+    for (int v = 0; v < KV; ++v) {
+      const float4 q = *(const float4*)(U + 4 * v);
+      ut[4 * v] = q.x;
+      ut[4 * v + 1] = q.y;
+      ut[4 * v + 2] = q.z;
+      ut[4 * v + 3] = q.w;
+    }
+    const float* Wp = F + TF * TF + nc * 16 * TF + i * 16 + 4 * kq;
+#pragma unroll
+    for (int tb = 0; tb < NTB; ++tb) {
+      const float4 q = *(const float4*)(Wp + tb * 256);
+      uw[tb * 4] = q.x;
+      uw[tb * 4 + 1] = q.y;
+      uw[tb * 4 + 2] = q.z;
+      uw[tb * 4 + 3] = q.w;
+    }
+  };
+
+  float ut[KQ], uw[4 * NTB], utn[KQ], uwn[4 * NTB];
+  long long bn = 0;
+  int cxn = 0, cyn = 0, czn = 0, pn = 0;
+  float wn = 0.f;
+  if (npatch > 0) {
+    patch_of(0, &bn, &cxn, &cyn, &czn);
+    pn = min(max(rank[bn], 0), T);  // a rank above T would index past the factor block
+    wn = weight[bn];
+    load_fac(bn, 0, utn, uwn);
+  }
+  for (int q = 0; q < npatch; ++q) {
+    const long long b = bn;
+    const int cx = cxn, cy = cyn, cz = czn, p = pn;
+    const float w = wn;
+#pragma unroll
+    for (int k = 0; k < KQ; ++k) ut[k] = utn[k];
+#pragma unroll
+    for (int k = 0; k < 4 * NTB; ++k) uw[k] = uwn[k];
+    if (q + 1 < npatch) {  // next patch's factors in flight under this one's MFMAs
+      patch_of(q + 1, &bn, &cxn, &cyn, &czn);
+      pn = min(max(rank[bn], 0), T);
+      wn = weight[bn];
+      load_fac(bn, 0, utn, uwn);
+    }
+    const int jlo = max(cx - x0, 0), jhi = min(cx + g.px - x0, SX);
+    // membership of row voxel i (C mask) and of output voxel (kq, r) (W)
+    const float mrow = (yi >= cy && yi < cy + g.py && zi >= cz && zi < cz + g.pz) ? 1.f : 0.f;
+    const bool yin = y0 + kq >= cy && y0 + kq < cy + g.py;
+#pragma unroll
+    for (int j = 0; j < SX; ++j) {
+      if (j < jlo || j >= jhi) continue;
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        Wv[j][r] += (yin && z0 + r >= cz && z0 + r < cz + g.pz) ? w : 0.f;
+    }
+    const int nct = (p + 15) >> 4;
+    for (int nc = 0; nc < nct; ++nc) {
+      if (nc > 0) load_fac(b, nc, ut, uw);  // rank above 16: further slot tiles
+      const int qn = min(16, p - 16 * nc), S2 = (qn + 3) >> 2;
+#pragma unroll
+      for (int j = 0; j < SX; ++j) {
+        if (j < jlo || j >= jhi) continue;
+        floatx4 ct = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int s = 0; s < KQ; ++s) ct = __builtin_amdgcn_mfma_f32_16x16x4f32(ut[s], xv[j][s], ct, 0, 0, 0);
+        ct *= mrow;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+          if (s >= S2) break;
+#pragma unroll
+          for (int tb = 0; tb < NTB; ++tb)
+            D[j][tb] = __builtin_amdgcn_mfma_f32_16x16x4f32(ct[s], uw[tb * 4 + s], D[j][tb], 0, 0, 0);
+        }
+      }
+    }
+  }
+
+  // out = mean + D / W (0 where no patch covers the voxel), as k_normalize
+#pragma unroll
+  for (int j = 0; j < SX; ++j) {
+    if (j >= ex) break;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      if (kq >= ey || r >= ez) continue;
+      const long long vox = ((long long)(x0 + j) * g.Y + (y0 + kq)) * g.Z + (z0 + r);
+      const float W = Wv[j][r];
+      const float m = mean[vox];
+      float* orow = out + vox * T;
+#pragma unroll
+      for (int tb = 0; tb < NTB; ++tb) {
+        const int t = tb * 16 + i;
+        if (t < T) orow[t] = W > 0.f ? m + D[j][tb][r] / W : 0.f;
+      }
+    }
+  }
+}
+
+template <int NTB, int SX>
+static int launch_recon_lr_t(const float* xc, const float* fac, const int* rank,
+                             const float* weight, const float* mean, float* out, const Geom& g,
+                             hipStream_t st) {
+  const int nbx = (g.X + SX - 1) / SX, nby = (g.Y + 3) / 4, nbz = (g.Z + 15) / 16;
+  const long long nbt = (long long)nbx * nby * nbz;
+  if (nbt > (1LL << 30)) return set_err(MPPCA_EUNSUPPORTED, "recon: volume too large");
+  const int per_xcd = (int)((nbt + 7) / 8);
+  hipLaunchKernelGGL((k_recon_lr<NTB, SX>), dim3((unsigned)(8 * per_xcd)), dim3(256), 0, st, xc,
+                     fac, rank, weight, mean, out, g, nbx, nby, nbz, per_xcd);
+  return launch_status("recon (factors) kernel");
+}
+
+int launch_recon_factors(const float* xc, const float* fac, const int* rank, const float* weight,
+                         const float* mean, float* out, const Geom& g, hipStream_t st) {
+  switch (ntb_of(g.T)) {
+    case 1: return launch_recon_lr_t<1, 2>(xc, fac, rank, weight, mean, out, g, st);
+    case 2: return launch_recon_lr_t<2, 2>(xc, fac, rank, weight, mean, out, g, st);
+    case 3: return launch_recon_lr_t<3, 2>(xc, fac, rank, weight, mean, out, g, st);
+    case 4: return launch_recon_lr_t<4, 2>(xc, fac, rank, weight, mean, out, g, st);
+    default:
+      return set_err(MPPCA_EUNSUPPORTED, "recon (factors): T = %d above 64", g.T);
   }
 }
 

@@ -27,6 +27,7 @@ from .geometry import check_geometry, num_patches
 #          (matches step 0 to ~1e-7 on any input); the default.
 # "fp32":  the reference's all-FP32 arithmetic (FP32 MFMA Gram, FP32 Jacobi).
 FORMS = ("plain", "fp32")
+FACTOR_MAX_T = 64
 
 
 class Denoiser:
@@ -62,6 +63,9 @@ class Denoiser:
         self._ws_bytes = ws
         self.vol_d = torch.empty(self.shape, dtype=torch.float32, device=dev)
         self.out_d = torch.empty(self.shape, dtype=torch.float32, device=dev)
+        # plain form, T <= 64: eigensolver -> rank-p factors -> fused recon + normalise;
+        # otherwise projector -> atomic scatter -> normalise
+        self.route = "factors" if form == "plain" and self.shape[3] <= FACTOR_MAX_T else "projector"
         self.rank = torch.empty(self.n_patches, dtype=torch.int32, device=dev)
         self.margin = torch.empty(self.n_patches, dtype=torch.float32, device=dev)
         self.sweeps = torch.empty(self.n_patches, dtype=torch.int32, device=dev)
@@ -94,7 +98,11 @@ class Denoiser:
         return out_d
 
     # -- device -> device, one C-ABI call per stage ---------------------------
-    STAGES = ("centre", "gram", "eigh_mp", "zero", "recon", "normalize")
+    @property
+    def STAGES(self):
+        if self.route == "factors":
+            return ("centre", "gram", "eigh_mp", "recon")
+        return ("centre", "gram", "eigh_mp", "zero", "recon", "normalize")
 
     def run_staged(self, vol_d, out_d, mark=None, max_sweeps=20):
         """Same kernels as :meth:`run_device`, launched stage by stage.
@@ -115,9 +123,15 @@ class Denoiser:
                 "xc": torch.empty(self.shape, dtype=torch.float32, device=dev),
                 "mean": torch.empty(self.shape[:3], dtype=torch.float32, device=dev),
                 "gram": torch.empty((P, t, t), dtype=gdt, device=dev),
-                "proj": torch.empty((P, t, t), dtype=torch.float32, device=dev),
                 "weight": torch.empty(P, dtype=torch.float32, device=dev),
             }
+            if self.route == "factors":
+                nf = int(self.lib.mppca_factor_floats(t))
+                self.stage_buffers["factors"] = torch.empty((P, nf), dtype=torch.float32,
+                                                            device=dev)
+            else:
+                self.stage_buffers["proj"] = torch.empty((P, t, t), dtype=torch.float32,
+                                                         device=dev)
         sb = self.stage_buffers
         st = torch.cuda.current_stream(self.device).cuda_stream
         L, chk, g = self.lib, _lib.check, self._geom_args()
@@ -131,6 +145,18 @@ class Denoiser:
             chk(L.mppca_gram64(sb["xc"].data_ptr(), sb["gram"].data_ptr(), *g, 0, P, st),
                 "mppca_gram64")
             mark("eigh_mp")
+            if self.route == "factors":
+                chk(L.mppca_eig64_factors(sb["gram"].data_ptr(), P, t, n_vox,
+                                          sb["factors"].data_ptr(), self.rank.data_ptr(),
+                                          sb["weight"].data_ptr(), self.margin.data_ptr(), None,
+                                          self.sweeps.data_ptr(), st), "mppca_eig64_factors")
+                mark("recon")
+                chk(L.mppca_recon_factors(sb["xc"].data_ptr(), sb["factors"].data_ptr(),
+                                          self.rank.data_ptr(), sb["weight"].data_ptr(),
+                                          sb["mean"].data_ptr(), out_d.data_ptr(), *g, st),
+                    "mppca_recon_factors")
+                mark(None)
+                return out_d
             chk(L.mppca_eig64_mp(sb["gram"].data_ptr(), P, t, n_vox, sb["proj"].data_ptr(),
                                  self.rank.data_ptr(), sb["weight"].data_ptr(),
                                  self.margin.data_ptr(), None, self.sweeps.data_ptr(), st),

@@ -53,13 +53,37 @@ def test_workspace_bytes(lib):
     ws = lib.mppca_workspace_bytes(x, y, z, t, 11, 11, 11, 3, FP32)
     assert 4 * (nvox * t + nvox + p * t * t + 4 * p) <= ws
     assert ws < 4 * (nvox * t + nvox + p * t * t + 4 * p) + 256 * 16
-    ws = lib.mppca_workspace_bytes(x, y, z, t, 11, 11, 11, 3, PLAIN)  # + FP64 Gram
-    assert 4 * (nvox * t + nvox + 3 * p * t * t + 4 * p) <= ws
-    assert ws < 4 * (nvox * t + nvox + 3 * p * t * t + 4 * p) + 256 * 16
+    # plain, T <= 64: FP64 Gram + rank-p factor blocks (2 * TF^2 floats, TF = 48 here)
+    ws = lib.mppca_workspace_bytes(x, y, z, t, 11, 11, 11, 3, PLAIN)
+    assert lib.mppca_factor_floats(t) == 2 * 48 * 48
+    assert 4 * (nvox * t + nvox + 2 * p * t * t + 2 * p * 48 * 48 + 4 * p) <= ws
+    assert ws < 4 * (nvox * t + nvox + 2 * p * t * t + 2 * p * 48 * 48 + 4 * p) + 256 * 16
+    # plain, T > 64: FP64 Gram + float32 projectors
+    nv, pp, tt = 16 * 16 * 16, 6 ** 3, 96
+    ws = lib.mppca_workspace_bytes(16, 16, 16, tt, 6, 6, 6, 2, PLAIN)
+    assert 4 * (nv * tt + nv + 3 * pp * tt * tt + 4 * pp) <= ws
+    assert ws < 4 * (nv * tt + nv + 3 * pp * tt * tt + 4 * pp) + 256 * 16
     assert lib.mppca_workspace_bytes(x, y, z, t, 11, 11, 11, 3, 7) < 0  # unknown form
     assert lib.mppca_workspace_bytes(x, y, z, 129, 11, 11, 11, 3, 0) < 0  # T > 128
     assert lib.mppca_workspace_bytes(x, y, z, 1, 11, 11, 11, 3, 0) < 0   # T < 2
     assert lib.mppca_workspace_bytes(40, 40, 40, 48, 17, 17, 17, 3, 0) < 0  # N > 4096
+
+
+def test_factor_entry_points_validate(lib):
+    assert lib.mppca_factor_floats(17) == 2 * 32 * 32
+    assert lib.mppca_factor_floats(64) == 2 * 64 * 64
+    assert lib.mppca_factor_floats(65) < 0 and lib.mppca_factor_floats(1) < 0
+    assert lib.mppca_eig64_factors(None, 4, 48, 100, None, None, None, None, None, None,
+                                   None) < 0
+    assert lib.mppca_eig64_factors(16, 4, 80, 100, 16, 16, 16, None, None, None, None) == -2
+    assert lib.mppca_eig64_factors(16, 4, 48, 100, 18, 16, 16, None, None, None, None) == -1
+    assert b"aligned" in lib.mppca_last_error()
+    assert lib.mppca_recon_factors(None, None, None, None, None, None, 16, 16, 16, 48,
+                                   5, 5, 5, 2, None) < 0
+    assert lib.mppca_recon_factors(16, 16, 16, 16, 16, 32, 16, 16, 16, 96,
+                                   5, 5, 5, 2, None) == -2
+    assert lib.mppca_recon_factors(16, 16, 16, 16, 16, 16, 16, 16, 16, 48,
+                                   5, 5, 5, 2, None) == -1  # out aliases xc
 
 
 def test_host_side_validation_launches_nothing(lib):

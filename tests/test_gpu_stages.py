@@ -236,6 +236,92 @@ def test_recon_and_normalize_vs_fp64(pkg, lib):
     ref_out = mean[..., None] + ref_acc / wsum[..., None]
     np.testing.assert_allclose(out.cpu().numpy(), ref_out, rtol=0,
                                atol=4 * EPS32 * (np.abs(ref_out).max() + scale))
+    # fused factor form on P_b = q q^T: same result, no zeroed accumulator needed
+    fused = torch.full_like(acc, float("nan"))
+    fac = _dev(_factor_blocks([q[b].astype(np.float32) for b in range(P)], wts, t))
+    rk = _dev(np.full(P, 3, np.int32))
+    pkg._lib.check(lib.mppca_recon_factors(dxc.data_ptr(), fac.data_ptr(), rk.data_ptr(),
+                                           dw.data_ptr(), dm.data_ptr(), fused.data_ptr(), *shape,
+                                           *patch, stride, _stream()), "recon_factors")
+    _sync()
+    np.testing.assert_allclose(fused.cpu().numpy(), ref_out, rtol=0,
+                               atol=4 * EPS32 * (np.abs(ref_out).max() + scale))
+
+
+def _factor_blocks(vecs, wts, t):
+    """Factor blocks in the layout mppca_eig64_factors writes (include/mppca.h):
+    TF = 16*ceil(T/16); Ut[nc][c][t] at nc*16*TF + c*TF + t, Uw[nc][t][c] at
+    TF*TF + nc*16*TF + t*16 + c; vector g of a tile of q vectors in slot
+    4*(g // S2) + g % S2 with S2 = ceil(q/4)."""
+    tf = 16 * ((t + 15) // 16)
+    F = np.zeros((len(vecs), 2 * tf * tf), np.float32)
+    for b, V in enumerate(vecs):
+        p = V.shape[1]
+        for g in range(p):
+            nc, g16 = divmod(g, 16)
+            q = min(16, p - 16 * nc)
+            s2 = (q + 3) // 4
+            c = 4 * (g16 // s2) + g16 % s2
+            F[b, nc * 16 * tf + c * tf + np.arange(t)] = V[:, g]
+            F[b, tf * tf + nc * 16 * tf + np.arange(t) * 16 + c] = (
+                np.float64(wts[b]) * V[:, g].astype(np.float64)).astype(np.float32)
+    return F
+
+
+@pytest.mark.parametrize("shape,patch,stride", [
+    ((21, 19, 37, 48), (11, 11, 11), 3),   # tiles cut by the volume edge on every axis
+    ((9, 10, 13, 16), (3, 4, 5), 2),
+    ((12, 9, 8, 10), (4, 3, 5), 5),        # gaps: uncovered voxels must come out 0
+    ((7, 7, 7, 64), (7, 7, 7), 1),         # one patch
+    ((11, 9, 10, 33), (5, 4, 6), 2),       # T not a multiple of 4: scalar loads
+    ((10, 12, 9, 40), (6, 6, 6), 2),       # ranks up to 20: two slot tiles
+])
+def test_recon_factors_shapes(pkg, lib, shape, patch, stride):
+    """Fused factor reconstruction + normalisation against float64 NumPy, with
+    ranks 0 .. min(T, 20) (empty patches, partial slot groups, two slot tiles)."""
+    torch = _torch()
+    rng = np.random.default_rng(12)
+    t = shape[3]
+    xc = rng.standard_normal(shape).astype(np.float32)
+    mean = rng.standard_normal(shape[:3]).astype(np.float32)
+    P = pkg.num_patches(shape, patch, stride)
+    ranks = rng.integers(0, min(t, 20) + 1, P).astype(np.int32)
+    vecs = [np.linalg.qr(rng.standard_normal((t, t)))[0][:, :r].astype(np.float32)
+            for r in ranks]
+    wts = (1.0 / (2.0 + ranks)).astype(np.float32)
+    out = torch.full(shape, float("nan"), dtype=torch.float32, device="cuda")
+    dxc, dw, dm = _dev(xc), _dev(wts), _dev(mean)
+    fac, rk = _dev(_factor_blocks(vecs, wts, t)), _dev(ranks)
+    assert lib.mppca_factor_floats(t) == fac.shape[1]
+    pkg._lib.check(lib.mppca_recon_factors(dxc.data_ptr(), fac.data_ptr(), rk.data_ptr(),
+                                           dw.data_ptr(), dm.data_ptr(), out.data_ptr(), *shape,
+                                           *patch, stride, _stream()), "recon_factors")
+    _sync()
+    acc = np.zeros(shape)
+    wsum = np.zeros(shape[:3])
+    px, py, pz = patch
+    for b, (i, j, k) in enumerate(pkg.patch_corners(shape, patch, stride)):
+        sl = np.s_[i:i + px, j:j + py, k:k + pz]
+        V = vecs[b].astype(np.float64)
+        acc[sl] += wts[b] * (xc[sl].astype(np.float64) @ V @ V.T)
+        wsum[sl] += wts[b]
+    with np.errstate(invalid="ignore", divide="ignore"):
+        ref = np.where(wsum[..., None] > 0, mean[..., None] + acc / wsum[..., None], 0.0)
+    got = out.cpu().numpy()
+    assert np.isfinite(got).all()
+    np.testing.assert_allclose(got, ref, rtol=0, atol=1e-5 * (np.abs(ref).max() + 1))
+
+
+def test_recon_factors_rejects_large_t(pkg, lib):
+    torch = _torch()
+    shape = (6, 6, 6, 96)
+    d = torch.zeros(shape, dtype=torch.float32, device="cuda")
+    m = torch.zeros(shape[:3], dtype=torch.float32, device="cuda")
+    assert lib.mppca_factor_floats(96) < 0
+    rc = lib.mppca_recon_factors(d.data_ptr(), d.data_ptr(), d.data_ptr(), d.data_ptr(),
+                                 m.data_ptr(), m.data_ptr(), *shape, 3, 3, 3, 2, _stream())
+    assert rc == -2  # MPPCA_EUNSUPPORTED
+    assert b"64" in lib.mppca_last_error()
 
 
 def test_gram64_exact(pkg, lib):
@@ -325,3 +411,50 @@ def test_eig64_degenerate(pkg, lib):
     assert rank[1] == 3 and info[1] == 0
     np.testing.assert_allclose(proj[1], q[:, :3] @ q[:, :3].T, rtol=0, atol=1e-6)
     assert info[2] == -2 and rank[2] == 0
+
+
+@pytest.mark.parametrize("n_vox,t", [(1331, 48), (125, 16), (343, 30), (60, 17), (40, 64),
+                                     (200, 40)])
+def test_eig64_factors_match_projector(pkg, lib, n_vox, t):
+    """mppca_eig64_factors: same rank / weight as mppca_eig64_mp, sum of the
+    slot outer products equals the projector, Uw = w * Ut^T, empty slots and
+    padding frames exactly zero. Ranks up to 24 cover two slot tiles."""
+    torch = _torch()
+    rng = np.random.default_rng(21)
+    b = 24
+    X = np.empty((b, n_vox, t))
+    for i in range(b):
+        r = int(rng.integers(0, min(24, t - 2) + 1))
+        sig = rng.random((n_vox, r)) @ rng.standard_normal((r, t)) * rng.uniform(0.5, 3.0)
+        X[i] = sig + 0.01 * rng.standard_normal((n_vox, t))
+    X -= X.mean(axis=2, keepdims=True)
+    G = np.einsum("bnt,bns->bts", X, X)
+    proj, rank, w, _, _, info = _eig64(pkg, lib, G, n_vox)
+    dg = _dev(G)
+    nf = lib.mppca_factor_floats(t)
+    fac = torch.full((b, nf), float("nan"), dtype=torch.float32, device="cuda")
+    rank2 = torch.empty(b, dtype=torch.int32, device="cuda")
+    w2 = torch.empty(b, dtype=torch.float32, device="cuda")
+    pkg._lib.check(lib.mppca_eig64_factors(dg.data_ptr(), b, t, n_vox, fac.data_ptr(),
+                                           rank2.data_ptr(), w2.data_ptr(), None, None, None,
+                                           _stream()), "eig64_factors")
+    _sync()
+    np.testing.assert_array_equal(rank2.cpu().numpy(), rank)
+    np.testing.assert_array_equal(w2.cpu().numpy(), w)
+    if t >= 30:
+        assert rank.max() > 16
+    F = fac.cpu().numpy()
+    tf = 16 * ((t + 15) // 16)
+    for i in range(b):
+        p = int(rank[i])
+        nct = (p + 15) // 16
+        ut = F[i, :nct * 16 * tf].reshape(nct, 16, tf)
+        uw = F[i, tf * tf:tf * tf + nct * 16 * tf].reshape(nct, tf, 16)
+        assert np.isfinite(ut).all() and np.isfinite(uw).all()
+        assert np.all(ut[:, :, t:] == 0) and np.all(uw[:, t:, :] == 0)
+        U = ut.reshape(nct * 16, tf)[:, :t].astype(np.float64)
+        np.testing.assert_allclose(U.T @ U, proj[i], rtol=0, atol=1e-6)
+        filled = np.abs(U).sum(1) > 0
+        assert filled.sum() == p
+        np.testing.assert_allclose(np.swapaxes(uw, 1, 2).reshape(nct * 16, tf)[:, :t],
+                                   np.float64(w[i]) * U, rtol=2 * EPS32, atol=0)
