@@ -4,10 +4,18 @@ Workload: float32 volume 96x96x48x48 (seed 1, rank-6 signal, maps smoothed at
 sigma 4 voxels, noise N(0, 0.03^2)), 11^3 patches, stride 3 -> 12,600 patches.
 A step is one denoise of that volume with the input already resident in HBM
 (``value`` / ``ms_per_step``). The host->host time of the reference's harness
-(upload + denoise + download, common.py:129-143) and its p50/p99 over the
-same number of calls are reported beside it, never as ``value``.
+(upload + denoise + download, common.py:129-143) and its p50/p99 over a fixed
+number of back-to-back calls are reported beside it, never as ``value``.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+                    [--no-traffic]
+
+Per-stage kernel times come from HIP events on the launch stream: each stage is
+queued behind a spacer fill, so its start event fires as the stage's kernel
+starts and its end event as it ends. HBM traffic per launch comes from two
+``rocprofv3 --pmc`` child runs (FETCH_SIZE, then WRITE_SIZE), started before
+this process touches the GPU, each scaled by the same counter read on a device
+copy of known size.
 
 Prints ONE JSON line.
 """
@@ -15,10 +23,16 @@ Prints ONE JSON line.
 from __future__ import annotations
 
 import argparse
+import csv
+import glob
 import json
 import os
 import platform
+import shutil
+import signal
+import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -30,27 +44,48 @@ import __graft_entry__  # noqa: E402
 
 METRIC = ("ms/volume host->host (96x96x48x48, 11³, stride 3) + p50/p99 latency "
           "at fixed load")
-PEAK_HBM_GBS = 8000.0
-PEAK_FP32_TFLOPS = 157.3
+PEAK = {"hbm": (8000.0, "GB/s"), "fp32": (157.3, "TFLOP/s"), "fp64": (78.6, "TFLOP/s")}
+# kernel-symbol prefix of each stage, as rocprofv3 reports it
+STAGE_KERNEL = {"centre": "k_centre", "gram": "k_gram", "eigh_mp": "k_eig", "recon": "k_recon",
+                "normalize": "k_normalize", "zero": "k_zero"}
+COPY_BYTES = 256 * 2 ** 20
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def kernel_work(shape, patch, n_patches):
-    """Algorithmic work per launch of each kernel (bytes or flops)."""
+def stage_work(stage, shape, patch, n_patches, ranks, form, route):
+    """(bound, algorithmic amount per launch, peak key) of one stage.
+
+    centre    bytes: read vol, write xc and mean                4*(2*V*T + V)
+    gram      flops: symmetric rank-N update per patch (syrk)   N*T*(T+1) * P
+    eigh_mp   flops: tridiagonalisation + p back-transformed
+                     vectors per patch (LAPACK counts)          sum_b 4/3*T^3 + 2*T^2*p_b
+    recon     flops: factors: (Xc V_p) V_p^T per patch           sum_b 4*N*T*p_b
+                     projector: Xc P_b per patch                 2*N*T^2 * P
+    zero / normalize bytes: output write / read acc, write out.
+    """
     x, y, z, t = shape
     nvox = x * y * z
     n = int(np.prod(patch))
-    return {
-        "centre": ("hbm", 4 * (2 * nvox * t + nvox)),              # read vol, write xc + mean
-        "gram": ("mfma", 2 * n * t * t * n_patches),               # 2 N T^2 per patch
-        "recon": ("mfma", 2 * n * t * t * n_patches),              # dense X_c P_b per patch
-        "normalize": ("hbm", 4 * (2 * nvox * t + nvox) + 4 * n_patches),
-        "zero": ("hbm", 4 * nvox * t),
-        "eigh_mp": (None, None),                                    # VALU / LDS bound
-    }
+    fp = "fp64" if form == "plain" else "fp32"
+    psum = float(np.sum(ranks))
+    if stage == "centre":
+        return "hbm", 4.0 * (2 * nvox * t + nvox), "hbm"
+    if stage == "gram":
+        return "mfma", float(n) * t * (t + 1) * n_patches, fp
+    if stage == "eigh_mp":
+        return "mfma", 4.0 / 3.0 * t ** 3 * n_patches + 2.0 * t * t * psum, fp
+    if stage == "recon":
+        if route == "factors":
+            return "mfma", 4.0 * n * t * psum, "fp32"
+        return "mfma", 2.0 * n * t * t * n_patches, "fp32"
+    if stage == "zero":
+        return "hbm", 4.0 * nvox * t, "hbm"
+    if stage == "normalize":
+        return "hbm", 4.0 * (2 * nvox * t + nvox) + 4.0 * n_patches, "hbm"
+    raise KeyError(stage)
 
 
 def cpu_baseline(vol, patch, stride, budget_s=15.0):
@@ -99,6 +134,127 @@ def cpu_baseline(vol, patch, stride, budget_s=15.0):
     }
 
 
+# ---- HBM traffic: rocprofv3 --pmc child runs ---------------------------------
+def pmc_child(args):
+    """Body of a profiled child run: two denoises, then an elementwise kernel
+    that reads and writes COPY_BYTES (the calibration kernel, the run's last
+    dispatch; torch's copy_ may become a DMA copy with no kernel)."""
+    import torch
+
+    pkg = __graft_entry__.load_package()
+    vol, _, patch, stride = pkg.workload_volume(args.workload)
+    d = pkg.get_denoiser(vol.shape, patch, stride, form=args.form)
+    v = torch.from_numpy(vol).cuda()
+    o = torch.empty_like(v)
+    d.run_device(v, o)
+    d.run_device(v, o)
+    a = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.neg(a, out=b)
+    torch.neg(b, out=a)
+    torch.cuda.synchronize()
+    print("pmc child done", flush=True)
+
+
+def _read_counter_csv(path):
+    """({kernel name: [value per dispatch, in dispatch order]}, value of the
+    run's last dispatch)."""
+    rows = {}
+    last = (-1, None)
+    with open(path, newline="") as f:
+        for r in csv.DictReader(f):
+            did, val = int(r["Dispatch_Id"]), float(r["Counter_Value"])
+            rows.setdefault(r["Kernel_Name"], []).append((did, val))
+            if did > last[0]:
+                last = (did, val)
+    return {k: [v for _, v in sorted(vs)] for k, vs in rows.items()}, last[1]
+
+
+def measure_traffic(args, timeout_s=150):
+    """HBM bytes per launch of each stage kernel (last dispatch), scaled by the
+    copy calibration; None (and a reason) when rocprofv3 is unavailable."""
+    exe = shutil.which("rocprofv3")
+    if exe is None:
+        return None, "rocprofv3 not on PATH"
+    per_counter = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        out_dir = tempfile.mkdtemp(prefix="mppca_pmc_")
+        cmd = [exe, "--pmc", counter, "--output-format", "csv", "-d", out_dir, "-o", "pmc", "--",
+               sys.executable, os.path.abspath(__file__), "--pmc-child",
+               "--workload", args.workload, "--form", args.form]
+        log(f"traffic pass {counter}")
+        env = dict(os.environ, TMPDIR="/tmp")
+        proc = subprocess.Popen(cmd, cwd="/tmp", env=env, stdout=subprocess.PIPE,
+                                stderr=subprocess.STDOUT, start_new_session=True)
+        try:
+            out, _ = proc.communicate(timeout=timeout_s)
+        except subprocess.TimeoutExpired:
+            os.killpg(proc.pid, signal.SIGKILL)
+            proc.communicate()
+            return None, f"rocprofv3 {counter} pass timed out"
+        files = glob.glob(os.path.join(out_dir, "**", "*counter_collection.csv"), recursive=True)
+        if proc.returncode != 0 and not files:
+            tail = out.decode(errors="replace").strip().splitlines()[-3:]
+            return None, f"rocprofv3 {counter} pass failed ({proc.returncode}): {tail}"
+        if not files:
+            return None, f"rocprofv3 {counter} pass wrote no counter file"
+        per_counter[counter] = _read_counter_csv(files[0])
+        shutil.rmtree(out_dir, ignore_errors=True)
+
+    def find(table, key):
+        names = [k for k in table if "mppca" in k and key in k]
+        return table[names[0]][-1] if names else None
+
+    (fetch, cf), (write, cw) = per_counter["FETCH_SIZE"], per_counter["WRITE_SIZE"]
+    if not cf or not cw:
+        return None, "calibration kernel reads no traffic"
+    kib = COPY_BYTES / 1024.0
+    scale_f, scale_w = kib / cf, kib / cw
+    res = {"calibration": {"copy_bytes": COPY_BYTES, "fetch_kib": cf, "write_kib": cw,
+                           "fetch_scale": round(scale_f, 4), "write_scale": round(scale_w, 4)}}
+    for stage, sym in STAGE_KERNEL.items():
+        f, w = find(fetch, sym), find(write, sym)
+        if f is None or w is None:
+            continue
+        res[stage] = {"fetch_bytes": f * scale_f * 1024.0, "write_bytes": w * scale_w * 1024.0}
+    return res, None
+
+
+# ---- per-stage HIP-event timing -------------------------------------------------
+def stage_times(d, vol_d, out_d, reps):
+    """Mean duration (ms) of each stage kernel over ``reps`` staged passes.
+
+    Before each stage the queue is drained and a spacer fill is enqueued, then
+    the start event; the stage's launch is queued while the spacer runs, so the
+    events bracket the stage kernel alone (no launch latency in the interval).
+    """
+    import torch
+
+    spacer = torch.empty(16 * 2 ** 20, dtype=torch.float32, device=vol_d.device)
+    acc = {}
+    for _ in range(reps):
+        evs = []
+
+        def mark(name, evs=evs):
+            if evs and evs[-1][2] is None:
+                e = torch.cuda.Event(enable_timing=True)
+                e.record()
+                evs[-1][2] = e
+            if name is None:
+                return
+            torch.cuda.synchronize()
+            spacer.fill_(1.0)
+            e = torch.cuda.Event(enable_timing=True)
+            e.record()
+            evs.append([name, e, None])
+
+        d.run_staged(vol_d, out_d, mark=mark)
+        torch.cuda.synchronize()
+        for name, e0, e1 in evs:
+            acc.setdefault(name, []).append(e0.elapsed_time(e1))
+    return {n: float(np.mean(v)) for n, v in acc.items()}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -108,12 +264,25 @@ def main():
     ap.add_argument("--form", default="plain", choices=("plain", "fp32"))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-traffic", action="store_true")
+    ap.add_argument("--latency-calls", type=int, default=30)
+    ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.pmc_child:
+        pmc_child(args)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
               "sharding of this path is a design note, not an implemented mode.",
               file=sys.stderr)
         sys.exit(2)
+
+    # the profiled children run before this process touches the GPU
+    traffic, traffic_note = (None, "skipped (--no-traffic)")
+    if not args.no_traffic:
+        traffic, traffic_note = measure_traffic(args)
+        if traffic is None:
+            log(f"traffic: {traffic_note}")
 
     import torch
 
@@ -127,7 +296,7 @@ def main():
     out_d = torch.empty_like(vol_d)
     K, W = args.steps, args.warmup
 
-    log(f"{shape} patch {patch} stride {stride}: {P} patches; warmup {W}")
+    log(f"{shape} patch {patch} stride {stride}: {P} patches, route {d.route}; warmup {W}")
     for _ in range(W):
         d.run_device(vol_d, out_d)
     torch.cuda.synchronize()
@@ -139,45 +308,41 @@ def main():
     torch.cuda.synchronize()
     dev_ms = (time.perf_counter() - t0) * 1e3 / K
     log(f"device-resident: {dev_ms:.3f} ms/volume")
+    info = d.info()
 
-    # ---- per-kernel durations with events on the launch stream --------------
-    names = list(d.STAGES)
-    acc = {n: 0.0 for n in names}
-    for _ in range(K):
-        evs = []
-
-        def mark(name, evs=evs):
-            e = torch.cuda.Event(enable_timing=True)
-            e.record()
-            evs.append((name, e))
-
-        d.run_staged(vol_d, out_d, mark=mark)
-        torch.cuda.synchronize()
-        for (n, e0), (_, e1) in zip(evs[:-1], evs[1:]):
-            acc[n] += e0.elapsed_time(e1)
-    kernel_ms = {n: acc[n] / K for n in names}
+    # ---- per-stage kernel durations, roofline of the dominant kernel --------
+    kernel_ms = stage_times(d, vol_d, out_d, max(K, 5))
     log("per-kernel ms: " + ", ".join(f"{n} {v:.3f}" for n, v in kernel_ms.items()))
-    work = kernel_work(shape, patch, P)
-    rooflined = {n: v for n, v in kernel_ms.items() if work[n][0] is not None}
-    dom_all = max(kernel_ms, key=kernel_ms.get)
-    dom = max(rooflined, key=rooflined.get)
-    bound, amount = work[dom]
+    dom = max(kernel_ms, key=kernel_ms.get)
+    bound, amount, peak_key = stage_work(dom, shape, patch, P, info["rank"], args.form, d.route)
+    peak, unit = PEAK[peak_key]
     sec = kernel_ms[dom] * 1e-3
-    if bound == "hbm":
-        achieved, peak, unit = amount / sec / 1e9, PEAK_HBM_GBS, "GB/s"
-    else:
-        achieved, peak, unit = amount / sec / 1e12, PEAK_FP32_TFLOPS, "TFLOP/s"
+    achieved = amount / sec / (1e9 if unit == "GB/s" else 1e12)
+    roofline = {"kernel": dom, "bound": bound, "achieved": round(achieved, 3), "peak": peak,
+                "unit": unit, "frac": round(achieved / peak, 4), "traffic": None,
+                "work_per_launch": amount, "peak_of": peak_key}
+    kernels = {}
+    for n, ms in kernel_ms.items():
+        b, a, pk = stage_work(n, shape, patch, P, info["rank"], args.form, d.route)
+        pv, u = PEAK[pk]
+        ach = a / (ms * 1e-3) / (1e9 if u == "GB/s" else 1e12)
+        kernels[n] = {"ms": round(ms, 4), "bound": b, "achieved": round(ach, 3), "unit": u,
+                      "frac": round(ach / pv, 4)}
+        if traffic and n in traffic:
+            kernels[n]["hbm_bytes"] = round(traffic[n]["fetch_bytes"] + traffic[n]["write_bytes"])
+    if traffic and dom in traffic:
+        roofline["traffic"] = round(traffic[dom]["fetch_bytes"] + traffic[dom]["write_bytes"])
 
     # ---- host -> host, the reference harness's timing ------------------------
+    res = d(vol)  # first call allocates the pinned staging buffers
     lat = []
-    for _ in range(max(K, 10)):
+    for _ in range(max(args.latency_calls, K)):
         t0 = time.perf_counter()
         res = d(vol)
         lat.append((time.perf_counter() - t0) * 1e3)
     lat = np.array(lat)
     log(f"host->host: best {lat.min():.2f} p50 {np.percentile(lat, 50):.2f} "
         f"p99 {np.percentile(lat, 99):.2f} ms")
-    info = d.info()
 
     line = {
         "metric": METRIC,
@@ -190,23 +355,24 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f32" if args.form == "fp32" else "f32+f64",
         "data": "synthetic (rank-6 + noise generator, seed 1)",
         "config": {"workload": "96x96x48x48 float32, 11^3 patches, stride 3, 12600 patches",
-                   "form": args.form, "input": "HBM-resident"},
+                   "form": args.form, "route": d.route, "input": "HBM-resident"},
         "volumes_per_s": round(1e3 / dev_ms, 2),
         "host_to_host_ms": {"best": round(float(lat.min()), 3),
                             "p50": round(float(np.percentile(lat, 50)), 3),
                             "p99": round(float(np.percentile(lat, 99)), 3),
                             "calls": int(len(lat))},
-        "kernel_ms": {n: round(v, 4) for n, v in kernel_ms.items()},
-        "dominant_kernel": dom_all,
-        "roofline": {"kernel": dom, "bound": bound, "achieved": round(achieved, 2),
-                     "peak": peak, "unit": unit, "frac": round(achieved / peak, 4),
-                     "traffic": None},
+        "kernels": kernels,
+        "roofline": roofline,
+        "traffic_note": traffic_note if traffic is None else
+        ("rocprofv3 FETCH_SIZE/WRITE_SIZE, last dispatch, scaled by a "
+         f"{COPY_BYTES >> 20} MiB elementwise read+write (fetch x{traffic['calibration']['fetch_scale']}, "
+         f"write x{traffic['calibration']['write_scale']})"),
         "ranks": {"min": int(info["rank"].min()), "max": int(info["rank"].max()),
-                  "margin_below_bound": int((info["margin"] < 1).sum()),
-                  "max_sweeps": int(info["sweeps"].max())},
+                  "mean": round(float(info["rank"].mean()), 3),
+                  "margin_below_bound": int((info["margin"] < 1).sum())},
     }
     if not args.no_cpu_baseline:
         log(f"cpu baseline (step-0 oracle, 1 thread, ~{args.cpu_budget:.0f} s)")

@@ -1,0 +1,72 @@
+"""The reference-side binding shown in INTEGRATION.md, run as written.
+
+The stub module is extracted from the document and executed in a fresh
+process without torch (numpy + ctypes + libamdhip64 + libmppca only), with a
+stand-in for the reference's ``common`` module that supplies the geometry
+globals (common.py:27-29). Its output must be bit-identical to the package's
+``denoise``: both enqueue the same kernels on the same inputs, and the path
+has no atomics.
+"""
+
+import os
+import pathlib
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = pathlib.Path(__file__).resolve().parents[1]
+
+COMMON_STANDIN = '''
+PATCH = (5, 5, 5)
+STEP = 2
+
+
+def run(name, denoise):
+    raise SystemExit("the harness is not exercised here")
+'''
+
+RUNNER = '''
+import sys
+
+import numpy as np
+
+import step9_mi355x
+
+vol = np.load(sys.argv[1])
+np.save(sys.argv[2], step9_mi355x.denoise(vol))
+np.save(sys.argv[3], step9_mi355x.denoise(vol))  # second call reuses the device buffers
+print("torch loaded:", "torch" in sys.modules, flush=True)
+'''
+
+
+def _stub_source():
+    text = (ROOT / "INTEGRATION.md").read_text()
+    m = re.search(r"<!-- stub:step9_mi355x.py -->\s*```python\n(.*?)```", text, re.S)
+    assert m, "INTEGRATION.md lost its step9_mi355x.py block"
+    return m.group(1)
+
+
+def test_reference_side_binding(pkg, tmp_path):
+    rng = np.random.default_rng(3)
+    shape = (14, 13, 12, 10)
+    maps = rng.random(shape[:3] + (3,))
+    curves = rng.standard_normal((3, shape[3]))
+    vol = (maps @ curves + 0.05 * rng.standard_normal(shape)).astype(np.float32)
+    (tmp_path / "step9_mi355x.py").write_text(_stub_source())
+    (tmp_path / "common.py").write_text(COMMON_STANDIN)
+    (tmp_path / "run.py").write_text(RUNNER)
+    np.save(tmp_path / "in.npy", vol)
+    env = dict(os.environ, MPPCA_LIB=str(ROOT / "lowrank-gpu-fastest_amd" / "libmppca.so"))
+    print("running the INTEGRATION.md stub on", shape, flush=True)
+    r = subprocess.run([sys.executable, "run.py", "in.npy", "out1.npy", "out2.npy"],
+                       cwd=tmp_path, env=env, timeout=120, capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert "torch loaded: False" in r.stdout
+    want = pkg.denoise(vol, patch=(5, 5, 5), stride=2)
+    np.testing.assert_array_equal(np.load(tmp_path / "out1.npy"), want)
+    np.testing.assert_array_equal(np.load(tmp_path / "out2.npy"), want)
