@@ -466,6 +466,66 @@ __device__ __forceinline__ int sturm_count3(const double* ds, const double* e2s,
   return c;
 }
 
+// Register-resident forms for the register kernel. dr[i] holds the scaled
+// diagonal (padded with 4 from row T on), er[i] the squared scaled coupling of
+// rows i and i+1 (0 from row T-1 on). A padded row multiplies the minor by
+// 4 - x > 0 (|x| <= 1 + slack) and couples to nothing, so it never changes a
+// sign. The loops are unrolled over TP: no register is indexed at run time.
+template <int TP>
+__device__ __forceinline__ int sturm_count_reg(const double (&dr)[TP], const double (&er)[TP],
+                                               double x) {
+  constexpr double kTiny = 1e-290;
+  double pm = 1.0;
+  double pc = dr[0] - x;
+  if (pc == 0.0) pc = -kTiny;
+  int c = pc < 0.0 ? 1 : 0;
+#pragma unroll
+  for (int i = 1; i < TP; ++i) {
+    double pn = fma(dr[i] - x, pc, -er[i - 1] * pm);
+    if (pn == 0.0) pn = pc < 0.0 ? kTiny : -kTiny;
+    c += (pn < 0.0) != (pc < 0.0) ? 1 : 0;
+    pm = pc;
+    pc = pn;
+    if ((i & 7) == 0) {  // frexp exponent: 0 for a zero argument
+      const int ex = __builtin_amdgcn_frexp_exp(fmax(fabs(pc), fabs(pm)));
+      pc = __builtin_amdgcn_ldexp(pc, -ex);
+      pm = __builtin_amdgcn_ldexp(pm, -ex);
+    }
+  }
+  return c;
+}
+
+// Newton correction f(x) / f'(x) of the characteristic polynomial
+// f = det(T_T - x I) of the leading T rows, from the minor recurrence over all
+// TP rows and its derivative dp_i = (d_i - x) dp_{i-1} - e_{i-1}^2 dp_{i-2} -
+// p_{i-1}, the four values rescaled together (the ratio is unchanged). The
+// k = TP - T padded rows make P = f (4 - x)^k, so f / f' = P / (P' + k P / (4 - x)).
+// NaN when the derivative vanishes.
+template <int TP>
+__device__ __forceinline__ double newton_corr_reg(const double (&dr)[TP], const double (&er)[TP],
+                                                  int T, double x) {
+  double pm = 1.0, pc = dr[0] - x, dpm = 0.0, dpc = -1.0;
+#pragma unroll
+  for (int i = 1; i < TP; ++i) {
+    const double t = dr[i] - x;
+    const double pn = fma(t, pc, -er[i - 1] * pm);
+    const double dpn = fma(t, dpc, fma(-er[i - 1], dpm, -pc));
+    pm = pc;
+    pc = pn;
+    dpm = dpc;
+    dpc = dpn;
+    if ((i & 7) == 0) {
+      const int ex = __builtin_amdgcn_frexp_exp(
+          fmax(fmax(fabs(pc), fabs(pm)), fmax(fabs(dpc), fabs(dpm))));
+      pc = __builtin_amdgcn_ldexp(pc, -ex);
+      pm = __builtin_amdgcn_ldexp(pm, -ex);
+      dpc = __builtin_amdgcn_ldexp(dpc, -ex);
+      dpm = __builtin_amdgcn_ldexp(dpm, -ex);
+    }
+  }
+  return pc / (dpc + (double)(TP - T) * pc / (4.0 - x));
+}
+
 // MP rule of mp_rule (common.h) evaluated with every lane: the float32
 // eigenvalue scaling and the float32 suffix sums keep step 0's sequential order
 // (lane 0), then lane i evaluates inequality i. Same results as mp_rule.
@@ -656,20 +716,42 @@ __global__ __launch_bounds__(64) void k_eig64_rows(double* __restrict__ gram, in
     s_e2s[lane] = s_e2[lane] * (inv * inv);
   }
   __syncthreads();
-  if (lane < T) {
-    const int kk = T - 1 - lane;  // kk-th smallest
-    const double tol = 4.0 * u;   // of the scaled problem, |T| <= 1
-    double lo = lo_g * inv, hi = hi_g * inv;
-    for (int it = 0; it < 96 && hi - lo > tol; ++it) {
-      const double x = 0.5 * (lo + hi);
-      if (sturm_count3(s_ds, s_e2s, T, x) > kk)
-        hi = x;
-      else
-        lo = x;
+  {
+    // the scaled tridiagonal in registers (the rows of A are dead by now)
+    double dr[TP], er[TP];
+#pragma unroll
+    for (int i = 0; i < TP; ++i) {
+      dr[i] = i < T ? s_ds[i] : 4.0;
+      er[i] = i + 1 < T ? s_e2s[i] : 0.0;
     }
-    const double lam = 0.5 * (lo + hi) * bsc;
-    s_lam[lane] = lam;
-    s_lamf[lane] = (float)lam;
+    if (lane < T) {
+      // Bisection to 2^-33 of the scaled norm brackets each eigenvalue (the MP
+      // rule needs ~2^-24 relative to lambda_max); two Newton steps on the
+      // characteristic polynomial, clamped to the bracket (a root near its edge
+      // makes the first step overshoot), then reach full precision for every
+      // eigenvalue separated from its neighbours. Copies of a cluster share
+      // one bracket and so take identical steps (the cluster test below relies
+      // on that); a vanishing derivative keeps the current point.
+      const int kk = T - 1 - lane;  // kk-th smallest
+      const double tol = 0x1p-33;   // of the scaled problem, |T| <= 1
+      double lo = lo_g * inv, hi = hi_g * inv;
+      for (int it = 0; it < 96 && hi - lo > tol; ++it) {
+        const double x = 0.5 * (lo + hi);
+        if (sturm_count_reg<TP>(dr, er, x) > kk)
+          hi = x;
+        else
+          lo = x;
+      }
+      double x = 0.5 * (lo + hi);
+#pragma unroll
+      for (int st = 0; st < 2; ++st) {
+        const double xn = x - newton_corr_reg<TP>(dr, er, T, x);
+        x = isfinite(xn) ? fmin(fmax(xn, lo), hi) : x;
+      }
+      const double lam = x * bsc;
+      s_lam[lane] = lam;
+      s_lamf[lane] = (float)lam;
+    }
   }
   __syncthreads();
   if (lam_desc)
