@@ -128,6 +128,21 @@ __device__ __forceinline__ double wave_max(double x) {
               fmax(lane_read64(x, 32), lane_read64(x, 48)));
 }
 
+// Value of x held by lane `src` (any lane pattern; through the LDS crossbar,
+// no LDS memory).
+__device__ __forceinline__ double bperm64(double x, int src) {
+  const int lo = __builtin_amdgcn_ds_bpermute(src << 2, __double2loint(x));
+  const int hi = __builtin_amdgcn_ds_bpermute(src << 2, __double2hiint(x));
+  return __hiloint2double(hi, lo);
+}
+// Sum over the 8 consecutive lanes of an aligned group (every lane gets it).
+__device__ __forceinline__ double group8_sum(double x) {
+  x += dpp64<kDppQuadXor1>(x);
+  x += dpp64<kDppQuadXor2>(x);
+  x += dpp64<kDppRowHalfMirror>(x);
+  return x;
+}
+
 // 1/q to full precision: hardware estimate plus two Newton steps.
 __device__ __forceinline__ double rcp64(double q) {
   double r = __builtin_amdgcn_rcp(q);
@@ -583,8 +598,7 @@ __global__ __launch_bounds__(64) void k_eig64_rows(double* __restrict__ gram, in
                                                    float* __restrict__ lam_desc,
                                                    int* __restrict__ info) {
   constexpr int LD = TP + 1;
-  __shared__ double s_bc[TP];  // published row
-  __shared__ double s_v[TP], s_w[TP];
+  constexpr int R = TP / 8;  // register rows / columns per lane of the 8 x 8 lane grid
   __shared__ double s_d[TP], s_e[TP], s_e2[TP], s_tau[TP];
   __shared__ double s_lam[TP];
   __shared__ double s_dm[kRowGroup * LD];
@@ -597,15 +611,20 @@ __global__ __launch_bounds__(64) void k_eig64_rows(double* __restrict__ gram, in
   if (b >= nb) return;
   const int lane = threadIdx.x;
   const bool row_ok = lane < T;
+  const int gr = lane >> 3, gc = lane & 7;  // lane-grid row and column
   double* Gb = gram + b * (long long)T * T;
-  double a[TP];
+  // lane (gr, gc) holds A[gr + 8m][gc + 8n], m, n < R (zero past T)
+  double a[R][R];
   bool finite = true;
 #pragma unroll
-  for (int j = 0; j < TP; ++j) {
-    a[j] = (row_ok && j < T) ? Gb[lane * T + j] : 0.0;
-    finite = finite && isfinite(a[j]);
-  }
-  const bool all_finite = __any(!finite) == 0;  // also: every row is in registers
+  for (int m = 0; m < R; ++m)
+#pragma unroll
+    for (int n = 0; n < R; ++n) {
+      const int i = gr + 8 * m, j = gc + 8 * n;
+      a[m][n] = (i < T && j < T) ? Gb[i * T + j] : 0.0;
+      finite = finite && isfinite(a[m][n]);
+    }
+  const bool all_finite = __any(!finite) == 0;  // also: the whole matrix is in registers
   float* Pb = proj ? proj + b * (long long)T * T : nullptr;
   float* Fb = fac ? fac + b * (2LL * TP * TP) : nullptr;  // TP = 16*ceil(T/16)
   if (!all_finite) {
@@ -623,62 +642,95 @@ __global__ __launch_bounds__(64) void k_eig64_rows(double* __restrict__ gram, in
   }
 
   // ---- 1. tridiagonalisation ------------------------------------------------
-  for (int k = 0; k + 2 < T; ++k) {
-    if (lane == k) {
+  // Step k = 8 mb + r0: row k is register row mb of lane-grid row r0. The
+  // Householder vector v (v_{k+1} = 1) and w = p - K v reach the lanes that
+  // hold the matching rows and columns through ds_bpermute; A v is summed over
+  // the 8 lanes of a grid row with DPP. Register blocks above mb are finished
+  // (their v and w entries vanish) and are skipped at compile time.
 #pragma unroll
-      for (int j = 0; j < TP; ++j) s_bc[j] = a[j];
-    }
-    __syncthreads();
-    const double x = (lane > k && lane < TP) ? s_bc[lane] : 0.0;
-    const double alpha = s_bc[k + 1];
-    const double ss = wave_sum(lane > k + 1 ? x * x : 0.0);
-    double tk = 0.0, beta = alpha, sc = 0.0;
-    if (ss > 0.0) {
-      const double mu = sqrt(alpha * alpha + ss);
-      beta = alpha >= 0.0 ? -mu : mu;
-      tk = (beta - alpha) / beta;
-      sc = 1.0 / (alpha - beta);
-    }
-    const double v = lane == k + 1 ? 1.0 : (lane > k + 1 ? x * sc : 0.0);
-    if (lane < TP) s_v[lane] = v;
-    if (row_ok) Gb[k * T + lane] = v;  // reflector k, zero for indices <= k
-    if (lane == 0) {
-      s_d[k] = s_bc[k];
-      s_e[k] = beta;
-      s_tau[k] = tk;
-    }
-    __syncthreads();
-    if (tk != 0.0) {
-      double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
+  for (int mb = 0; mb < R; ++mb) {
+    for (int r0 = 0; r0 < 8; ++r0) {
+      const int k = 8 * mb + r0;
+      if (k + 2 >= T) break;
+      const double dk = lane_read64(a[mb][mb], 9 * r0);  // A[k][k]
+      const int nb1 = mb + 1 < R ? mb + 1 : mb;  // column k+1 = 8(mb+1) when r0 = 7
+      const double asrc = r0 == 7 ? a[mb][nb1] : a[mb][mb];
+      const double alpha = lane_read64(asrc, 8 * r0 + ((r0 + 1) & 7));  // A[k][k+1]
+      double part = 0.0;
 #pragma unroll
-      for (int j = 0; j < TP; ++j) {
-        const double vj = s_v[j];
-        if ((j & 3) == 0) p0 = fma(a[j], vj, p0);
-        if ((j & 3) == 1) p1 = fma(a[j], vj, p1);
-        if ((j & 3) == 2) p2 = fma(a[j], vj, p2);
-        if ((j & 3) == 3) p3 = fma(a[j], vj, p3);
+      for (int n = mb; n < R; ++n) {
+        const double xv = a[mb][n];
+        part = gc + 8 * n > k + 1 ? fma(xv, xv, part) : part;
       }
-      const double pl = lane > k ? tk * ((p0 + p1) + (p2 + p3)) : 0.0;
-      const double K = 0.5 * tk * wave_sum(pl * v);
-      const double w = pl - K * v;
-      if (lane < TP) s_w[lane] = w;
-      __syncthreads();
+      const double ss = lane_read64(group8_sum(part), 8 * r0);
+      double tk = 0.0, beta = alpha, sc = 0.0;
+      if (ss > 0.0) {
+        const double mu = sqrt(alpha * alpha + ss);
+        beta = alpha >= 0.0 ? -mu : mu;
+        tk = (beta - alpha) / beta;
+        sc = 1.0 / (alpha - beta);
+      }
+      // v at this lane's columns gc + 8n and rows gr + 8m (row k of A lives in
+      // grid row r0: column j in grid column j % 8, register column j / 8)
+      double vc[R], vr[R];
 #pragma unroll
-      for (int j = 0; j < TP; ++j) a[j] = fma(-v, s_w[j], fma(-w, s_v[j], a[j]));
-    }
-    __syncthreads();
-  }
-  // last 2x2 block: rows T-2 and T-1 (T is runtime: select out of the row)
-  if (lane == T - 2 || lane == T - 1) {
-    double dd = 0.0, ee = 0.0;
+      for (int n = 0; n < R; ++n) {
+        vc[n] = 0.0;
+        vr[n] = 0.0;
+      }
 #pragma unroll
-    for (int j = 0; j < TP; ++j) {
-      if (j == lane) dd = a[j];
-      if (lane == T - 1 && j == T - 2) ee = a[j];
+      for (int n = mb; n < R; ++n) {
+        const double xc = bperm64(a[mb][n], 8 * r0 + gc);
+        const double xr = bperm64(a[mb][n], 8 * r0 + gr);
+        const int j = gc + 8 * n, i = gr + 8 * n;
+        vc[n] = j == k + 1 ? 1.0 : (j > k + 1 ? xc * sc : 0.0);
+        vr[n] = i == k + 1 ? 1.0 : (i > k + 1 ? xr * sc : 0.0);
+      }
+      if (gr == r0) {  // reflector k, zero for indices <= k
+#pragma unroll
+        for (int n = 0; n < R; ++n)
+          if (gc + 8 * n < T) Gb[k * T + gc + 8 * n] = vc[n];
+      }
+      if (lane == 0) {
+        s_d[k] = dk;
+        s_e[k] = beta;
+        s_tau[k] = tk;
+      }
+      if (tk != 0.0) {
+        double pl[R], part2 = 0.0;
+#pragma unroll
+        for (int m = mb; m < R; ++m) {
+          double acc = 0.0;
+#pragma unroll
+          for (int n = mb; n < R; ++n) acc = fma(a[m][n], vc[n], acc);
+          acc = group8_sum(acc);
+          pl[m] = gr + 8 * m > k ? tk * acc : 0.0;
+          part2 = fma(pl[m], vr[m], part2);
+        }
+        const double K = 0.5 * tk * wave_sum(gc == 0 ? part2 : 0.0);  // each row once
+        double wr[R], wc[R];
+#pragma unroll
+        for (int m = mb; m < R; ++m) wr[m] = pl[m] - K * vr[m];
+#pragma unroll
+        for (int n = mb; n < R; ++n) wc[n] = bperm64(wr[n], 8 * gc);  // row gc + 8n: grid row gc
+#pragma unroll
+        for (int m = mb; m < R; ++m)
+#pragma unroll
+          for (int n = mb; n < R; ++n)
+            a[m][n] = fma(-vr[m], wc[n], fma(-wr[m], vc[n], a[m][n]));
+      }
     }
-    s_d[lane] = dd;
-    if (lane == T - 1) s_e[T - 2] = ee;
   }
+  // last 2x2 block: rows T-2 and T-1
+#pragma unroll
+  for (int m = 0; m < R; ++m)
+#pragma unroll
+    for (int n = 0; n < R; ++n) {
+      const int i = gr + 8 * m, j = gc + 8 * n;
+      if (i == T - 2 && j == T - 2) s_d[T - 2] = a[m][n];
+      if (i == T - 1 && j == T - 1) s_d[T - 1] = a[m][n];
+      if (i == T - 1 && j == T - 2) s_e[T - 2] = a[m][n];
+    }
   if (lane == 0) s_e[T - 1] = 0.0;
   __syncthreads();
   // split the tridiagonal where |e_i| <= 8u (|d_i| + |d_{i+1}|): a backward
@@ -847,46 +899,43 @@ __global__ __launch_bounds__(64) void k_eig64_rows(double* __restrict__ gram, in
       if (lane < T) s_z[a2 * LD + lane] *= sc2;
       __syncthreads();
     }
-    // back-transform x = H_0 (H_1 (... H_{T-3} z)), one lane per vector; the
-    // reflectors come back from global memory kHChunk rows at a time, loaded
-    // by the whole wave (coalesced) into LDS, zero-padded to TP
-    double zr[TP];
+    // back-transform x = H_0 (H_1 (... H_{T-3} z)) on the lane grid: lane
+    // (g, c) holds components c + 8n of vector g, so a reflector costs each
+    // lane R staged entries, R FMAs for its share of the dot product, a sum
+    // over the 8 lanes of the vector and R FMAs for the update. The reflectors
+    // come back from global memory kHChunk rows at a time, loaded by the whole
+    // wave (coalesced) into LDS, zero-padded to TP.
+    double zr[R];
 #pragma unroll
-    for (int j = 0; j < TP; ++j) zr[j] = (lane < gn && j < T) ? s_z[lane * LD + j] : 0.0;
+    for (int n = 0; n < R; ++n)
+      zr[n] = (gr < gn && gc + 8 * n < T) ? s_z[gr * LD + gc + 8 * n] : 0.0;
+    __syncthreads();
     for (int kc = T - 3; kc >= 0; kc -= kHChunk) {
       const int klo = kc - kHChunk + 1 > 0 ? kc - kHChunk + 1 : 0;
       for (int r = 0; r <= kc - klo; ++r)
         for (int c = lane; c < TP; c += 64)
           s_h[r * TP + c] = c < T ? Gb[(klo + r) * T + c] : 0.0;
       __syncthreads();
-      if (lane < gn) {
-        for (int k = kc; k >= klo; --k) {
-          const double tk = s_tau[k];
-          if (tk == 0.0) continue;
-          const double* hk = s_h + (k - klo) * TP;
-          double s0 = 0.0, s1 = 0.0;
+      for (int k = kc; k >= klo; --k) {
+        const double tk = s_tau[k];
+        if (tk == 0.0) continue;
+        const double* hk = s_h + (k - klo) * TP + gc;
+        double h[R], dot = 0.0;
 #pragma unroll
-          for (int j = 0; j < TP; ++j) {
-            if (j & 1)
-              s1 = fma(hk[j], zr[j], s1);
-            else
-              s0 = fma(hk[j], zr[j], s0);
-            if ((j & 7) == 7) __builtin_amdgcn_sched_barrier(0);
-          }
-          const double sk = tk * (s0 + s1);
-#pragma unroll
-          for (int j = 0; j < TP; ++j) {
-            zr[j] = fma(-sk, hk[j], zr[j]);
-            if ((j & 7) == 7) __builtin_amdgcn_sched_barrier(0);
-          }
+        for (int n = 0; n < R; ++n) {
+          h[n] = hk[8 * n];
+          dot = fma(h[n], zr[n], dot);
         }
+        const double sk = tk * group8_sum(dot);
+#pragma unroll
+        for (int n = 0; n < R; ++n) zr[n] = fma(-sk, h[n], zr[n]);
       }
       __syncthreads();
     }
-    if (lane < gn) {
+    if (gr < gn) {
 #pragma unroll
-      for (int j = 0; j < TP; ++j)
-        if (j < T) s_z[lane * LD + j] = zr[j];
+      for (int n = 0; n < R; ++n)
+        if (gc + 8 * n < T) s_z[gr * LD + gc + 8 * n] = zr[n];
     }
     __syncthreads();
     if (Fb) write_factors(Fb, s_z, LD, g0, gn, p, T, TP, 1.0 / (2.0 + (double)p), lane);
