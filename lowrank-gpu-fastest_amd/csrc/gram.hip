@@ -40,15 +40,23 @@ struct Vec4<double> {
   static __device__ __forceinline__ int row(int kq, int r) { return kq + 4 * r; }
 };
 
+// Slices of 4 voxels in flight ahead of the one being multiplied: the loads
+// of slice n0 + 4*kPrefetch + 3 stay inside the kOffPad padding of the table.
+constexpr int kPrefetch = 3;
+static_assert(4 * kPrefetch + 3 < kOffPad, "prefetch past the padded offset table");
+
 template <int NTB, typename S>
 __global__ __launch_bounds__(256) void k_gram(const float* __restrict__ xc, S* __restrict__ gram,
-                                              Geom g, long long b0, long long nb) {
+                                              Geom g, long long b0, long long nb, int per_xcd) {
   typedef typename Vec4<S>::type V4;
   extern __shared__ int s_off[];
   fill_offsets(s_off, g);
   __syncthreads();
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const long long bl = (long long)blockIdx.x * 4 + wave;
+  // workgroups are dealt round-robin to the 8 XCDs; XCD k takes a contiguous
+  // run of patches, so neighbouring patches (which share most voxels) share an L2
+  const long long blk = (long long)(blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+  const long long bl = blk * 4 + wave;
   if (bl >= nb) return;
   const int T = g.T, N = g.N;
   const int i = lane & 15, kq = lane >> 4;
@@ -62,8 +70,8 @@ __global__ __launch_bounds__(256) void k_gram(const float* __restrict__ xc, S* _
   // table, frame clamped to T-1); the zero mask for voxels past N and frames
   // past T is applied when the slice is consumed, one iteration after its
   // load, so the next slice's loads stay in flight under the MFMAs.
-  float a[NTB], an[NTB];
-  auto load_raw = [&](int n, float* dst) {  // n < N + 8 <= N + kOffPad
+  float a[NTB], ring[kPrefetch][NTB];
+  auto load_raw = [&](int n, float* dst) {  // n < N + 4*kPrefetch + 4 <= N + kOffPad
     const float* row = xb + (long long)s_off[n] * T;
 #pragma unroll
     for (int tb = 0; tb < NTB; ++tb) {
@@ -74,12 +82,17 @@ __global__ __launch_bounds__(256) void k_gram(const float* __restrict__ xc, S* _
   float tmask[NTB];
 #pragma unroll
   for (int tb = 0; tb < NTB; ++tb) tmask[tb] = tb * 16 + i < T ? 1.f : 0.f;
-  load_raw(kq, an);
+#pragma unroll
+  for (int d = 0; d < kPrefetch; ++d) load_raw(4 * d + kq, ring[d]);
   for (int n0 = 0; n0 < N; n0 += 4) {
     const float nmask = n0 + kq < N ? 1.f : 0.f;
 #pragma unroll
-    for (int tb = 0; tb < NTB; ++tb) a[tb] = an[tb] * (nmask * tmask[tb]);
-    load_raw(n0 + 4 + kq, an);  // next slice in flight while this one multiplies
+    for (int tb = 0; tb < NTB; ++tb) a[tb] = ring[0][tb] * (nmask * tmask[tb]);
+#pragma unroll
+    for (int d = 0; d + 1 < kPrefetch; ++d)
+#pragma unroll
+      for (int tb = 0; tb < NTB; ++tb) ring[d][tb] = ring[d + 1][tb];
+    load_raw(n0 + 4 * kPrefetch + kq, ring[kPrefetch - 1]);  // in flight under the MFMAs
     int k = 0;
 #pragma unroll
     for (int tb1 = 0; tb1 < NTB; ++tb1)
@@ -108,10 +121,11 @@ __global__ __launch_bounds__(256) void k_gram(const float* __restrict__ xc, S* _
 template <int NTB, typename S>
 static int launch_gram_t(const float* xc, S* gram, const Geom& g, long long b0, long long nb,
                          hipStream_t st) {
-  const unsigned blocks = (unsigned)((nb + 3) / 4);
-  hipLaunchKernelGGL((k_gram<NTB, S>), dim3(blocks), dim3(256), (g.N + kOffPad) * sizeof(int), st,
-                     xc, gram,
-                     g, b0, nb);
+  const long long blocks = (nb + 3) / 4;
+  const long long per_xcd = (blocks + 7) / 8;
+  if (per_xcd > (1LL << 28)) return set_err(MPPCA_EUNSUPPORTED, "gram: too many patches");
+  hipLaunchKernelGGL((k_gram<NTB, S>), dim3((unsigned)(8 * per_xcd)), dim3(256),
+                     (g.N + kOffPad) * sizeof(int), st, xc, gram, g, b0, nb, (int)per_xcd);
   return launch_status("gram kernel");
 }
 
