@@ -206,15 +206,31 @@ __global__ __launch_bounds__(256) void k_recon_lr(const float* __restrict__ xc,
             nzs = zh >= zl ? zh - zl + 1 : 0;
   const int nyl = nys + (ye ? 1 : 0), nzl = nzs + (ze ? 1 : 0);
   const int npatch = (nxs + (xe ? 1 : 0)) * nyl * nzl;
-  auto patch_of = [&](int q, long long* b, int* cx, int* cy, int* cz) {
-    const int qz = q % nzl, qy = (q / nzl) % nyl, qx = q / (nzl * nyl);
-    const int ix = qx < nxs ? xl + qx : g.nsx;
-    const int iy = qy < nys ? yl + qy : g.nsy;
-    const int iz = qz < nzs ? zl + qz : g.nsz;
-    *b = ((long long)ix * g.ncy + iy) * g.ncz + iz;
-    *cx = corner_of(ix, g.nsx, g.X, g.px, g.stride);
-    *cy = corner_of(iy, g.nsy, g.Y, g.py, g.stride);
-    *cz = corner_of(iz, g.nsz, g.Z, g.pz, g.stride);
+  // Patch table, 64 patches at a time: lane l decodes patch q0 + l of a chunk
+  // (index, corners, rank, weight) with vector arithmetic and gathers; the
+  // loop takes each patch's fields with readlane. The next chunk is decoded a
+  // whole chunk ahead, so the rank and weight gathers never stall the loop.
+  // Fields: b; cx | cy << 16; cz | p << 16; w (bits).
+  auto decode = [&](int q, int* eb, int* exy, int* ezp, int* ew) {
+    *eb = 0;
+    *exy = 0;
+    *ezp = 0;
+    *ew = 0;
+    if (q < npatch) {
+      const int qz = q % nzl, qy = (q / nzl) % nyl, qx = q / (nzl * nyl);
+      const int ix = qx < nxs ? xl + qx : g.nsx;
+      const int iy = qy < nys ? yl + qy : g.nsy;
+      const int iz = qz < nzs ? zl + qz : g.nsz;
+      const int b = (ix * g.ncy + iy) * g.ncz + iz;  // P < 2^31 (checked at launch)
+      const int cx = corner_of(ix, g.nsx, g.X, g.px, g.stride);
+      const int cy = corner_of(iy, g.nsy, g.Y, g.py, g.stride);
+      const int cz = corner_of(iz, g.nsz, g.Z, g.pz, g.stride);
+      const int p = min(max(rank[b], 0), T);  // a rank above T would index past the block
+      *eb = b;
+      *exy = cx | (cy << 16);
+      *ezp = cz | (p << 16);
+      *ew = __float_as_int(weight[b]);
+    }
   };
   // factor fragments of slot tile nc: ut[s] = Ut[slot i][frame kq*KQ + s],
   // uw[tb*4 + s] = Uw[frame tb*16 + i][slot 4*kq + s]
@@ -241,28 +257,24 @@ __global__ __launch_bounds__(256) void k_recon_lr(const float* __restrict__ xc,
   };
 
   float ut[KQ], uw[4 * NTB], utn[KQ], uwn[4 * NTB];
-  long long bn = 0;
-  int cxn = 0, cyn = 0, czn = 0, pn = 0;
-  float wn = 0.f;
-  if (npatch > 0) {
-    patch_of(0, &bn, &cxn, &cyn, &czn);
-    pn = min(max(rank[bn], 0), T);  // a rank above T would index past the factor block
-    wn = weight[bn];
-    load_fac(bn, 0, utn, uwn);
-  }
+  int cb, cxy, czp, cw, nbq, nxy, nzp, nw;  // this chunk's table, the next chunk's
+  decode(lane, &cb, &cxy, &czp, &cw);
+  decode(64 + lane, &nbq, &nxy, &nzp, &nw);
+  if (npatch > 0) load_fac(__builtin_amdgcn_readlane(cb, 0), 0, utn, uwn);
   for (int q = 0; q < npatch; ++q) {
-    const long long b = bn;
-    const int cx = cxn, cy = cyn, cz = czn, p = pn;
-    const float w = wn;
+    const int jq = q & 63;
+    const long long b = __builtin_amdgcn_readlane(cb, jq);
+    const int xy = __builtin_amdgcn_readlane(cxy, jq), zp = __builtin_amdgcn_readlane(czp, jq);
+    const float w = __int_as_float(__builtin_amdgcn_readlane(cw, jq));
+    const int cx = xy & 0xFFFF, cy = xy >> 16, cz = zp & 0xFFFF, p = zp >> 16;
 #pragma unroll
     for (int k = 0; k < KQ; ++k) ut[k] = utn[k];
 #pragma unroll
     for (int k = 0; k < 4 * NTB; ++k) uw[k] = uwn[k];
     if (q + 1 < npatch) {  // next patch's factors in flight under this one's MFMAs
-      patch_of(q + 1, &bn, &cxn, &cyn, &czn);
-      pn = min(max(rank[bn], 0), T);
-      wn = weight[bn];
-      load_fac(bn, 0, utn, uwn);
+      const int bq = jq < 63 ? __builtin_amdgcn_readlane(cb, jq + 1)
+                             : __builtin_amdgcn_readlane(nbq, 0);
+      load_fac(bq, 0, utn, uwn);
     }
     const int jlo = max(cx - x0, 0), jhi = min(cx + g.px - x0, SX);
     // membership of row voxel i (C mask) and of output voxel (kq, r) (W)
@@ -295,6 +307,13 @@ __global__ __launch_bounds__(256) void k_recon_lr(const float* __restrict__ xc,
         }
       }
     }
+    if (jq == 63) {  // next chunk becomes current; decode the one after
+      cb = nbq;
+      cxy = nxy;
+      czp = nzp;
+      cw = nw;
+      decode(q + 65 + lane, &nbq, &nxy, &nzp, &nw);
+    }
   }
 
   // out = mean + D / W (0 where no patch covers the voxel), as k_normalize
@@ -325,6 +344,9 @@ static int launch_recon_lr_t(const float* xc, const float* fac, const int* rank,
   const long long nbt = (long long)nbx * nby * nbz;
   if (nbt > (1LL << 30)) return set_err(MPPCA_EUNSUPPORTED, "recon: volume too large");
   const int per_xcd = (int)((nbt + 7) / 8);
+  if (g.P >= (1LL << 31)) return set_err(MPPCA_EUNSUPPORTED, "recon: 2^31 patches or more");
+  if (g.X > 65535 || g.Y > 65535 || g.Z > 65535)
+    return set_err(MPPCA_EUNSUPPORTED, "recon: volume axis above 65535");
   hipLaunchKernelGGL((k_recon_lr<NTB, SX>), dim3((unsigned)(8 * per_xcd)), dim3(256), 0, st, xc,
                      fac, rank, weight, mean, out, g, nbx, nby, nbz, per_xcd);
   return launch_status("recon (factors) kernel");
