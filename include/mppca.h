@@ -207,6 +207,35 @@ int mppca_denoise_device(const float* vol, float* out, int X, int Y, int Z,
                          int32_t* rank, float* margin, int32_t* info,
                          void* stream);
 
+/* Host -> host denoise with the PCIe transfers overlapped with the kernels:
+ * the whole of step8_tuned.denoise (upload, denoise, download;
+ * step8_tuned.py:116-146) as one blocking call. Returns once `out` holds the
+ * result, equal to mppca_denoise_device's (bit for bit on the factor route,
+ * which has no atomics; up to summation order on the projector route).
+ *   vol, out          host buffers (X, Y, Z, T) float32.
+ *   workspace         device, >= mppca_workspace_bytes(...), 256-byte aligned.
+ *   d_vol, d_out      device volume buffers (distinct).
+ *   h_stage_in        pinned host buffer of the volume's size that vol is
+ *                     copied into (host threads, slab by slab) before upload;
+ *                     NULL uploads from vol directly (vol pinned).
+ *   h_stage_out       pinned host buffer the result is downloaded to and then
+ *                     copied from into out; NULL downloads into out (pinned).
+ *   nslabs            x-slabs of the pipeline (>= 1; the plain form with
+ *                     T <= 64 pipelines, other routes run whole-volume).
+ *   nthreads          host copy threads (1..256).
+ *   s_compute, s_in, s_out  three streams (hipStream_t) for the kernels, the
+ *                     uploads and the downloads; distinct non-blocking streams
+ *                     overlap, one stream (or NULL) serialises.
+ *   rank, margin, info  device arrays of P entries, or NULL.
+ * The library creates (and destroys) its events and copy threads per call. */
+int mppca_denoise_host(const float* vol, float* out, int X, int Y, int Z, int T,
+                       int px, int py, int pz, int stride, int form,
+                       void* workspace, int64_t workspace_bytes,
+                       float* d_vol, float* d_out, float* h_stage_in,
+                       float* h_stage_out, int nslabs, int nthreads,
+                       void* s_compute, void* s_in, void* s_out,
+                       int32_t* rank, float* margin, int32_t* info);
+
 #ifdef __cplusplus
 }
 #endif

@@ -37,6 +37,8 @@ SIGNATURES = {
     "mppca_recon_factors": (_i32, [_vp] * 6 + [_i32] * 8 + [_vp]),
     "mppca_workspace_bytes": (_i64, [_i32] * 9),
     "mppca_denoise_device": (_i32, [_vp, _vp] + [_i32] * 9 + [_vp, _i64, _vp, _vp, _vp, _vp]),
+    "mppca_denoise_host": (_i32, [_vp, _vp] + [_i32] * 9 + [_vp, _i64] + [_vp] * 4
+                           + [_i32, _i32] + [_vp] * 6),
 }
 
 

@@ -70,21 +70,6 @@ int make_geom(Geom* g, int X, int Y, int Z, int T, int px, int py, int pz, int s
   return MPPCA_OK;
 }
 
-namespace {
-
-struct Workspace {
-  float* xc;
-  float* mean;
-  void* gram;   // P*T*T doubles (plain form) or floats (fp32 form)
-  float* proj;  // P*T*T floats: the fp32 form (in place of gram) and the plain form above T = 64
-  float* fac;   // P*factor_floats(T) floats: rank-p factors, plain form with T <= 64
-  float* weight;
-  int* rank;
-  float* margin;
-  int* info;
-  size_t bytes;
-};
-
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 Workspace carve_workspace(char* base, const Geom& g, int form) {
@@ -120,7 +105,6 @@ Workspace carve_workspace(char* base, const Geom& g, int form) {
 
 bool valid_form(int form) { return form == MPPCA_FORM_PLAIN || form == MPPCA_FORM_FP32; }
 
-}  // namespace
 }  // namespace mppca
 
 using namespace mppca;
@@ -270,7 +254,8 @@ int mppca_recon_factors(const float* xc, const float* factors, const int32_t* ra
   if ((uintptr_t)factors % 16 != 0 || (uintptr_t)xc % 16 != 0)
     return set_err(MPPCA_EINVAL, "recon_factors: xc and factors must be 16-byte aligned");
   if (out == xc) return set_err(MPPCA_EINVAL, "recon_factors: out must not alias xc");
-  return launch_recon_factors(xc, factors, rank, weight, mean, out, g, (hipStream_t)stream);
+  return launch_recon_factors(xc, factors, rank, weight, mean, out, g, 0, g.X,
+                              (hipStream_t)stream);
 }
 
 int64_t mppca_workspace_bytes(int X, int Y, int Z, int T, int px, int py, int pz, int stride,
@@ -297,7 +282,17 @@ int mppca_denoise_device(const float* vol, float* out, int X, int Y, int Z, int 
   if ((int64_t)w.bytes > workspace_bytes)
     return set_err(MPPCA_ENOSPACE, "denoise: workspace %lld < %zu bytes",
                    (long long)workspace_bytes, w.bytes);
-  hipStream_t st = (hipStream_t)stream;
+  return run_device_path(vol, out, g, form, w, rank, margin, info, (hipStream_t)stream);
+}
+
+}  // extern "C"
+
+namespace mppca {
+
+int run_device_path(const float* vol, float* out, const Geom& g, int form, const Workspace& w,
+                    int* rank, float* margin, int* info, hipStream_t st) {
+  int rc;
+  const int T = g.T;
   int* rk = rank ? rank : w.rank;
   float* mg = margin ? margin : w.margin;
   int* inf = info ? info : w.info;
@@ -307,7 +302,7 @@ int mppca_denoise_device(const float* vol, float* out, int X, int Y, int Z, int 
     if ((rc = launch_eig64_mp((double*)w.gram, g.P, T, g.N, w.proj, w.fac, rk, w.weight, mg,
                               nullptr, inf, st)))
       return rc;
-    if (w.fac) return launch_recon_factors(w.xc, w.fac, rk, w.weight, w.mean, out, g, st);
+    if (w.fac) return launch_recon_factors(w.xc, w.fac, rk, w.weight, w.mean, out, g, 0, g.X, st);
   } else {
     if ((rc = launch_gram(w.xc, (float*)w.gram, g, 0, g.P, st))) return rc;
     if ((rc = launch_eigh_mp((const float*)w.gram, g.P, T, g.N, 20, w.proj, rk, w.weight, mg,
@@ -320,4 +315,4 @@ int mppca_denoise_device(const float* vol, float* out, int X, int Y, int Z, int 
   return launch_normalize(out, w.mean, w.weight, out, g, st);
 }
 
-}  // extern "C"
+}  // namespace mppca

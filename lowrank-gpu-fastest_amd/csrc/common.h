@@ -41,7 +41,7 @@ int make_geom(Geom* g, int X, int Y, int Z, int T, int px, int py, int pz, int s
 
 inline int ntb_of(int T) { return (T + 15) / 16; }
 
-__device__ __forceinline__ int corner_of(int i, int ns, int S, int p, int s) {
+__host__ __device__ __forceinline__ int corner_of(int i, int ns, int S, int p, int s) {
   return i < ns ? i * s : S - p;
 }
 
@@ -189,6 +189,26 @@ __device__ inline void zero_factor_slots(float* F, int p, int TF, int lane) {
 constexpr MpErr kErrFp32Route = {16.0, 0.0};
 constexpr MpErr kErrFp64Route = {1.0, 32.0};
 
+// ---- workspace of the whole path (abi.hip) --------------------------------------
+// One caller buffer, 256-byte aligned, carved in this order.
+struct Workspace {
+  float* xc;
+  float* mean;
+  void* gram;   // P*T*T doubles (plain form) or floats (fp32 form)
+  float* proj;  // P*T*T floats: the fp32 form (in place of gram) and the plain form above T = 64
+  float* fac;   // P*factor_floats(T) floats: rank-p factors, plain form with T <= 64
+  float* weight;
+  int* rank;
+  float* margin;
+  int* info;
+  size_t bytes;
+};
+Workspace carve_workspace(char* base, const Geom& g, int form);  // base null: sizes only
+bool valid_form(int form);
+// Enqueues the whole path on device buffers (mppca_denoise_device after its checks).
+int run_device_path(const float* vol, float* out, const Geom& g, int form, const Workspace& w,
+                    int* rank, float* margin, int* info, hipStream_t st);
+
 // ---- launchers (defined per stage file) ----------------------------------------
 int zero_fill(float* p, long long n, hipStream_t st);
 int launch_centre(const float* vol, float* xc, float* mean, long long nvox, int T, hipStream_t st);
@@ -211,7 +231,11 @@ int launch_eig64_mp(double* gram, long long nb, int T, int N, float* proj, float
                     float* weight, float* margin, float* lam_desc, int* info, hipStream_t st);
 int launch_recon(const float* xc, const float* proj, const float* weight, float* acc,
                  const Geom& g, long long b0, long long nb, hipStream_t st);
+// Output planes x in [x0, x1) (x0 a multiple of recon_factor_tile_x(), x1 too
+// or X); reads the factors of every patch that meets them.
 int launch_recon_factors(const float* xc, const float* fac, const int* rank, const float* weight,
-                         const float* mean, float* out, const Geom& g, hipStream_t st);
+                         const float* mean, float* out, const Geom& g, int x0, int x1,
+                         hipStream_t st);
+int recon_factor_tile_x();
 
 }  // namespace mppca

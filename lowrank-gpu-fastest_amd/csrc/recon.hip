@@ -141,7 +141,7 @@ __global__ __launch_bounds__(256) void k_recon_lr(const float* __restrict__ xc,
                                                   const int* __restrict__ rank,
                                                   const float* __restrict__ weight,
                                                   const float* __restrict__ mean,
-                                                  float* __restrict__ out, Geom g, int nbx,
+                                                  float* __restrict__ out, Geom g, int bx0, int nbx,
                                                   int nby, int nbz, int per_xcd) {
   constexpr int TF = NTB * 16, KQ = TF / 4, KV = KQ / 4;
   const int T = g.T;
@@ -150,7 +150,7 @@ __global__ __launch_bounds__(256) void k_recon_lr(const float* __restrict__ xc,
   // share most of their patches, share one L2.
   const int bt = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
   if (bt >= nbx * nby * nbz) return;
-  const int bz = bt % nbz, byx = bt / nbz, by = byx % nby, bx = byx / nby;
+  const int bz = bt % nbz, byx = bt / nbz, by = byx % nby, bx = bx0 + byx / nby;
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int x0 = bx * SX, y0 = by * 4, z0 = (bz * 4 + wave) * 4;
   if (z0 >= g.Z) return;
@@ -339,8 +339,10 @@ __global__ __launch_bounds__(256) void k_recon_lr(const float* __restrict__ xc,
 template <int NTB, int SX>
 static int launch_recon_lr_t(const float* xc, const float* fac, const int* rank,
                              const float* weight, const float* mean, float* out, const Geom& g,
-                             hipStream_t st) {
-  const int nbx = (g.X + SX - 1) / SX, nby = (g.Y + 3) / 4, nbz = (g.Z + 15) / 16;
+                             int x0, int x1, hipStream_t st) {
+  const int bx0 = x0 / SX, bx1 = (x1 + SX - 1) / SX;
+  const int nbx = bx1 - bx0, nby = (g.Y + 3) / 4, nbz = (g.Z + 15) / 16;
+  if (nbx <= 0) return MPPCA_OK;
   const long long nbt = (long long)nbx * nby * nbz;
   if (nbt > (1LL << 30)) return set_err(MPPCA_EUNSUPPORTED, "recon: volume too large");
   const int per_xcd = (int)((nbt + 7) / 8);
@@ -348,17 +350,23 @@ static int launch_recon_lr_t(const float* xc, const float* fac, const int* rank,
   if (g.X > 65535 || g.Y > 65535 || g.Z > 65535)
     return set_err(MPPCA_EUNSUPPORTED, "recon: volume axis above 65535");
   hipLaunchKernelGGL((k_recon_lr<NTB, SX>), dim3((unsigned)(8 * per_xcd)), dim3(256), 0, st, xc,
-                     fac, rank, weight, mean, out, g, nbx, nby, nbz, per_xcd);
+                     fac, rank, weight, mean, out, g, bx0, nbx, nby, nbz, per_xcd);
   return launch_status("recon (factors) kernel");
 }
 
+int recon_factor_tile_x() { return 2; }
+
 int launch_recon_factors(const float* xc, const float* fac, const int* rank, const float* weight,
-                         const float* mean, float* out, const Geom& g, hipStream_t st) {
+                         const float* mean, float* out, const Geom& g, int x0, int x1,
+                         hipStream_t st) {
+  if (x0 < 0 || x1 > g.X || x0 % recon_factor_tile_x() != 0 ||
+      (x1 != g.X && x1 % recon_factor_tile_x() != 0))
+    return set_err(MPPCA_EINVAL, "recon (factors): x range [%d, %d) not tile aligned", x0, x1);
   switch (ntb_of(g.T)) {
-    case 1: return launch_recon_lr_t<1, 2>(xc, fac, rank, weight, mean, out, g, st);
-    case 2: return launch_recon_lr_t<2, 2>(xc, fac, rank, weight, mean, out, g, st);
-    case 3: return launch_recon_lr_t<3, 2>(xc, fac, rank, weight, mean, out, g, st);
-    case 4: return launch_recon_lr_t<4, 2>(xc, fac, rank, weight, mean, out, g, st);
+    case 1: return launch_recon_lr_t<1, 2>(xc, fac, rank, weight, mean, out, g, x0, x1, st);
+    case 2: return launch_recon_lr_t<2, 2>(xc, fac, rank, weight, mean, out, g, x0, x1, st);
+    case 3: return launch_recon_lr_t<3, 2>(xc, fac, rank, weight, mean, out, g, x0, x1, st);
+    case 4: return launch_recon_lr_t<4, 2>(xc, fac, rank, weight, mean, out, g, x0, x1, st);
     default:
       return set_err(MPPCA_EUNSUPPORTED, "recon (factors): T = %d above 64", g.T);
   }

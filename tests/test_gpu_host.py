@@ -1,0 +1,87 @@
+"""Host -> host pipeline (mppca_denoise_host): bit-identical to the
+device-resident path for every slab count and route, through the C ABI."""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+def _volume(shape, seed, rank=4):
+    rng = np.random.default_rng(seed)
+    maps = rng.random(shape[:3] + (rank,))
+    curves = rng.standard_normal((rank, shape[3]))
+    return (maps @ curves + 0.05 * rng.standard_normal(shape)).astype(np.float32)
+
+
+def _device_result(d, vol):
+    torch = _torch()
+    v = torch.from_numpy(vol).cuda()
+    o = torch.empty_like(v)
+    d.run_device(v, o)
+    torch.cuda.synchronize()
+    return o.cpu().numpy(), d.info()["rank"].copy()
+
+
+@pytest.mark.parametrize("shape,patch,stride,form", [
+    ((40, 22, 18, 24), (5, 5, 5), 2, "plain"),
+    ((33, 20, 16, 16), (7, 6, 5), 3, "plain"),   # odd X, shifted last corners
+    ((24, 12, 12, 12), (3, 3, 3), 5, "plain"),   # stride > patch: uncovered planes
+    ((20, 12, 12, 80), (4, 4, 4), 2, "plain"),   # T > 64: whole-volume route
+    ((24, 16, 16, 16), (5, 5, 5), 2, "fp32"),    # projector route: whole-volume
+])
+def test_host_pipeline_matches_device_path(pkg, shape, patch, stride, form):
+    vol = _volume(shape, 5)
+    d = pkg.Denoiser(shape, patch, stride, form=form)
+    want, want_rank = _device_result(d, vol)
+    for nslabs in (1, 3, 6, shape[0] + 5):
+        d.nslabs = nslabs
+        print("host pipeline", shape, patch, stride, form, "nslabs", nslabs, flush=True)
+        got = d(vol)
+        if d.route == "factors":  # no atomics anywhere: bit for bit
+            np.testing.assert_array_equal(got, want)
+        else:  # float atomics in the scatter: same up to summation order
+            np.testing.assert_allclose(got, want, rtol=0, atol=1e-6 * np.abs(want).max())
+        np.testing.assert_array_equal(d.info()["rank"], want_rank)
+
+
+def test_host_pipeline_pageable_output_through_staging(pkg, lib):
+    torch = _torch()
+    shape, patch, stride = (36, 20, 16, 24), (5, 5, 5), 3
+    vol = _volume(shape, 7)
+    d = pkg.Denoiser(shape, patch, stride)
+    want, _ = _device_result(d, vol)
+    stage_in = torch.empty(shape, dtype=torch.float32, pin_memory=True)
+    stage_out = torch.empty(shape, dtype=torch.float32, pin_memory=True)
+    out = np.full(shape, np.nan, np.float32)  # pageable
+    streams = [torch.cuda.Stream() for _ in range(3)]
+    torch.cuda.synchronize()
+    rc = lib.mppca_denoise_host(
+        vol.ctypes.data, out.ctypes.data, *shape, *patch, stride, 0, d._ws_ptr, d._ws_bytes,
+        d.vol_d.data_ptr(), d.out_d.data_ptr(), stage_in.data_ptr(), stage_out.data_ptr(),
+        4, 3, *(s.cuda_stream for s in streams), None, None, None)
+    pkg._lib.check(rc, "mppca_denoise_host")
+    np.testing.assert_array_equal(out, want)
+
+
+def test_host_pipeline_one_stream(pkg, lib):
+    """All three streams the same (here the null stream): serial, same result."""
+    torch = _torch()
+    shape, patch, stride = (30, 14, 14, 16), (5, 5, 5), 2
+    vol = _volume(shape, 9)
+    d = pkg.Denoiser(shape, patch, stride)
+    want, _ = _device_result(d, vol)
+    pin_in = torch.empty(shape, dtype=torch.float32, pin_memory=True)
+    out = torch.empty(shape, dtype=torch.float32, pin_memory=True)
+    rc = lib.mppca_denoise_host(
+        vol.ctypes.data, out.data_ptr(), *shape, *patch, stride, 0, d._ws_ptr, d._ws_bytes,
+        d.vol_d.data_ptr(), d.out_d.data_ptr(), pin_in.data_ptr(), None, 5, 2,
+        None, None, None, None, None, None)
+    pkg._lib.check(rc, "mppca_denoise_host")
+    np.testing.assert_array_equal(out.numpy(), want)
