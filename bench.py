@@ -266,6 +266,10 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-traffic", action="store_true")
     ap.add_argument("--latency-calls", type=int, default=30)
+    ap.add_argument("--serve-rate", type=float, default=100.0,
+                    help="Poisson arrival rate (volumes/s) of the fixed-load serving run")
+    ap.add_argument("--serve-requests", type=int, default=200)
+    ap.add_argument("--serve-workers", type=int, default=2)
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.pmc_child:
@@ -344,6 +348,33 @@ def main():
     log(f"host->host: best {lat.min():.2f} p50 {np.percentile(lat, 50):.2f} "
         f"p99 {np.percentile(lat, 99):.2f} ms")
 
+    # ---- serving: Poisson arrivals of this volume at a fixed rate -----------
+    srv_res = None
+    if args.serve_requests > 0:
+        with pkg.Server(workers=args.serve_workers) as srv:
+            warm = [srv.submit(vol, patch, stride, keep=False)
+                    for _ in range(2 * args.serve_workers)]
+            for f in warm:
+                f.result()
+            nb = 4 * args.serve_workers * 5
+            t0 = time.perf_counter()
+            burst = [srv.submit(vol, patch, stride, keep=False) for _ in range(nb)]
+            for f in burst:
+                f.result()
+            burst_vps = nb / (time.perf_counter() - t0)
+            r = pkg.poisson_run(srv, [(vol, patch, stride)], rate_hz=args.serve_rate,
+                                n_requests=args.serve_requests, seed=0)
+        srv_res = {"arrivals": "poisson", "rate_hz": args.serve_rate,
+                   "requests": args.serve_requests, "workers": args.serve_workers,
+                   "p50_ms": round(r["latency_ms"]["p50"], 3),
+                   "p99_ms": round(r["latency_ms"]["p99"], 3),
+                   "max_ms": round(r["latency_ms"]["max"], 3),
+                   "volumes_per_s": round(r["volumes_per_s"], 2),
+                   "burst_volumes_per_s": round(burst_vps, 2),
+                   "latency": "arrival to host result, host->host"}
+        log(f"serving at {args.serve_rate:.0f}/s: p50 {srv_res['p50_ms']:.2f} p99 "
+            f"{srv_res['p99_ms']:.2f} ms; burst {burst_vps:.1f} volumes/s")
+
     line = {
         "metric": METRIC,
         "value": round(dev_ms, 4),
@@ -364,6 +395,7 @@ def main():
                             "p50": round(float(np.percentile(lat, 50)), 3),
                             "p99": round(float(np.percentile(lat, 99)), 3),
                             "calls": int(len(lat))},
+        "serving_at_fixed_load": srv_res,
         "kernels": kernels,
         "roofline": roofline,
         "traffic_note": traffic_note if traffic is None else

@@ -14,9 +14,12 @@ from .phantom import (  # noqa: F401
     gaussian_smooth,
     make_lowrank_volume,
     make_phantom,
+    mixed_volume,
     workload_volume,
+    MIXED_GEOMETRIES,
 )
 from .denoise import FORMS, Denoiser, denoise, get_denoiser  # noqa: F401
+from .serve import AdmissionError, Server, device_bytes, poisson_run  # noqa: F401
 
 __all__ = [
     "denoise",
@@ -31,4 +34,10 @@ __all__ = [
     "workload_volume",
     "WORKLOADS",
     "gaussian_smooth",
+    "mixed_volume",
+    "MIXED_GEOMETRIES",
+    "Server",
+    "poisson_run",
+    "device_bytes",
+    "AdmissionError",
 ]

@@ -72,8 +72,8 @@ class Denoiser:
         self.sweeps = torch.empty(self.n_patches, dtype=torch.int32, device=dev)
         self._pin_in = None
         # host -> host pipeline: x-slabs and host copy threads
-        self.nslabs = 6
-        self.nthreads = max(1, min(8, os.cpu_count() or 1))
+        self.nslabs = 4
+        self.nthreads = max(1, min(4, os.cpu_count() or 1))
 
     def _geom_args(self):
         x, y, z, t = self.shape

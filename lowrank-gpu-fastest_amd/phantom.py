@@ -122,3 +122,27 @@ def workload_volume(name: str):
     shape, patch, stride, rank, ss, ns, seed = WORKLOADS[name]
     noisy, clean = make_lowrank_volume(shape, rank, ss, ns, seed)
     return noisy, clean, patch, stride
+
+
+# The six geometries of the mixed-shape and serving workloads:
+# (shape (X, Y, Z, T), patch, stride).
+MIXED_GEOMETRIES = (
+    ((64, 64, 40, 30), (7, 7, 7), 2),
+    ((96, 96, 48, 48), (11, 11, 11), 3),
+    ((128, 128, 60, 100), (9, 9, 9), 3),
+    ((80, 80, 32, 64), (7, 7, 7), 2),
+    ((112, 112, 56, 24), (11, 11, 11), 4),
+    ((48, 48, 48, 128), (5, 5, 5), 2),
+)
+
+
+def mixed_volume(k: int, seed: int):
+    """``(noisy, patch, stride)`` of request ``seed`` on mixed geometry ``k``:
+    the same generator, rank drawn from 4..8, noise sigma from U(0.02, 0.08),
+    maps smoothed at sigma 4 voxels (all drawn from ``seed``)."""
+    shape, patch, stride = MIXED_GEOMETRIES[k]
+    rng = np.random.default_rng(seed)
+    rank = int(rng.integers(4, 9))
+    noise = float(rng.uniform(0.02, 0.08))
+    noisy, _ = make_lowrank_volume(shape, rank, 4.0, noise, seed)
+    return noisy, patch, stride
