@@ -439,16 +439,9 @@ __global__ __launch_bounds__(64) void k_eig64_mp(const double* __restrict__ gram
   if (Fb) zero_factor_slots(Fb, p, factor_tf(T), lane);
 }
 
-// ---------------------------------------------------------------------------
-// Register-resident form for T <= 48 (TP = T rounded up to 16, 32 or 48).
-// Lane l holds row l of A in VGPRs; every loop over a row is unrolled over TP
-// with zero padding, so no register is indexed at run time. Because A stays
-// symmetric, column k of the trailing block is row k, which lane k publishes
-// to LDS once per step. The reflector v_k overwrites row k of the patch's
-// Gram matrix in global memory (the matrix itself is in registers by then):
-// that buffer is clobbered. The top-p eigenvectors are back-transformed at
-// the end, x = H_0 H_1 ... H_{T-3} z, one lane per vector.
-// ---------------------------------------------------------------------------
+// Helpers of the register-resident eigensolver (k_tri_grid / k_eig64_rows
+// below). Every loop over a row is unrolled over TP with zero padding, so no
+// register is indexed at run time.
 
 constexpr int kRowGroup = 8;  // eigenvectors per pass (one lane each)
 constexpr int kHChunk = 8;    // reflectors staged through LDS per back-transform pass
@@ -588,29 +581,25 @@ __device__ void mp_rule_wave(const float* lam, float* e, float* rc, double* dsfx
   *margin_out = (float)wave_min(mg);
 }
 
+// ---------------------------------------------------------------------------
+// Register-resident eigensolver for T <= 48 (TP = T rounded up to 16, 32 or 48),
+// two kernels on the 8 x 8 lane grid of a wavefront: k_tri_grid reduces each
+// matrix to tridiagonal form in registers; k_eig64_rows finds its eigenvalues,
+// MP rank and top-p eigenvectors. The tridiagonalisation runs at the occupancy
+// of its own registers instead of the eigenvalue phase's. Hand-off in the
+// (clobbered) Gram buffer of each matrix:
+//   row k <= T-3   reflector v_k at columns k+1 .. T-1 (v_{k+1} = 1), tau_k
+//                  at column 0 (v_k vanishes there)
+//   row T-2        the diagonal d_0 .. d_{T-1}
+//   row T-1        the off-diagonal e_0 .. e_{T-2}
+// A non-finite input matrix leaves d_0 = NaN and nothing else.
+// ---------------------------------------------------------------------------
 template <int TP>
-__global__ __launch_bounds__(64) void k_eig64_rows(double* __restrict__ gram, int T, int N,
-                                                   long long nb, float* __restrict__ proj,
-                                                   float* __restrict__ fac,
-                                                   int* __restrict__ rank,
-                                                   float* __restrict__ weight,
-                                                   float* __restrict__ margin,
-                                                   float* __restrict__ lam_desc,
-                                                   int* __restrict__ info) {
-  constexpr int LD = TP + 1;
+__global__ __launch_bounds__(64) void k_tri_grid(double* __restrict__ gram, int T, long long nb) {
   constexpr int R = TP / 8;  // register rows / columns per lane of the 8 x 8 lane grid
-  __shared__ double s_d[TP], s_e[TP], s_e2[TP], s_tau[TP];
-  __shared__ double s_lam[TP];
-  __shared__ double s_dm[kRowGroup * LD];
-  __shared__ double s_z[kRowGroup * LD];  // D+ pivots, then the vector
-  __shared__ double s_h[kHChunk * TP];    // staged reflectors
-  __shared__ double s_ds[TP], s_e2s[TP], s_dsfx[TP];
-  __shared__ float s_lamf[TP], s_ef[TP], s_rcf[TP];
-
   const long long b = blockIdx.x;
   if (b >= nb) return;
   const int lane = threadIdx.x;
-  const bool row_ok = lane < T;
   const int gr = lane >> 3, gc = lane & 7;  // lane-grid row and column
   double* Gb = gram + b * (long long)T * T;
   // lane (gr, gc) holds A[gr + 8m][gc + 8n], m, n < R (zero past T)
@@ -624,20 +613,8 @@ __global__ __launch_bounds__(64) void k_eig64_rows(double* __restrict__ gram, in
       a[m][n] = (i < T && j < T) ? Gb[i * T + j] : 0.0;
       finite = finite && isfinite(a[m][n]);
     }
-  const bool all_finite = __any(!finite) == 0;  // also: the whole matrix is in registers
-  float* Pb = proj ? proj + b * (long long)T * T : nullptr;
-  float* Fb = fac ? fac + b * (2LL * TP * TP) : nullptr;  // TP = 16*ceil(T/16)
-  if (!all_finite) {
-    if (Pb)
-      for (int idx = lane; idx < T * T; idx += 64) Pb[idx] = 0.f;
-    if (lane == 0) {
-      rank[b] = 0;
-      weight[b] = 0.5f;
-      if (margin) margin[b] = 0.f;
-      if (info) info[b] = -2;
-    }
-    if (lam_desc)
-      for (int c = lane; c < T; c += 64) lam_desc[b * T + c] = NAN;
+  if (__any(!finite)) {  // the whole matrix is in registers: flag it
+    if (lane == 0) Gb[(T - 2) * T] = NAN;
     return;
   }
 
@@ -686,15 +663,16 @@ __global__ __launch_bounds__(64) void k_eig64_rows(double* __restrict__ gram, in
         vc[n] = j == k + 1 ? 1.0 : (j > k + 1 ? xc * sc : 0.0);
         vr[n] = i == k + 1 ? 1.0 : (i > k + 1 ? xr * sc : 0.0);
       }
-      if (gr == r0) {  // reflector k, zero for indices <= k
+      if (gr == r0) {  // reflector k (zero for indices <= k; column 0 carries tau_k)
 #pragma unroll
-        for (int n = 0; n < R; ++n)
-          if (gc + 8 * n < T) Gb[k * T + gc + 8 * n] = vc[n];
+        for (int n = 0; n < R; ++n) {
+          const int j = gc + 8 * n;
+          if (j < T) Gb[k * T + j] = j == 0 ? tk : vc[n];
+        }
       }
       if (lane == 0) {
-        s_d[k] = dk;
-        s_e[k] = beta;
-        s_tau[k] = tk;
+        Gb[(T - 2) * T + k] = dk;
+        Gb[(T - 1) * T + k] = beta;
       }
       if (tk != 0.0) {
         double pl[R], part2 = 0.0;
@@ -722,16 +700,75 @@ __global__ __launch_bounds__(64) void k_eig64_rows(double* __restrict__ gram, in
     }
   }
   // last 2x2 block: rows T-2 and T-1
+  double dd2 = 0.0, dd1 = 0.0, ee2 = 0.0;
 #pragma unroll
   for (int m = 0; m < R; ++m)
 #pragma unroll
     for (int n = 0; n < R; ++n) {
       const int i = gr + 8 * m, j = gc + 8 * n;
-      if (i == T - 2 && j == T - 2) s_d[T - 2] = a[m][n];
-      if (i == T - 1 && j == T - 1) s_d[T - 1] = a[m][n];
-      if (i == T - 1 && j == T - 2) s_e[T - 2] = a[m][n];
+      if (i == T - 2 && j == T - 2) dd2 = a[m][n];
+      if (i == T - 1 && j == T - 1) dd1 = a[m][n];
+      if (i == T - 1 && j == T - 2) ee2 = a[m][n];
     }
-  if (lane == 0) s_e[T - 1] = 0.0;
+  // written after every lane has read its last value of rows T-2 / T-1
+  // (the registers are complete: nothing of the buffer is read any more)
+  if (gr == (T - 2) % 8 && gc == (T - 2) % 8) Gb[(T - 2) * T + T - 2] = dd2;
+  if (gr == (T - 1) % 8 && gc == (T - 1) % 8) Gb[(T - 2) * T + T - 1] = dd1;
+  if (gr == (T - 1) % 8 && gc == (T - 2) % 8) Gb[(T - 1) * T + T - 2] = ee2;
+}
+
+template <int TP>
+__global__ __launch_bounds__(64) void k_eig64_rows(double* __restrict__ gram, int T, int N,
+                                                   long long nb, float* __restrict__ proj,
+                                                   float* __restrict__ fac,
+                                                   int* __restrict__ rank,
+                                                   float* __restrict__ weight,
+                                                   float* __restrict__ margin,
+                                                   float* __restrict__ lam_desc,
+                                                   int* __restrict__ info) {
+  constexpr int LD = TP + 1;
+  constexpr int R = TP / 8;  // register columns per lane of the 8 x 8 lane grid
+  __shared__ double s_d[TP], s_e[TP], s_e2[TP], s_tau[TP];
+  __shared__ double s_lam[TP];
+  __shared__ double s_dm[kRowGroup * LD];
+  __shared__ double s_z[kRowGroup * LD];  // D+ pivots, then the vector
+  __shared__ double s_h[kHChunk * TP];    // staged reflectors
+  __shared__ double s_ds[TP], s_e2s[TP], s_dsfx[TP];
+  __shared__ float s_lamf[TP], s_ef[TP], s_rcf[TP];
+
+  const long long b = blockIdx.x;
+  if (b >= nb) return;
+  const int lane = threadIdx.x;
+  const bool row_ok = lane < T;
+  const int gr = lane >> 3, gc = lane & 7;  // lane-grid row and column
+  double* Gb = gram + b * (long long)T * T;
+  // the tridiagonal form and the reflector scales left by k_tri_grid
+  bool finite = true;
+  if (lane < T) {
+    const double dl = Gb[(T - 2) * T + lane];
+    const double el = lane + 1 < T ? Gb[(T - 1) * T + lane] : 0.0;
+    s_d[lane] = dl;
+    s_e[lane] = el;
+    if (lane + 2 < T) s_tau[lane] = Gb[lane * T];
+    finite = isfinite(dl) && isfinite(el);
+  }
+  const bool all_finite = __any(!finite) == 0;
+  float* Pb = proj ? proj + b * (long long)T * T : nullptr;
+  float* Fb = fac ? fac + b * (2LL * TP * TP) : nullptr;  // TP = 16*ceil(T/16)
+  if (!all_finite) {
+    if (Pb)
+      for (int idx = lane; idx < T * T; idx += 64) Pb[idx] = 0.f;
+    if (lane == 0) {
+      rank[b] = 0;
+      weight[b] = 0.5f;
+      if (margin) margin[b] = 0.f;
+      if (info) info[b] = -2;
+    }
+    if (lam_desc)
+      for (int c = lane; c < T; c += 64) lam_desc[b * T + c] = NAN;
+    return;
+  }
+
   __syncthreads();
   // split the tridiagonal where |e_i| <= 8u (|d_i| + |d_{i+1}|): a backward
   // perturbation of the order of the reduction's own rounding, after which an
@@ -914,7 +951,7 @@ __global__ __launch_bounds__(64) void k_eig64_rows(double* __restrict__ gram, in
       const int klo = kc - kHChunk + 1 > 0 ? kc - kHChunk + 1 : 0;
       for (int r = 0; r <= kc - klo; ++r)
         for (int c = lane; c < TP; c += 64)
-          s_h[r * TP + c] = c < T ? Gb[(klo + r) * T + c] : 0.0;
+          s_h[r * TP + c] = (c > 0 && c < T) ? Gb[(klo + r) * T + c] : 0.0;  // column 0: tau
       __syncthreads();
       for (int k = kc; k >= klo; --k) {
         const double tk = s_tau[k];
@@ -962,6 +999,9 @@ template <int TP>
 static int launch_rows(double* gram, long long nb, int T, int N, float* proj, float* fac,
                        int* rank, float* weight, float* margin, float* lam_desc, int* info,
                        hipStream_t st) {
+  hipLaunchKernelGGL((k_tri_grid<TP>), dim3((unsigned)nb), dim3(64), 0, st, gram, T, nb);
+  int rc = launch_status("tridiagonalisation kernel");
+  if (rc) return rc;
   hipLaunchKernelGGL((k_eig64_rows<TP>), dim3((unsigned)nb), dim3(64), 0, st, gram, T, N, nb,
                      proj, fac, rank, weight, margin, lam_desc, info);
   return launch_status("eig64 (register) kernel");
