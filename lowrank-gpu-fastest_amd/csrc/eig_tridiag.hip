@@ -479,27 +479,32 @@ __device__ __forceinline__ int sturm_count3(const double* ds, const double* e2s,
 // rows i and i+1 (0 from row T-1 on). A padded row multiplies the minor by
 // 4 - x > 0 (|x| <= 1 + slack) and couples to nothing, so it never changes a
 // sign. The loops are unrolled over TP: no register is indexed at run time.
+//
+// Sturm count of sturm_count3 on the register-resident tridiagonal, without
+// the zero-minor substitution (the common case): *zero reports whether any minor came out exactly zero, in which case the
+// caller recounts that lane with the substituting count (sturm_count3 on the
+// same scaled tridiagonal in LDS; the two agree otherwise).
 template <int TP>
-__device__ __forceinline__ int sturm_count_reg(const double (&dr)[TP], const double (&er)[TP],
-                                               double x) {
-  constexpr double kTiny = 1e-290;
+__device__ __forceinline__ int sturm_count_fast(const double (&dr)[TP], const double (&er)[TP],
+                                                double x, bool* zero) {
   double pm = 1.0;
   double pc = dr[0] - x;
-  if (pc == 0.0) pc = -kTiny;
+  bool z = pc == 0.0;
   int c = pc < 0.0 ? 1 : 0;
 #pragma unroll
   for (int i = 1; i < TP; ++i) {
-    double pn = fma(dr[i] - x, pc, -er[i - 1] * pm);
-    if (pn == 0.0) pn = pc < 0.0 ? kTiny : -kTiny;
+    const double pn = fma(dr[i] - x, pc, -er[i - 1] * pm);
+    z = z || pn == 0.0;
     c += (pn < 0.0) != (pc < 0.0) ? 1 : 0;
     pm = pc;
     pc = pn;
-    if ((i & 7) == 0) {  // frexp exponent: 0 for a zero argument
+    if ((i & 7) == 0) {
       const int ex = __builtin_amdgcn_frexp_exp(fmax(fabs(pc), fabs(pm)));
       pc = __builtin_amdgcn_ldexp(pc, -ex);
       pm = __builtin_amdgcn_ldexp(pm, -ex);
     }
   }
+  *zero = z;
   return c;
 }
 
@@ -826,7 +831,10 @@ __global__ __launch_bounds__(64) void k_eig64_rows(double* __restrict__ gram, in
       double lo = lo_g * inv, hi = hi_g * inv;
       for (int it = 0; it < 96 && hi - lo > tol; ++it) {
         const double x = 0.5 * (lo + hi);
-        if (sturm_count_reg<TP>(dr, er, x) > kk)
+        bool zero;
+        int cnt = sturm_count_fast<TP>(dr, er, x, &zero);
+        if (zero) cnt = sturm_count3(s_ds, s_e2s, T, x);  // exact zero minor: rare
+        if (cnt > kk)
           hi = x;
         else
           lo = x;
