@@ -55,11 +55,31 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def stage_work(stage, shape, patch, n_patches, ranks, form, route):
+def gram_sliding(shape, patch, form):
+    """Whether the FP64 Gram runs as the sliding window over x-y columns: the
+    plain form at T <= 64 with a footprint of at most 16 staging passes of
+    2 (ceil(T/16) + 1) rows (the launcher's conditions in gram.hip)."""
+    x, y, z, t = shape
+    ntb = (t + 15) // 16
+    f = patch[0] * patch[1]
+    return (form == "plain" and t <= 64 and -(-f // (2 * (ntb + 1))) <= 16
+            and patch[0] * y * z * t * 4 < 2 ** 31)
+
+
+def covered_planes(size, p, stride):
+    """Planes of an axis that lie in at least one window (corner rule of common.py:37-44)."""
+    last = size - p
+    corners = sorted(set(range(0, last + 1, stride)) | {last})
+    return len({c + o for c in corners for o in range(p)})
+
+
+def stage_work(stage, shape, patch, stride, n_patches, ranks, form, route):
     """(bound, algorithmic amount per launch, peak key) of one stage.
 
     centre    bytes: read vol, write xc and mean                4*(2*V*T + V)
     gram      flops: symmetric rank-N update per patch (syrk)   N*T*(T+1) * P
+                     sliding window: each covered plane of each
+                     x-y column's footprint once                F*T*(T+1) * Zc * columns
     eigh_mp   flops: tridiagonalisation + p back-transformed
                      vectors per patch (LAPACK counts)          sum_b 4/3*T^3 + 2*T^2*p_b
     recon     flops: factors: (Xc V_p) V_p^T per patch           sum_b 4*N*T*p_b
@@ -74,6 +94,11 @@ def stage_work(stage, shape, patch, n_patches, ranks, form, route):
     if stage == "centre":
         return "hbm", 4.0 * (2 * nvox * t + nvox), "hbm"
     if stage == "gram":
+        if gram_sliding(shape, patch, form):
+            ncz = len(set(range(0, z - patch[2] + 1, stride)) | {z - patch[2]})
+            ncol = n_patches // ncz
+            f = patch[0] * patch[1]
+            return "mfma", float(f) * t * (t + 1) * covered_planes(z, patch[2], stride) * ncol, fp
         return "mfma", float(n) * t * (t + 1) * n_patches, fp
     if stage == "eigh_mp":
         return "mfma", 4.0 / 3.0 * t ** 3 * n_patches + 2.0 * t * t * psum, fp
@@ -318,7 +343,8 @@ def main():
     kernel_ms = stage_times(d, vol_d, out_d, max(K, 5))
     log("per-kernel ms: " + ", ".join(f"{n} {v:.3f}" for n, v in kernel_ms.items()))
     dom = max(kernel_ms, key=kernel_ms.get)
-    bound, amount, peak_key = stage_work(dom, shape, patch, P, info["rank"], args.form, d.route)
+    bound, amount, peak_key = stage_work(dom, shape, patch, stride, P, info["rank"], args.form,
+                                         d.route)
     peak, unit = PEAK[peak_key]
     sec = kernel_ms[dom] * 1e-3
     achieved = amount / sec / (1e9 if unit == "GB/s" else 1e12)
@@ -327,7 +353,7 @@ def main():
                 "work_per_launch": amount, "peak_of": peak_key}
     kernels = {}
     for n, ms in kernel_ms.items():
-        b, a, pk = stage_work(n, shape, patch, P, info["rank"], args.form, d.route)
+        b, a, pk = stage_work(n, shape, patch, stride, P, info["rank"], args.form, d.route)
         pv, u = PEAK[pk]
         ach = a / (ms * 1e-3) / (1e9 if u == "GB/s" else 1e12)
         kernels[n] = {"ms": round(ms, 4), "bound": b, "achieved": round(ach, 3), "unit": u,

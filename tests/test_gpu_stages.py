@@ -324,26 +324,52 @@ def test_recon_factors_rejects_large_t(pkg, lib):
     assert b"64" in lib.mppca_last_error()
 
 
+def _slide_bound(xc, patch, stride):
+    """Rounding bound of the sliding-window Gram (T <= 64): G = P(cz + pz) - P(cz)
+    with P(z) the FP64 prefix over the planes z' < z of the patch's x-y
+    footprint. Products of float32 values are exact; each prefix carries at
+    most (cz + pz) * px * py rounded additions of terms bounded by
+    Q = sum_{z' < cz + pz} |x| |x|^T, and the difference one more rounding."""
+    import oracle as o
+
+    px, py, pz = patch
+    A = np.abs(xc.astype(np.float64))
+    out = []
+    for i, j, k in o.patch_corners(xc.shape, patch, stride):
+        F = A[i:i + px, j:j + py, :k + pz].reshape(-1, xc.shape[-1])
+        out.append(2.0 * (k + pz) * px * py * 2.0 ** -52 * (F.T @ F))
+    return np.stack(out)
+
+
 def test_gram64_exact(pkg, lib):
-    """FP64 MFMA Gram: products of float32 values are exact, sums carry 53 bits."""
+    """FP64 MFMA Gram: products of float32 values are exact, sums carry 53 bits.
+    T > 64 (per-patch sums) is held to the patch-sum bound, T <= 64 (sliding
+    window over each x-y column) to the prefix-difference bound; a launch over
+    part of the patches (not whole columns) takes the per-patch kernel."""
     torch = _torch()
     rng = np.random.default_rng(8)
     for shape, patch, stride in [((13, 13, 13, 48), (11, 11, 11), 2), ((9, 8, 8, 100), (4, 4, 4), 2),
-                                 ((10, 9, 8, 17), (5, 3, 4), 2), ((8, 8, 8, 128), (5, 5, 5), 3)]:
+                                 ((10, 9, 8, 17), (5, 3, 4), 2), ((8, 8, 8, 128), (5, 5, 5), 3),
+                                 ((9, 7, 30, 33), (3, 4, 5), 3), ((8, 9, 23, 64), (4, 3, 3), 5),
+                                 ((6, 5, 40, 2), (2, 2, 7), 1)]:
+        print("gram64", shape, patch, stride, flush=True)
         vol = (rng.random(shape[:3] + (3,)) @ rng.standard_normal((3, shape[3]))
                + 0.05 * rng.standard_normal(shape)).astype(np.float32)
         xc = (vol - vol.mean(-1, keepdims=True, dtype=np.float64)).astype(np.float32)
         P, t = pkg.num_patches(shape, patch, stride), shape[3]
-        g = torch.empty((P, t, t), dtype=torch.float64, device="cuda")
-        dx = _dev(xc)
-        pkg._lib.check(lib.mppca_gram64(dx.data_ptr(), g.data_ptr(), *shape, *patch, stride, 0, P,
-                                        _stream()), "gram64")
-        _sync()
         X = _gather(xc, patch, stride).astype(np.float64)
         G64 = np.einsum("bnt,bns->bts", X, X)
         A = np.abs(X)
         bound = np.einsum("bnt,bns->bts", A, A) * X.shape[1] * 2.0 ** -52
-        assert np.all(np.abs(g.cpu().numpy() - G64) <= bound)
+        if t <= 64:
+            bound = np.maximum(bound, _slide_bound(xc, patch, stride) + np.abs(G64) * 2.0 ** -52)
+        dx = _dev(xc)
+        for b0, nb in ((0, P), (1, P - 2)):  # whole volume; a range of partial columns
+            g = torch.full((nb, t, t), float("nan"), dtype=torch.float64, device="cuda")
+            pkg._lib.check(lib.mppca_gram64(dx.data_ptr(), g.data_ptr(), *shape, *patch, stride,
+                                            b0, nb, _stream()), "gram64")
+            _sync()
+            assert np.all(np.abs(g.cpu().numpy() - G64[b0:b0 + nb]) <= bound[b0:b0 + nb])
 
 
 def _eig64(pkg, lib, G, n_vox):
