@@ -170,48 +170,60 @@ int launch_gram(const float* xc, float* gram, const Geom& g, long long b0, long 
 constexpr int kSlideMaxLoads = 16;  // plane floats each thread stages per plane
 __host__ __device__ constexpr int slide_rows(int F) { return (F + 31) / 32 * 32; }
 __host__ __device__ constexpr int slide_stride(int ntb) { return ntb % 2 ? ntb * 16 : ntb * 16 + 16; }
-// rows staged per pass: the block's 64 * NTB (NTB + 1) / 2 threads over NTB * 16 frames
+// rows staged per pass: a column's 64 * NTB (NTB + 1) / 2 threads over NTB * 16 frames
 __host__ __device__ constexpr int slide_rows_per_pass(int ntb) { return 2 * (ntb + 1); }
+// columns per workgroup: 4, 12, 12 and 10 wavefronts for NTB = 1..4 (a multiple
+// of the 4 SIMDs where the 1024-thread limit allows)
+__host__ __device__ constexpr int slide_cols(int ntb) { return ntb >= 4 ? 1 : ntb == 3 ? 2 : 4; }
+__host__ __device__ constexpr size_t slide_lds(int ntb, int F) {
+  return (size_t)slide_cols(ntb) * ((size_t)slide_rows(F) * slide_stride(ntb) + F) * 4;
+}
 
 template <int NTB>
-__global__ __launch_bounds__(64 * NTB * (NTB + 1) / 2) void k_gram_slide(
+__global__ __launch_bounds__(64 * NTB * (NTB + 1) / 2 * slide_cols(NTB)) __attribute__((amdgpu_waves_per_eu(6))) void k_gram_slide(
     const float* __restrict__ xc, double* __restrict__ gram, Geom g, int ix0, int ncol,
     int per_xcd) {
-  constexpr int NTHR = 64 * NTB * (NTB + 1) / 2;
+  constexpr int NBLK = NTB * (NTB + 1) / 2, NCOL = 64 * NBLK;  // threads per column
   constexpr int TP = slide_stride(NTB), TS = 16 * NTB, RPT = slide_rows_per_pass(NTB);
-  static_assert(RPT * TS == NTHR, "staging grid must cover the block");
-  extern __shared__ float s_pl[];  // slide_rows(F) x TP plane tile
-  // XCD-contiguous columns: blocks are dealt to the 8 XCDs round robin, so XCD
-  // k walks columns [k * per_xcd, (k + 1) * per_xcd) and neighbouring columns,
-  // whose footprints overlap in (px - stride) / px of their voxels, share its L2
-  const int col = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
-  if (col >= ncol) return;
+  static_assert(RPT * TS == NCOL, "staging grid must cover a column's threads");
+  extern __shared__ float s_lds[];
   const int T = g.T, F = g.px * g.py, R = slide_rows(F);
+  // XCD-contiguous columns: workgroups are dealt to the 8 XCDs round robin, so
+  // XCD k walks columns [k * per_xcd, (k + 1) * per_xcd) * CPB and neighbouring
+  // columns, whose footprints overlap in (px - stride) / px of their voxels,
+  // share its L2
+  const int blk = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
+  const int h = threadIdx.x / NCOL, tid = threadIdx.x - h * NCOL;  // column slot, thread in it
+  const int col_raw = blk * slide_cols(NTB) + h;
+  if (blk * slide_cols(NTB) >= ncol) return;  // whole workgroup past the end
+  // a slot past the last column repeats it without storing, to keep the barriers
+  const bool writer = col_raw < ncol;
+  const int col = writer ? col_raw : ncol - 1;
+  float* s_pl = s_lds + h * (R * TP + F);                 // this column's plane tile
+  unsigned* s_rowb = (unsigned*)(s_pl + R * TP);          // byte offset of footprint row n
   const int ix = ix0 + col / g.ncy, iy = col % g.ncy;
   const int cx = corner_of(ix, g.nsx, g.X, g.px, g.stride);
   const int cy = corner_of(iy, g.nsy, g.Y, g.py, g.stride);
-  for (int e = threadIdx.x; e < R * TP; e += NTHR) s_pl[e] = 0.f;
+  for (int e = tid; e < R * TP; e += NCOL) s_pl[e] = 0.f;
+  for (int n = tid; n < F; n += NCOL) {
+    const int dx = n / g.py, dy = n - dx * g.py;
+    s_rowb[n] = (unsigned)((dx * g.Y + dy) * g.Z) * (unsigned)T * 4u;
+  }
 
   // staging role: thread -> frame st of rows sr0 + RPT * k (k < nk <= kSlideMaxLoads,
   // checked by the launcher); frames past T and rows past F stay zero in the tile
-  const int st = threadIdx.x % TS, sr0 = threadIdx.x / TS;
+  const int st = tid % TS, sr0 = tid / TS;
   const int nk = (F + RPT - 1) / RPT;
   const bool st_ok = st < T;
   const char* cb = (const char*)(xc + (((long long)cx * g.Y + cy) * g.Z) * T);  // (cx, cy, 0)
-  unsigned goff[kSlideMaxLoads];  // byte offset of (row, frame) at plane 0
-#pragma unroll
-  for (int k = 0; k < kSlideMaxLoads; ++k) {
-    const int n0 = sr0 + RPT * k, n = n0 < F ? n0 : F - 1;
-    const int dx = n / g.py, dy = n - dx * g.py;
-    goff[k] = ((unsigned)((dx * g.Y + dy) * g.Z) * (unsigned)T + (st_ok ? st : T - 1)) * 4u;
-  }
-  const unsigned zbytes = 4u * (unsigned)T;
+  const unsigned sto = 4u * (unsigned)(st_ok ? st : T - 1);
   float stage[kSlideMaxLoads];
+  unsigned ro[kSlideMaxLoads];  // byte offsets of this thread's staging rows at plane 0
   auto fetch = [&](int z) {  // plane z of the footprint into registers
-    const unsigned zo = (unsigned)z * zbytes;
+    const unsigned zo = (unsigned)z * 4u * (unsigned)T + sto;
 #pragma unroll
     for (int k = 0; k < kSlideMaxLoads; ++k)
-      if (k < nk) stage[k] = *(const float*)(cb + (goff[k] + zo));
+      if (k < nk) stage[k] = *(const float*)(cb + (ro[k] + zo));
   };
   float* s_put = s_pl + sr0 * TP + st;
   auto put = [&]() {  // registers into the LDS tile
@@ -219,14 +231,12 @@ __global__ __launch_bounds__(64 * NTB * (NTB + 1) / 2) void k_gram_slide(
     for (int k = 0; k < kSlideMaxLoads; ++k)
       if (k < nk && st_ok && sr0 + RPT * k < F) s_put[RPT * TP * k] = stage[k];
   };
-  __syncthreads();  // the tile is zeroed
 
   // compute role: one wavefront per 16 x 16 block (tb1, tb2) of the upper triangle
-  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int lane = threadIdx.x & 63, i = lane & 15, kq = lane >> 4;
   int tb1 = 0, tb2 = 0;
   {
-    int w = wave;
+    int w = __builtin_amdgcn_readfirstlane(tid >> 6);
     for (int t1 = 0; t1 < NTB; ++t1) {
       if (w < NTB - t1) {
         tb1 = t1;
@@ -241,20 +251,21 @@ __global__ __launch_bounds__(64 * NTB * (NTB + 1) / 2) void k_gram_slide(
   const float* opB = s_pl + kq * TP + tb2 * 16 + i;
   double* Gc = gram + (long long)col * g.ncz * T * T;  // patch (ix, iy, 0)
   // block entries of this lane: register r holds row tb1*16 + kq + 4r, column t2
+  const bool ent_ok = writer && t2 < T;
   auto entry = [&](int iz, int r) -> double* {
     return Gc + (long long)iz * T * T + (long long)(tb1 * 16 + kq + 4 * r) * T + t2;
   };
   auto start = [&](int iz, const doublex4& P) {
 #pragma unroll
     for (int r = 0; r < 4; ++r)
-      if (tb1 * 16 + kq + 4 * r < T && t2 < T) *entry(iz, r) = -P[r];
+      if (ent_ok && tb1 * 16 + kq + 4 * r < T) *entry(iz, r) = -P[r];
   };
   auto finish = [&](int iz, const doublex4& P) {
     // the stores of -P precede these loads in this wave's program order
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int ti = tb1 * 16 + kq + 4 * r;
-      if (ti < T && t2 < T) {
+      if (ent_ok && ti < T) {
         double* e = entry(iz, r);
         const double v = P[r] + *e;
         *e = v;
@@ -262,6 +273,12 @@ __global__ __launch_bounds__(64 * NTB * (NTB + 1) / 2) void k_gram_slide(
       }
     }
   };
+  __syncthreads();  // tiles zeroed, offset tables written
+#pragma unroll
+  for (int k = 0; k < kSlideMaxLoads; ++k) {
+    const int n = sr0 + RPT * k;
+    ro[k] = k < nk ? s_rowb[n < F ? n : F - 1] : 0u;
+  }
 
   // two accumulators (even and odd chunks) break the MFMA dependency chain;
   // the prefix is their sum
@@ -271,51 +288,50 @@ __global__ __launch_bounds__(64 * NTB * (NTB + 1) / 2) void k_gram_slide(
   for (int z = 0; z < g.Z; ++z) {
     if (is < g.ncz && corner_of(is, g.nsz, g.Z, g.pz, g.stride) == z) start(is++, P0 + P1);
     const bool act = is > ie;  // plane z lies in a window
-    __syncthreads();           // every wave is done with plane z - 1
+    const bool ends = ie < is && corner_of(ie, g.nsz, g.Z, g.pz, g.stride) + g.pz == z + 1;
+    __syncthreads();  // every wave is done with plane z - 1
     put();
     __syncthreads();
     if (z + 1 < g.Z) fetch(z + 1);  // in flight under this plane's MFMAs
     if (act) {
-      for (int c = 0; c < R / 4; c += 8) {
-        float a[8], b[8];
+      for (int c = 0; c < R / 4; c += 4) {
+        float a[4], b[4];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
+        for (int u = 0; u < 4; ++u) {
           a[u] = opA[(4 * (c + u)) * TP];
           b[u] = opB[(4 * (c + u)) * TP];
         }
 #pragma unroll
-        for (int u = 0; u < 8; u += 2) {
+        for (int u = 0; u < 4; u += 2) {
           P0 = __builtin_amdgcn_mfma_f64_16x16x4f64((double)a[u], (double)b[u], P0, 0, 0, 0);
           P1 = __builtin_amdgcn_mfma_f64_16x16x4f64((double)a[u + 1], (double)b[u + 1], P1, 0,
                                                     0, 0);
         }
       }
     }
-    if (ie < is && corner_of(ie, g.nsz, g.Z, g.pz, g.stride) + g.pz == z + 1)
-      finish(ie++, P0 + P1);
+    if (ends) finish(ie++, P0 + P1);
   }
 }
 
 template <int NTB>
 static int launch_gram_slide_t(const float* xc, double* gram, const Geom& g, int ix0, int nix,
                                hipStream_t st) {
-  constexpr int NTHR = 64 * NTB * (NTB + 1) / 2;
-  const int F = g.px * g.py;
-  const int ncol = nix * g.ncy, per_xcd = (ncol + 7) / 8;
-  const size_t lds = (size_t)slide_rows(F) * slide_stride(NTB) * 4;
-  hipLaunchKernelGGL((k_gram_slide<NTB>), dim3(8u * (unsigned)per_xcd), dim3(NTHR), lds, st, xc,
-                     gram, g, ix0, ncol, per_xcd);
+  constexpr int NTHR = 64 * NTB * (NTB + 1) / 2 * slide_cols(NTB);
+  const int ncol = nix * g.ncy;
+  const int nblk = (ncol + slide_cols(NTB) - 1) / slide_cols(NTB), per_xcd = (nblk + 7) / 8;
+  hipLaunchKernelGGL((k_gram_slide<NTB>), dim3(8u * (unsigned)per_xcd), dim3(NTHR),
+                     slide_lds(NTB, g.px * g.py), st, xc, gram, g, ix0, ncol, per_xcd);
   return launch_status("gram (sliding) kernel");
 }
 
 // The sliding kernel applies to launches over whole x-corner columns whose
-// plane (footprint x frames) fits the staging registers and whose footprint
-// byte offsets fit in 31 bits.
+// plane (footprint x frames) fits the staging registers and the LDS, and whose
+// footprint byte offsets fit in 31 bits.
 static bool slide_fits(const Geom& g) {
   if (g.T > 64) return false;
   const int ntb = ntb_of(g.T), F = g.px * g.py;
   if ((F + slide_rows_per_pass(ntb) - 1) / slide_rows_per_pass(ntb) > kSlideMaxLoads) return false;
-  if ((size_t)slide_rows(F) * slide_stride(ntb) * 4 > 64 * 1024) return false;
+  if (slide_lds(ntb, F) > 96 * 1024) return false;
   return (long long)g.px * g.Y * g.Z * g.T * 4 < (1LL << 31);
 }
 
