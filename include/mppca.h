@@ -155,9 +155,11 @@ int mppca_normalize(const float* acc, const float* mean, const float* weight,
  * (TF = 16*ceil(T/16) frames; tile nc holds eigenvectors 16nc .. 16nc+15):
  *   Ut[nc][c][t] at nc*16*TF + c*TF + t           eigenvectors as rows
  *   Uw[nc][t][c] at TF*TF + nc*16*TF + t*16 + c   weight[b] * eigenvectors
- * Inside a tile of q = min(16, p - 16nc) vectors, S2 = ceil(q/4), vector g
- * sits in slot c = 4*(g / S2) + g % S2; other slots and frames t >= T are 0.
- * Only tiles nc < ceil(p/16) are written.
+ * Inside a tile of q = min(16, p - 16nc) vectors, vector g sits in slot
+ * c = 4*(g / S2) + g % S2, with S2 = ceil(q/2) when p <= 8 (slots 0..7 only,
+ * so that two such patches share one tile in the reconstruction) and
+ * S2 = ceil(q/4) otherwise; other slots and frames t >= T are 0. Tiles
+ * nc < max(1, ceil(p/16)) are written (a rank-0 patch gets a zero tile).
  * ------------------------------------------------------------------------- */
 
 /* Floats of one patch's factor block for T frames (< 0 if T > 64). */

@@ -149,8 +149,16 @@ __host__ __device__ inline int factor_tf(int T) { return 16 * ((T + 15) / 16); }
 __host__ __device__ inline long long factor_floats(int T) { return 2LL * factor_tf(T) * factor_tf(T); }
 constexpr int kFactorMaxT = 64;
 
-__device__ __forceinline__ int factor_slot(int g16, int q) {
-  const int S2 = (q + 3) >> 2;
+// Patches of rank p <= kFactorPairMaxRank use the compact layout: their q = p
+// vectors fill slots 0..7 only, two groups of S2 = ceil(q/2), so that the
+// reconstruction can put two such patches in one 16-slot MFMA tile. Other tiles
+// hold q = min(16, p - 16 nc) vectors in four groups of S2 = ceil(q/4).
+constexpr int kFactorPairMaxRank = 8;
+__host__ __device__ inline int factor_groups(int q, int p) {
+  return p <= kFactorPairMaxRank ? (q + 1) >> 1 : (q + 3) >> 2;
+}
+__device__ __forceinline__ int factor_slot(int g16, int q, int p) {
+  const int S2 = factor_groups(q, p);
   return 4 * (g16 / S2) + g16 % S2;
 }
 
@@ -162,20 +170,21 @@ __device__ inline void write_factors(float* F, const double* z, int ldz, int g0,
   for (int g = 0; g < gn; ++g) {
     const int G = g0 + g, nc = G >> 4;
     const int q = p - 16 * nc < 16 ? p - 16 * nc : 16;
-    const int c = factor_slot(G & 15, q);
+    const int c = factor_slot(G & 15, q, p);
     const double v = lane < T ? z[g * ldz + lane] : 0.0;
     F[nc * 16 * TF + c * TF + lane] = (float)v;
     F[TF * TF + nc * 16 * TF + lane * 16 + c] = (float)(w * v);
   }
 }
 
-// Zeros the slots of the written tiles that no vector fills.
+// Zeros the slots of the written tiles that no vector fills; a rank-0 patch
+// gets its first tile zeroed, since the reconstruction reads it in a pair.
 __device__ inline void zero_factor_slots(float* F, int p, int TF, int lane) {
   if (lane >= TF) return;
-  const int nct = (p + 15) >> 4;
+  const int nct = p > 0 ? (p + 15) >> 4 : 1;
   for (int nc = 0; nc < nct; ++nc) {
     const int q = p - 16 * nc < 16 ? p - 16 * nc : 16;
-    const int S2 = (q + 3) >> 2;
+    const int S2 = factor_groups(q, p);
     for (int c = 0; c < 16; ++c) {
       const int a = c >> 2, s = c & 3;
       if (s < S2 && a * S2 + s < q) continue;  // slot of vector a*S2 + s

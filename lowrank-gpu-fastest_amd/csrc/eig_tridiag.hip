@@ -355,6 +355,7 @@ __global__ __launch_bounds__(64) void k_eig64_mp(const double* __restrict__ gram
   if (!finite) {  // non-finite input: no rank, no projector, flagged
     if (Pb)
       for (int idx = lane; idx < T * T; idx += 64) Pb[idx] = 0.f;
+    if (Fb) zero_factor_slots(Fb, 0, factor_tf(T), lane);  // read by paired reconstruction
     if (lane == 0) {
       rank[b] = 0;
       weight[b] = 0.5f;
@@ -763,6 +764,7 @@ __global__ __launch_bounds__(64) void k_eig64_rows(double* __restrict__ gram, in
   if (!all_finite) {
     if (Pb)
       for (int idx = lane; idx < T * T; idx += 64) Pb[idx] = 0.f;
+    if (Fb) zero_factor_slots(Fb, 0, TP, lane);  // read by paired reconstruction
     if (lane == 0) {
       rank[b] = 0;
       weight[b] = 0.5f;

@@ -232,11 +232,55 @@ __global__ __launch_bounds__(256) void k_recon_lr(const float* __restrict__ xc,
       *ew = __float_as_int(weight[b]);
     }
   };
-  // factor fragments of slot tile nc: ut[s] = Ut[slot i][frame kq*KQ + s],
-  // uw[tb*4 + s] = Uw[frame tb*16 + i][slot 4*kq + s]
-  auto load_fac = [&](long long b, int nc, float* ut, float* uw) {
-    const float* F = fac + b * (2LL * TF * TF);
-    const float* U = F + nc * 16 * TF + i * TF + kq * KQ;
+  // Patches are visited in steps: two consecutive patches of rank <= 8 (the
+  // compact factor layout, vectors in slots 0..7) form a pair, any other patch
+  // is a single. A pair shares one MFMA tile: rows 0..7 of the A operand of
+  // C^T = Ut Xc^T are the first patch's slots, rows 8..15 the second's, against
+  // the same voxel rows; in D += C Uw^T the K lanes kq < 2 carry the first
+  // patch and kq >= 2 the second. That halves the C^T MFMAs of rank <= 8
+  // patches and shares the D MFMAs between two patches.
+  struct Step {
+    long long ba, bb;  // patch indices (bb = ba for a single)
+    int xya, zpa, xyb, zpb;
+    float wa, wb;
+    int kind;  // 0: single, standard layout (rank > 8); 1: single, compact; 2: pair
+  };
+  int base = 0;  // first patch index of the current chunk
+  auto field = [&](int cur, int nxt, int q) {  // q - base < 128, wave-uniform
+    const int l = q - base;
+    return l < 64 ? __builtin_amdgcn_readlane(cur, l) : __builtin_amdgcn_readlane(nxt, l - 64);
+  };
+  int cb, cxy, czp, cw, nbq, nxy, nzp, nw;  // this chunk's table, the next chunk's
+  auto step_at = [&](int q) {
+    Step S;
+    S.ba = field(cb, nbq, q);
+    S.xya = field(cxy, nxy, q);
+    S.zpa = field(czp, nzp, q);
+    S.wa = __int_as_float(field(cw, nw, q));
+    S.kind = (S.zpa >> 16) > kFactorPairMaxRank ? 0 : 1;
+    S.bb = S.ba;
+    S.xyb = S.xya;
+    S.zpb = S.zpa;
+    S.wb = S.wa;
+    if (S.kind == 1 && q + 1 < npatch) {
+      const int zp1 = field(czp, nzp, q + 1);
+      if ((zp1 >> 16) <= kFactorPairMaxRank) {
+        S.kind = 2;
+        S.bb = field(cb, nbq, q + 1);
+        S.xyb = field(cxy, nxy, q + 1);
+        S.zpb = zp1;
+        S.wb = __int_as_float(field(cw, nw, q + 1));
+      }
+    }
+    return S;
+  };
+  // factor fragments of a step (slot tile nc of a standard single): ut[s] = Ut[row][frame
+  // kq*KQ + s] (A operand), uw[tb*4 + s] = Uw[frame tb*16 + i][slot col + s] (B operand)
+  auto load_fac = [&](const Step& S, int nc, float* ut, float* uw) {
+    const bool std_ = S.kind == 0;
+    const long long bA = std_ || i < 8 ? S.ba : S.bb, bB = std_ || kq < 2 ? S.ba : S.bb;
+    const int rowA = std_ ? i : (i & 7), colB = std_ ? 4 * kq : 4 * (kq & 1);
+    const float* U = fac + bA * (2LL * TF * TF) + nc * 16 * TF + rowA * TF + kq * KQ;
 #pragma unroll
     for (int v = 0; v < KV; ++v) {
       const float4 q = *(const float4*)(U + 4 * v);
@@ -245,7 +289,7 @@ __global__ __launch_bounds__(256) void k_recon_lr(const float* __restrict__ xc,
       ut[4 * v + 2] = q.z;
       ut[4 * v + 3] = q.w;
     }
-    const float* Wp = F + TF * TF + nc * 16 * TF + i * 16 + 4 * kq;
+    const float* Wp = fac + bB * (2LL * TF * TF) + TF * TF + nc * 16 * TF + i * 16 + colB;
 #pragma unroll
     for (int tb = 0; tb < NTB; ++tb) {
       const float4 q = *(const float4*)(Wp + tb * 256);
@@ -257,47 +301,67 @@ __global__ __launch_bounds__(256) void k_recon_lr(const float* __restrict__ xc,
   };
 
   float ut[KQ], uw[4 * NTB], utn[KQ], uwn[4 * NTB];
-  int cb, cxy, czp, cw, nbq, nxy, nzp, nw;  // this chunk's table, the next chunk's
   decode(lane, &cb, &cxy, &czp, &cw);
   decode(64 + lane, &nbq, &nxy, &nzp, &nw);
-  if (npatch > 0) load_fac(__builtin_amdgcn_readlane(cb, 0), 0, utn, uwn);
-  for (int q = 0; q < npatch; ++q) {
-    const int jq = q & 63;
-    const long long b = __builtin_amdgcn_readlane(cb, jq);
-    const int xy = __builtin_amdgcn_readlane(cxy, jq), zp = __builtin_amdgcn_readlane(czp, jq);
-    const float w = __int_as_float(__builtin_amdgcn_readlane(cw, jq));
-    const int cx = xy & 0xFFFF, cy = xy >> 16, cz = zp & 0xFFFF, p = zp >> 16;
+  Step nxt_step;
+  if (npatch > 0) {
+    nxt_step = step_at(0);
+    load_fac(nxt_step, 0, utn, uwn);
+  }
+  for (int q = 0; q < npatch;) {
+    const Step S = nxt_step;
+    const int qn = q + (S.kind == 2 ? 2 : 1);
 #pragma unroll
     for (int k = 0; k < KQ; ++k) ut[k] = utn[k];
 #pragma unroll
     for (int k = 0; k < 4 * NTB; ++k) uw[k] = uwn[k];
-    if (q + 1 < npatch) {  // next patch's factors in flight under this one's MFMAs
-      const int bq = jq < 63 ? __builtin_amdgcn_readlane(cb, jq + 1)
-                             : __builtin_amdgcn_readlane(nbq, 0);
-      load_fac(bq, 0, utn, uwn);
+    if (qn < npatch) {  // next step's factors in flight under this one's MFMAs
+      nxt_step = step_at(qn);
+      load_fac(nxt_step, 0, utn, uwn);
     }
-    const int jlo = max(cx - x0, 0), jhi = min(cx + g.px - x0, SX);
-    // membership of row voxel i (C mask) and of output voxel (kq, r) (W)
-    const float mrow = (yi >= cy && yi < cy + g.py && zi >= cz && zi < cz + g.pz) ? 1.f : 0.f;
-    const bool yin = y0 + kq >= cy && y0 + kq < cy + g.py;
-#pragma unroll
-    for (int j = 0; j < SX; ++j) {
-      if (j < jlo || j >= jhi) continue;
-#pragma unroll
-      for (int r = 0; r < 4; ++r)
-        Wv[j][r] += (yin && z0 + r >= cz && z0 + r < cz + g.pz) ? w : 0.f;
-    }
-    const int nct = (p + 15) >> 4;
-    for (int nc = 0; nc < nct; ++nc) {
-      if (nc > 0) load_fac(b, nc, ut, uw);  // rank above 16: further slot tiles
-      const int qn = min(16, p - 16 * nc), S2 = (qn + 3) >> 2;
+    const int cxa = S.xya & 0xFFFF, cya = S.xya >> 16, cza = S.zpa & 0xFFFF, pa = S.zpa >> 16;
+    const int cxb = S.xyb & 0xFFFF, cyb = S.xyb >> 16, czb = S.zpb & 0xFFFF, pb = S.zpb >> 16;
+    // weight sums in patch order (step 0's wsum), output voxel (kq, r) of slab j
+    auto add_w = [&](int cx, int cy, int cz, float w) {
+      const bool yin = y0 + kq >= cy && y0 + kq < cy + g.py;
 #pragma unroll
       for (int j = 0; j < SX; ++j) {
-        if (j < jlo || j >= jhi) continue;
+        if (x0 + j < cx || x0 + j >= cx + g.px) continue;
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          Wv[j][r] += (yin && z0 + r >= cz && z0 + r < cz + g.pz) ? w : 0.f;
+      }
+    };
+    add_w(cxa, cya, cza, S.wa);
+    if (S.kind == 2) add_w(cxb, cyb, czb, S.wb);
+    // C^T row slot 4kq + r of this lane belongs to patch A, or to B (pairs, kq >= 2)
+    const bool laneB = S.kind != 0 && kq >= 2;
+    const bool lane_on = !laneB || S.kind == 2;
+    const int cxl = laneB ? cxb : cxa, cyl = laneB ? cyb : cya, czl = laneB ? czb : cza;
+    const bool myz = lane_on && yi >= cyl && yi < cyl + g.py && zi >= czl && zi < czl + g.pz;
+    const int nct = S.kind == 0 ? (pa + 15) >> 4 : 1;
+    for (int nc = 0; nc < nct; ++nc) {
+      if (nc > 0) load_fac(S, nc, ut, uw);  // rank above 16: further slot tiles
+      int S2;
+      if (S.kind == 0) {
+        S2 = (min(16, pa - 16 * nc) + 3) >> 2;
+      } else {
+        const int pm = S.kind == 2 ? max(pa, pb) : pa;
+        S2 = (pm + 1) >> 1;
+      }
+#pragma unroll
+      for (int j = 0; j < SX; ++j) {
+        const bool inA = x0 + j >= cxa && x0 + j < cxa + g.px;
+        const bool inB = S.kind == 2 && x0 + j >= cxb && x0 + j < cxb + g.px;
+        if (!inA && !inB) continue;
         floatx4 ct = floatx4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int s = 0; s < KQ; ++s) ct = __builtin_amdgcn_mfma_f32_16x16x4f32(ut[s], xv[j][s], ct, 0, 0, 0);
-        ct *= mrow;
+        // a select, not a product: factor tiles of rank-0 patches hold zeros,
+        // but the mask must not depend on that
+        const bool keep = myz && (laneB ? inB : inA);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) ct[r] = keep ? ct[r] : 0.f;
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
           if (s >= S2) break;
@@ -307,13 +371,15 @@ __global__ __launch_bounds__(256) void k_recon_lr(const float* __restrict__ xc,
         }
       }
     }
-    if (jq == 63) {  // next chunk becomes current; decode the one after
+    if (qn - base >= 64) {  // next chunk becomes current; decode the one after
+      base += 64;
       cb = nbq;
       cxy = nxy;
       czp = nzp;
       cw = nw;
-      decode(q + 65 + lane, &nbq, &nxy, &nzp, &nw);
+      decode(base + 64 + lane, &nbq, &nxy, &nzp, &nw);
     }
+    q = qn;
   }
 
   // out = mean + D / W (0 where no patch covers the voxel), as k_normalize

@@ -252,7 +252,7 @@ def _factor_blocks(vecs, wts, t):
     """Factor blocks in the layout mppca_eig64_factors writes (include/mppca.h):
     TF = 16*ceil(T/16); Ut[nc][c][t] at nc*16*TF + c*TF + t, Uw[nc][t][c] at
     TF*TF + nc*16*TF + t*16 + c; vector g of a tile of q vectors in slot
-    4*(g // S2) + g % S2 with S2 = ceil(q/4)."""
+    4*(g // S2) + g % S2 with S2 = ceil(q/2) for rank p <= 8, else ceil(q/4)."""
     tf = 16 * ((t + 15) // 16)
     F = np.zeros((len(vecs), 2 * tf * tf), np.float32)
     for b, V in enumerate(vecs):
@@ -260,7 +260,7 @@ def _factor_blocks(vecs, wts, t):
         for g in range(p):
             nc, g16 = divmod(g, 16)
             q = min(16, p - 16 * nc)
-            s2 = (q + 3) // 4
+            s2 = (q + 1) // 2 if p <= 8 else (q + 3) // 4
             c = 4 * (g16 // s2) + g16 % s2
             F[b, nc * 16 * tf + c * tf + np.arange(t)] = V[:, g]
             F[b, tf * tf + nc * 16 * tf + np.arange(t) * 16 + c] = (
@@ -482,5 +482,9 @@ def test_eig64_factors_match_projector(pkg, lib, n_vox, t):
         np.testing.assert_allclose(U.T @ U, proj[i], rtol=0, atol=1e-6)
         filled = np.abs(U).sum(1) > 0
         assert filled.sum() == p
+        if p <= 8:  # compact layout: slots 0..7 of the first tile only
+            assert not filled[8:].any()
+        if p == 0:  # the first tile is still written, as zeros
+            assert np.all(F[i, :16 * tf] == 0) and np.all(F[i, tf * tf:tf * tf + 16 * tf] == 0)
         np.testing.assert_allclose(np.swapaxes(uw, 1, 2).reshape(nct * 16, tf)[:, :t],
                                    np.float64(w[i]) * U, rtol=2 * EPS32, atol=0)
