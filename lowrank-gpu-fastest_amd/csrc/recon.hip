@@ -240,115 +240,103 @@ __global__ __launch_bounds__(256) void k_recon_lr(const float* __restrict__ xc,
   // patch and kq >= 2 the second. That halves the C^T MFMAs of rank <= 8
   // patches and shares the D MFMAs between two patches.
   struct Step {
-    long long ba, bb;  // patch indices (bb = ba for a single)
+    const float* fa;   // factor block of the first patch (wave-uniform)
+    unsigned offb;     // byte offset of the second patch's block from fa (0 for a single)
     int xya, zpa, xyb, zpb;
     float wa, wb;
     int kind;  // 0: single, standard layout (rank > 8); 1: single, compact; 2: pair
   };
+  constexpr long long FFB = 2LL * TF * TF * 4;  // bytes of one factor block
   int base = 0;  // first patch index of the current chunk
-  auto field = [&](int cur, int nxt, int q) {  // q - base < 128, wave-uniform
-    const int l = q - base;
-    return l < 64 ? __builtin_amdgcn_readlane(cur, l) : __builtin_amdgcn_readlane(nxt, l - 64);
-  };
   int cb, cxy, czp, cw, nbq, nxy, nzp, nw;  // this chunk's table, the next chunk's
+  // field of patch base + l, 0 <= l < 128, wave-uniform l: no branch
+  auto field = [&](int cur, int nxt, int l) {
+    return __builtin_amdgcn_readlane(l < 64 ? cur : nxt, l & 63);
+  };
   auto step_at = [&](int q) {
     Step S;
-    S.ba = field(cb, nbq, q);
-    S.xya = field(cxy, nxy, q);
-    S.zpa = field(czp, nzp, q);
-    S.wa = __int_as_float(field(cw, nw, q));
-    S.kind = (S.zpa >> 16) > kFactorPairMaxRank ? 0 : 1;
-    S.bb = S.ba;
-    S.xyb = S.xya;
-    S.zpb = S.zpa;
-    S.wb = S.wa;
-    if (S.kind == 1 && q + 1 < npatch) {
-      const int zp1 = field(czp, nzp, q + 1);
-      if ((zp1 >> 16) <= kFactorPairMaxRank) {
-        S.kind = 2;
-        S.bb = field(cb, nbq, q + 1);
-        S.xyb = field(cxy, nxy, q + 1);
-        S.zpb = zp1;
-        S.wb = __int_as_float(field(cw, nw, q + 1));
-      }
-    }
+    const int l = q - base;
+    const long long ba = field(cb, nbq, l);
+    S.xya = field(cxy, nxy, l);
+    S.zpa = field(czp, nzp, l);
+    S.wa = __int_as_float(field(cw, nw, l));
+    const long long bb = field(cb, nbq, l + 1);  // zero past the last patch
+    const int xyb = field(cxy, nxy, l + 1), zpb = field(czp, nzp, l + 1);
+    const float wb = __int_as_float(field(cw, nw, l + 1));
+    const bool ca = (S.zpa >> 16) <= kFactorPairMaxRank;
+    const bool pair = ca && q + 1 < npatch && (zpb >> 16) <= kFactorPairMaxRank &&
+                      (bb - ba) * FFB < (1LL << 31);
+    S.kind = pair ? 2 : ca ? 1 : 0;
+    S.fa = fac + ba * (2LL * TF * TF);
+    S.offb = pair ? (unsigned)((bb - ba) * FFB) : 0u;
+    S.xyb = pair ? xyb : S.xya;
+    S.zpb = pair ? zpb : S.zpa;
+    S.wb = pair ? wb : S.wa;
     return S;
   };
   // factor fragments of a step (slot tile nc of a standard single): ut[s] = Ut[row][frame
   // kq*KQ + s] (A operand), uw[tb*4 + s] = Uw[frame tb*16 + i][slot col + s] (B operand)
   auto load_fac = [&](const Step& S, int nc, float* ut, float* uw) {
     const bool std_ = S.kind == 0;
-    const long long bA = std_ || i < 8 ? S.ba : S.bb, bB = std_ || kq < 2 ? S.ba : S.bb;
+    const unsigned oA = std_ || i < 8 ? 0u : S.offb, oB = std_ || kq < 2 ? 0u : S.offb;
     const int rowA = std_ ? i : (i & 7), colB = std_ ? 4 * kq : 4 * (kq & 1);
-    const float* U = fac + bA * (2LL * TF * TF) + nc * 16 * TF + rowA * TF + kq * KQ;
+    const char* F = (const char*)S.fa;
+    const unsigned uA = oA + 4u * (unsigned)(nc * 16 * TF + rowA * TF + kq * KQ);
 #pragma unroll
     for (int v = 0; v < KV; ++v) {
-      const float4 q = *(const float4*)(U + 4 * v);
+      const float4 q = *(const float4*)(F + (uA + 16u * v));
       ut[4 * v] = q.x;
       ut[4 * v + 1] = q.y;
       ut[4 * v + 2] = q.z;
       ut[4 * v + 3] = q.w;
     }
-    const float* Wp = fac + bB * (2LL * TF * TF) + TF * TF + nc * 16 * TF + i * 16 + colB;
+    const unsigned uB = oB + 4u * (unsigned)(TF * TF + nc * 16 * TF + i * 16 + colB);
 #pragma unroll
     for (int tb = 0; tb < NTB; ++tb) {
-      const float4 q = *(const float4*)(Wp + tb * 256);
+      const float4 q = *(const float4*)(F + (uB + 1024u * tb));
       uw[tb * 4] = q.x;
       uw[tb * 4 + 1] = q.y;
       uw[tb * 4 + 2] = q.z;
       uw[tb * 4 + 3] = q.w;
     }
   };
-
-  float ut[KQ], uw[4 * NTB], utn[KQ], uwn[4 * NTB];
-  decode(lane, &cb, &cxy, &czp, &cw);
-  decode(64 + lane, &nbq, &nxy, &nzp, &nw);
+  // weight sums in patch order (step 0's wsum), output voxel (kq, r) of slab j
+  auto add_w = [&](int cx, int cy, int cz, float w) {
+    const float wy = (y0 + kq >= cy && y0 + kq < cy + g.py) ? w : 0.f;
+#pragma unroll
+    for (int j = 0; j < SX; ++j) {
+      const bool xin = x0 + j >= cx && x0 + j < cx + g.px;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const bool in = xin && z0 + r >= cz && z0 + r < cz + g.pz;
+        Wv[j][r] += in ? wy : 0.f;
+      }
+    }
+  };
+  // One step on the factors in (ut, uw), with the next step's factors loaded
+  // into (utn, uwn) under its MFMAs. Returns the next patch index.
   Step nxt_step;
-  if (npatch > 0) {
-    nxt_step = step_at(0);
-    load_fac(nxt_step, 0, utn, uwn);
-  }
-  for (int q = 0; q < npatch;) {
+  auto run_step = [&](int q, float* ut, float* uw, float* utn, float* uwn) {
     const Step S = nxt_step;
     const int qn = q + (S.kind == 2 ? 2 : 1);
-#pragma unroll
-    for (int k = 0; k < KQ; ++k) ut[k] = utn[k];
-#pragma unroll
-    for (int k = 0; k < 4 * NTB; ++k) uw[k] = uwn[k];
-    if (qn < npatch) {  // next step's factors in flight under this one's MFMAs
+    if (qn < npatch) {
       nxt_step = step_at(qn);
       load_fac(nxt_step, 0, utn, uwn);
     }
     const int cxa = S.xya & 0xFFFF, cya = S.xya >> 16, cza = S.zpa & 0xFFFF, pa = S.zpa >> 16;
     const int cxb = S.xyb & 0xFFFF, cyb = S.xyb >> 16, czb = S.zpb & 0xFFFF, pb = S.zpb >> 16;
-    // weight sums in patch order (step 0's wsum), output voxel (kq, r) of slab j
-    auto add_w = [&](int cx, int cy, int cz, float w) {
-      const bool yin = y0 + kq >= cy && y0 + kq < cy + g.py;
-#pragma unroll
-      for (int j = 0; j < SX; ++j) {
-        if (x0 + j < cx || x0 + j >= cx + g.px) continue;
-#pragma unroll
-        for (int r = 0; r < 4; ++r)
-          Wv[j][r] += (yin && z0 + r >= cz && z0 + r < cz + g.pz) ? w : 0.f;
-      }
-    };
     add_w(cxa, cya, cza, S.wa);
     if (S.kind == 2) add_w(cxb, cyb, czb, S.wb);
     // C^T row slot 4kq + r of this lane belongs to patch A, or to B (pairs, kq >= 2)
     const bool laneB = S.kind != 0 && kq >= 2;
     const bool lane_on = !laneB || S.kind == 2;
-    const int cxl = laneB ? cxb : cxa, cyl = laneB ? cyb : cya, czl = laneB ? czb : cza;
+    const int cyl = laneB ? cyb : cya, czl = laneB ? czb : cza;
     const bool myz = lane_on && yi >= cyl && yi < cyl + g.py && zi >= czl && zi < czl + g.pz;
     const int nct = S.kind == 0 ? (pa + 15) >> 4 : 1;
     for (int nc = 0; nc < nct; ++nc) {
       if (nc > 0) load_fac(S, nc, ut, uw);  // rank above 16: further slot tiles
-      int S2;
-      if (S.kind == 0) {
-        S2 = (min(16, pa - 16 * nc) + 3) >> 2;
-      } else {
-        const int pm = S.kind == 2 ? max(pa, pb) : pa;
-        S2 = (pm + 1) >> 1;
-      }
+      const int S2 = S.kind == 0 ? (min(16, pa - 16 * nc) + 3) >> 2
+                                 : ((S.kind == 2 ? max(pa, pb) : pa) + 1) >> 1;
 #pragma unroll
       for (int j = 0; j < SX; ++j) {
         const bool inA = x0 + j >= cxa && x0 + j < cxa + g.px;
@@ -357,8 +345,8 @@ __global__ __launch_bounds__(256) void k_recon_lr(const float* __restrict__ xc,
         floatx4 ct = floatx4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int s = 0; s < KQ; ++s) ct = __builtin_amdgcn_mfma_f32_16x16x4f32(ut[s], xv[j][s], ct, 0, 0, 0);
-        // a select, not a product: factor tiles of rank-0 patches hold zeros,
-        // but the mask must not depend on that
+        // a select, not a product, so that the mask does not depend on the
+        // masked-out factor values
         const bool keep = myz && (laneB ? inB : inA);
 #pragma unroll
         for (int r = 0; r < 4; ++r) ct[r] = keep ? ct[r] : 0.f;
@@ -379,7 +367,20 @@ __global__ __launch_bounds__(256) void k_recon_lr(const float* __restrict__ xc,
       cw = nw;
       decode(base + 64 + lane, &nbq, &nxy, &nzp, &nw);
     }
-    q = qn;
+    return qn;
+  };
+
+  float ua[KQ], wa_[4 * NTB], ub[KQ], wb_[4 * NTB];  // factor buffers, used in turn
+  decode(lane, &cb, &cxy, &czp, &cw);
+  decode(64 + lane, &nbq, &nxy, &nzp, &nw);
+  if (npatch > 0) {
+    nxt_step = step_at(0);
+    load_fac(nxt_step, 0, ua, wa_);
+  }
+  for (int q = 0; q < npatch;) {
+    q = run_step(q, ua, wa_, ub, wb_);
+    if (q >= npatch) break;
+    q = run_step(q, ub, wb_, ua, wa_);
   }
 
   // out = mean + D / W (0 where no patch covers the voxel), as k_normalize
