@@ -187,14 +187,11 @@ __global__ __launch_bounds__(256) void k_recon_lr(const float* __restrict__ xc,
   }
 
   floatx4 D[SX][NTB];
-  float Wv[SX][4];
+  float Wl = 0.f;  // weight sum of this lane's output voxel (see add_w)
 #pragma unroll
-  for (int j = 0; j < SX; ++j) {
+  for (int j = 0; j < SX; ++j)
 #pragma unroll
     for (int tb = 0; tb < NTB; ++tb) D[j][tb] = floatx4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int r = 0; r < 4; ++r) Wv[j][r] = 0.f;
-  }
 
   // patches meeting the tile, visited in increasing index b
   int xl, xh, yl, yh, zl, zh;
@@ -300,18 +297,13 @@ __global__ __launch_bounds__(256) void k_recon_lr(const float* __restrict__ xc,
       uw[tb * 4 + 3] = q.w;
     }
   };
-  // weight sums in patch order (step 0's wsum), output voxel (kq, r) of slab j
+  // weight sums in patch order (step 0's wsum), one output voxel per lane:
+  // lane (4j + r, kq), 4j + r < 4 SX, sums for voxel (x0 + j, y0 + kq, z0 + r)
+  const int wxl = x0 + ((i >> 2) < SX ? (i >> 2) : 0), wyl = y0 + kq, wzl = z0 + (i & 3);
   auto add_w = [&](int cx, int cy, int cz, float w) {
-    const float wy = (y0 + kq >= cy && y0 + kq < cy + g.py) ? w : 0.f;
-#pragma unroll
-    for (int j = 0; j < SX; ++j) {
-      const bool xin = x0 + j >= cx && x0 + j < cx + g.px;
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const bool in = xin && z0 + r >= cz && z0 + r < cz + g.pz;
-        Wv[j][r] += in ? wy : 0.f;
-      }
-    }
+    const bool in = (unsigned)(wxl - cx) < (unsigned)g.px && (unsigned)(wyl - cy) < (unsigned)g.py &&
+                    (unsigned)(wzl - cz) < (unsigned)g.pz;
+    Wl += in ? w : 0.f;
   };
   // One step on the factors in (ut, uw), with the next step's factors loaded
   // into (utn, uwn) under its MFMAs. Returns the next patch index.
@@ -319,10 +311,10 @@ __global__ __launch_bounds__(256) void k_recon_lr(const float* __restrict__ xc,
   auto run_step = [&](int q, float* ut, float* uw, float* utn, float* uwn) {
     const Step S = nxt_step;
     const int qn = q + (S.kind == 2 ? 2 : 1);
-    if (qn < npatch) {
-      nxt_step = step_at(qn);
-      load_fac(nxt_step, 0, utn, uwn);
-    }
+    // unconditional, so that no branch joins two wait states: past the last
+    // patch the table holds zeros and the loads read block 0, in bounds
+    nxt_step = step_at(qn);
+    load_fac(nxt_step, 0, utn, uwn);
     const int cxa = S.xya & 0xFFFF, cya = S.xya >> 16, cza = S.zpa & 0xFFFF, pa = S.zpa >> 16;
     const int cxb = S.xyb & 0xFFFF, cyb = S.xyb >> 16, czb = S.zpb & 0xFFFF, pb = S.zpb >> 16;
     add_w(cxa, cya, cza, S.wa);
@@ -383,7 +375,15 @@ __global__ __launch_bounds__(256) void k_recon_lr(const float* __restrict__ xc,
     q = run_step(q, ub, wb_, ua, wa_);
   }
 
-  // out = mean + D / W (0 where no patch covers the voxel), as k_normalize
+  // out = mean + D / W (0 where no patch covers the voxel), as k_normalize;
+  // W of voxel (j, kq, r) sits in lane (4j + r, kq)
+  float Wsum[SX][4];
+#pragma unroll
+  for (int j = 0; j < SX; ++j)
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+      Wsum[j][r] = __int_as_float(__builtin_amdgcn_ds_bpermute(4 * (16 * kq + 4 * j + r),
+                                                               __float_as_int(Wl)));
 #pragma unroll
   for (int j = 0; j < SX; ++j) {
     if (j >= ex) break;
@@ -391,7 +391,7 @@ __global__ __launch_bounds__(256) void k_recon_lr(const float* __restrict__ xc,
     for (int r = 0; r < 4; ++r) {
       if (kq >= ey || r >= ez) continue;
       const long long vox = ((long long)(x0 + j) * g.Y + (y0 + kq)) * g.Z + (z0 + r);
-      const float W = Wv[j][r];
+      const float W = Wsum[j][r];
       const float m = mean[vox];
       float* orow = out + vox * T;
 #pragma unroll
