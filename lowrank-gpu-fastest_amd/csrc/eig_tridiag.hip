@@ -440,7 +440,7 @@ __global__ __launch_bounds__(64) void k_eig64_mp(const double* __restrict__ gram
   if (Fb) zero_factor_slots(Fb, p, factor_tf(T), lane);
 }
 
-// Helpers of the register-resident eigensolver (k_tri_grid / k_eig64_rows
+// Helpers of the register-resident eigensolver (k_tri_grid / k_eig_vals / k_eig_vecs
 // below). Every loop over a row is unrolled over TP with zero padding, so no
 // register is indexed at run time.
 
@@ -589,11 +589,11 @@ __device__ void mp_rule_wave(const float* lam, float* e, float* rc, double* dsfx
 
 // ---------------------------------------------------------------------------
 // Register-resident eigensolver for T <= 48 (TP = T rounded up to 16, 32 or 48),
-// two kernels on the 8 x 8 lane grid of a wavefront: k_tri_grid reduces each
-// matrix to tridiagonal form in registers; k_eig64_rows finds its eigenvalues,
-// MP rank and top-p eigenvectors. The tridiagonalisation runs at the occupancy
-// of its own registers instead of the eigenvalue phase's. Hand-off in the
-// (clobbered) Gram buffer of each matrix:
+// three kernels on the 8 x 8 lane grid of a wavefront: k_tri_grid reduces each
+// matrix to tridiagonal form in registers; k_eig_vals finds its eigenvalues and
+// MP rank; k_eig_vecs its top-p eigenvectors. Each runs at the occupancy of its
+// own registers (the bisection holds the whole tridiagonal in every lane).
+// Hand-off in the (clobbered) Gram buffer of each matrix:
 //   row k <= T-3   reflector v_k at columns k+1 .. T-1 (v_{k+1} = 1), tau_k
 //                  at column 0 (v_k vanishes there)
 //   row T-2        the diagonal d_0 .. d_{T-1}
@@ -723,8 +723,17 @@ __global__ __launch_bounds__(64) void k_tri_grid(double* __restrict__ gram, int 
   if (gr == (T - 1) % 8 && gc == (T - 2) % 8) Gb[(T - 1) * T + T - 2] = ee2;
 }
 
+// Where k_eig_vals leaves the eigenvalues of a patch for k_eig_vecs: the last
+// 2T 32-bit words of the patch's projector block, or of its factor block (Uw of
+// the last slot tile, written only for ranks above TP - 16, after the vector
+// kernel has read them). 32-bit words: a projector block need not be 8-byte
+// aligned.
+__device__ __forceinline__ unsigned* lam_scratch(float* Pb, float* Fb, int T, int TP) {
+  return Fb ? (unsigned*)(Fb + 2LL * TP * TP - 2 * TP) : (unsigned*)(Pb + (long long)T * T - 2 * T);
+}
+
 template <int TP>
-__global__ __launch_bounds__(64) void k_eig64_rows(double* __restrict__ gram, int T, int N,
+__global__ __launch_bounds__(64) void k_eig_vals(double* __restrict__ gram, int T, int N,
                                                    long long nb, float* __restrict__ proj,
                                                    float* __restrict__ fac,
                                                    int* __restrict__ rank,
@@ -732,21 +741,14 @@ __global__ __launch_bounds__(64) void k_eig64_rows(double* __restrict__ gram, in
                                                    float* __restrict__ margin,
                                                    float* __restrict__ lam_desc,
                                                    int* __restrict__ info) {
-  constexpr int LD = TP + 1;
-  constexpr int R = TP / 8;  // register columns per lane of the 8 x 8 lane grid
-  __shared__ double s_d[TP], s_e[TP], s_e2[TP], s_tau[TP];
+  __shared__ double s_d[TP], s_e[TP], s_e2[TP];
   __shared__ double s_lam[TP];
-  __shared__ double s_dm[kRowGroup * LD];
-  __shared__ double s_z[kRowGroup * LD];  // D+ pivots, then the vector
-  __shared__ double s_h[kHChunk * TP];    // staged reflectors
   __shared__ double s_ds[TP], s_e2s[TP], s_dsfx[TP];
   __shared__ float s_lamf[TP], s_ef[TP], s_rcf[TP];
 
   const long long b = blockIdx.x;
   if (b >= nb) return;
   const int lane = threadIdx.x;
-  const bool row_ok = lane < T;
-  const int gr = lane >> 3, gc = lane & 7;  // lane-grid row and column
   double* Gb = gram + b * (long long)T * T;
   // the tridiagonal form and the reflector scales left by k_tri_grid
   bool finite = true;
@@ -755,7 +757,6 @@ __global__ __launch_bounds__(64) void k_eig64_rows(double* __restrict__ gram, in
     const double el = lane + 1 < T ? Gb[(T - 1) * T + lane] : 0.0;
     s_d[lane] = dl;
     s_e[lane] = el;
-    if (lane + 2 < T) s_tau[lane] = Gb[lane * T];
     finite = isfinite(dl) && isfinite(el);
   }
   const bool all_finite = __any(!finite) == 0;
@@ -868,6 +869,75 @@ __global__ __launch_bounds__(64) void k_eig64_rows(double* __restrict__ gram, in
       if (info) info[b] = 0;
     }
   }
+  // the eigenvalues for k_eig_vecs, exactly (two 32-bit halves each)
+  if ((Pb || Fb) && p > 0 && lane < T) {
+    unsigned* sc = lam_scratch(Pb, Fb, T, TP);
+    const long long bits = __double_as_longlong(s_lam[lane]);
+    sc[2 * lane] = (unsigned)bits;
+    sc[2 * lane + 1] = (unsigned)(bits >> 32);
+  }
+}
+
+// Top-p eigenvectors of the tridiagonal left by k_tri_grid, at the eigenvalues
+// and rank left by k_eig_vals: twisted factorisation, Gram-Schmidt inside the
+// group, back-transform on the lane grid, factors or projector rows. Its own
+// kernel so that it runs at the occupancy of its few registers (the bisection
+// keeps the whole tridiagonal in registers): these phases are latency-bound.
+template <int TP>
+__global__ __launch_bounds__(64) void k_eig_vecs(const double* __restrict__ gram, int T,
+                                                 long long nb, float* __restrict__ proj,
+                                                 float* __restrict__ fac,
+                                                 const int* __restrict__ rank) {
+  constexpr int LD = TP + 1;
+  constexpr int R = TP / 8;  // register columns per lane of the 8 x 8 lane grid
+  __shared__ double s_d[TP], s_e[TP], s_e2[TP], s_tau[TP];
+  __shared__ double s_lam[TP];
+  __shared__ double s_dm[kRowGroup * LD];
+  __shared__ double s_z[kRowGroup * LD];  // D+ pivots, then the vector
+  __shared__ double s_h[kHChunk * TP];    // staged reflectors
+
+  const long long b = blockIdx.x;
+  if (b >= nb) return;
+  const int lane = threadIdx.x;
+  const bool row_ok = lane < T;
+  const int gr = lane >> 3, gc = lane & 7;  // lane-grid row and column
+  const double* Gb = gram + b * (long long)T * T;
+  float* Pb = proj ? proj + b * (long long)T * T : nullptr;
+  float* Fb = fac ? fac + b * (2LL * TP * TP) : nullptr;  // TP = 16*ceil(T/16)
+  const int p = __builtin_amdgcn_readfirstlane(rank[b]);
+  if (p > 0) {
+    const unsigned* sc = lam_scratch(Pb, Fb, T, TP);
+    if (lane < T) {
+      s_d[lane] = Gb[(T - 2) * T + lane];
+      s_e[lane] = lane + 1 < T ? Gb[(T - 1) * T + lane] : 0.0;
+      if (lane + 2 < T) s_tau[lane] = Gb[lane * T];
+      s_lam[lane] = __longlong_as_double((long long)sc[2 * lane] |
+                                         ((long long)sc[2 * lane + 1] << 32));
+    }
+    __syncthreads();
+    // the same split and scale as k_eig_vals (the data are the same)
+    if (lane + 1 < T) {
+      const double thr = 8.8817841970012523e-16 * (fabs(s_d[lane]) + fabs(s_d[lane + 1]));
+      if (fabs(s_e[lane]) <= thr) s_e[lane] = 0.0;
+    }
+    __syncthreads();
+  }
+  double lo_g = INFINITY, hi_g = -INFINITY, emax2 = 0.0;
+  if (p > 0 && lane < T) {
+    const double el = lane > 0 ? fabs(s_e[lane - 1]) : 0.0;
+    const double er = fabs(s_e[lane]);
+    lo_g = s_d[lane] - el - er;
+    hi_g = s_d[lane] + el + er;
+    emax2 = s_e[lane] * s_e[lane];
+    s_e2[lane] = emax2;
+  }
+  lo_g = wave_min(lo_g);
+  hi_g = wave_max(hi_g);
+  emax2 = wave_max(emax2);
+  const double u = 1.1102230246251565e-16;
+  const double bnorm = fmax(fabs(lo_g), fabs(hi_g));
+  const double pivmin = 2.2250738585072014e-308 * fmax(1.0, emax2);
+  __syncthreads();
 
   // ---- 4./5. top-p eigenvectors, back-transform, projector rows -----------------
   const double pvfloor = fmax(u * bnorm, 2.2250738585072014e-308);
@@ -1012,9 +1082,13 @@ static int launch_rows(double* gram, long long nb, int T, int N, float* proj, fl
   hipLaunchKernelGGL((k_tri_grid<TP>), dim3((unsigned)nb), dim3(64), 0, st, gram, T, nb);
   int rc = launch_status("tridiagonalisation kernel");
   if (rc) return rc;
-  hipLaunchKernelGGL((k_eig64_rows<TP>), dim3((unsigned)nb), dim3(64), 0, st, gram, T, N, nb,
+  hipLaunchKernelGGL((k_eig_vals<TP>), dim3((unsigned)nb), dim3(64), 0, st, gram, T, N, nb,
                      proj, fac, rank, weight, margin, lam_desc, info);
-  return launch_status("eig64 (register) kernel");
+  if ((rc = launch_status("eigenvalue kernel"))) return rc;
+  if (!proj && !fac) return MPPCA_OK;
+  hipLaunchKernelGGL((k_eig_vecs<TP>), dim3((unsigned)nb), dim3(64), 0, st, gram, T, nb, proj,
+                     fac, rank);
+  return launch_status("eigenvector kernel");
 }
 
 int launch_eig64_mp(double* gram, long long nb, int T, int N, float* proj, float* fac, int* rank,
