@@ -745,6 +745,8 @@ __global__ __launch_bounds__(64) void k_eig_vals(double* __restrict__ gram, int 
   __shared__ double s_lam[TP];
   __shared__ double s_ds[TP], s_e2s[TP], s_dsfx[TP];
   __shared__ float s_lamf[TP], s_ef[TP], s_rcf[TP];
+  __shared__ double s_sec[65];  // 64-section points
+  __shared__ int s_cnt[64];     // eigenvalues below each point
 
   const long long b = blockIdx.x;
   if (b >= nb) return;
@@ -821,6 +823,21 @@ __global__ __launch_bounds__(64) void k_eig_vals(double* __restrict__ gram, int 
       dr[i] = i < T ? s_ds[i] : 4.0;
       er[i] = i + 1 < T ? s_e2s[i] : 0.0;
     }
+    // 64-section of the Gershgorin interval: lane j counts at point j + 1 of 64
+    // equal steps (point 0 = lo, point 64 = hi), one count for 6 bits of every
+    // bracket; an exact zero minor takes the substituting count
+    {
+      const double L = lo_g * inv, H = hi_g * inv;
+      const double xj = lane == 63 ? H : fma((double)(lane + 1) * 0x1p-6, H - L, L);
+      bool zero;
+      int cnt = sturm_count_fast<TP>(dr, er, xj, &zero);
+      if (zero) cnt = sturm_count3(s_ds, s_e2s, T, xj);
+      if (lane == 63) cnt = T;
+      s_cnt[lane] = cnt;
+      s_sec[lane + 1] = xj;
+      if (lane == 0) s_sec[0] = L;
+    }
+    __syncthreads();
     if (lane < T) {
       // Bisection to 2^-33 of the scaled norm brackets each eigenvalue (the MP
       // rule needs ~2^-24 relative to lambda_max); two Newton steps on the
@@ -831,7 +848,17 @@ __global__ __launch_bounds__(64) void k_eig_vals(double* __restrict__ gram, int 
       // on that); a vanishing derivative keeps the current point.
       const int kk = T - 1 - lane;  // kk-th smallest
       const double tol = 0x1p-33;   // of the scaled problem, |T| <= 1
-      double lo = lo_g * inv, hi = hi_g * inv;
+      double lo = s_sec[0], hi = s_sec[64];
+      {
+        // bracket from the 64-section: the last section point whose count is at
+        // most kk, and the next one (counts grow with the point)
+        int a = 0;  // s_cnt[a - 1] <= kk < s_cnt[a + 63]; find the first j with cnt_j > kk
+#pragma unroll
+        for (int step = 32; step >= 1; step >>= 1)
+          if (s_cnt[a + step - 1] <= kk) a += step;
+        lo = s_sec[a];
+        hi = s_sec[a + 1];
+      }
       for (int it = 0; it < 96 && hi - lo > tol; ++it) {
         const double x = 0.5 * (lo + hi);
         bool zero;
