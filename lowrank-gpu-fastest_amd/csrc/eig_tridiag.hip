@@ -1054,12 +1054,28 @@ __global__ __launch_bounds__(64) void k_eig_vecs(const double* __restrict__ gram
     for (int n = 0; n < R; ++n)
       zr[n] = (gr < gn && gc + 8 * n < T) ? s_z[gr * LD + gc + 8 * n] : 0.0;
     __syncthreads();
+    // reflector rows klo .. kc of a chunk, column `lane` (0 at column 0, where
+    // tau sits, and past T): the next chunk's rows are loaded into registers
+    // while the current chunk is applied. Unconditional, clamped loads.
+    const int cl = lane > 0 && lane < T ? lane : T - 1;
+    double hn[kHChunk];
+    auto fetch_chunk = [&](int kc) {
+#pragma unroll
+      for (int r = 0; r < kHChunk; ++r) {
+        const int row = kc - kHChunk + 1 + r;
+        hn[r] = Gb[(row > 0 ? row : 0) * T + cl];
+      }
+    };
+    fetch_chunk(T - 3);
     for (int kc = T - 3; kc >= 0; kc -= kHChunk) {
       const int klo = kc - kHChunk + 1 > 0 ? kc - kHChunk + 1 : 0;
-      for (int r = 0; r <= kc - klo; ++r)
-        for (int c = lane; c < TP; c += 64)
-          s_h[r * TP + c] = (c > 0 && c < T) ? Gb[(klo + r) * T + c] : 0.0;  // column 0: tau
+      const int r0 = klo - (kc - kHChunk + 1);  // first register row of the chunk
+      if (lane < TP)
+#pragma unroll
+        for (int r = 0; r < kHChunk; ++r)
+          if (r >= r0) s_h[(r - r0) * TP + lane] = lane > 0 && lane < T ? hn[r] : 0.0;
       __syncthreads();
+      fetch_chunk(kc - kHChunk > 0 ? kc - kHChunk : 0);  // rows of the next chunk
       for (int k = kc; k >= klo; --k) {
         const double tk = s_tau[k];
         if (tk == 0.0) continue;
