@@ -18,6 +18,13 @@ best of N); this module is the request-driven form around the same
   recently used), and otherwise the request waits for a set in use to return.
 * ``poisson_run`` drives a Server with Poisson arrivals at a fixed rate and
   returns per-request latencies (arrival to result), p50/p99 and volumes/s.
+* ``freeze_gc`` (default on): Python's cyclic garbage collector stops every
+  thread while it scans all tracked objects, and with torch loaded a full
+  pass takes tens of milliseconds: at 100 requests/s it set the p99 latency
+  (40 ms against a 5 ms median). The Server collects once and freezes the
+  objects that exist when it starts (``gc.freeze``), so later passes scan only
+  what the requests create. Frozen objects are still freed by reference
+  counting; only cycles among them are never collected.
 
 Every request's output is the same as denoising it alone: the kernels are
 deterministic per volume (bit for bit on the factor route; up to float atomic
@@ -27,6 +34,7 @@ summation order on the projector route, T > 64 or the fp32 form).
 from __future__ import annotations
 
 import collections
+import gc
 import queue
 import threading
 import time
@@ -57,7 +65,7 @@ class Server:
     """Concurrent host->host denoising of submitted volumes under an HBM budget."""
 
     def __init__(self, workers: int = 2, hbm_budget: int | None = None, form: str = "plain",
-                 nslabs: int = 4, nthreads: int = 4):
+                 nslabs: int = 4, nthreads: int = 4, freeze_gc: bool = True):
         import torch
 
         if workers < 1:
@@ -76,6 +84,9 @@ class Server:
         self._closed = False
         self.admitted = 0
         self.evictions = 0
+        if freeze_gc:
+            gc.collect()
+            gc.freeze()
         self._threads = [threading.Thread(target=self._work, args=(i,), daemon=True)
                          for i in range(workers)]
         for th in self._threads:
