@@ -268,7 +268,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NTB >= 4 ? 
     S.offb = pair ? (unsigned)((bb - ba) * FFB) : 0u;
     S.xyb = pair ? xyb : S.xya;
     S.zpb = pair ? zpb : S.zpa;
-    S.wb = pair ? wb : S.wa;
+    S.wb = pair ? wb : 0.f;  // a single adds no second weight
     return S;
   };
   // factor fragments of a step (slot tile nc of a standard single): ut[s] = Ut[row][frame
@@ -301,8 +301,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NTB >= 4 ? 
   // lane (4j + r, kq), 4j + r < 4 SX, sums for voxel (x0 + j, y0 + kq, z0 + r)
   const int wxl = x0 + ((i >> 2) < SX ? (i >> 2) : 0), wyl = y0 + kq, wzl = z0 + (i & 3);
   auto add_w = [&](int cx, int cy, int cz, float w) {
-    const bool in = (unsigned)(wxl - cx) < (unsigned)g.px && (unsigned)(wyl - cy) < (unsigned)g.py &&
-                    (unsigned)(wzl - cz) < (unsigned)g.pz;
+    // non-short-circuit &: compares and a select, no exec-masked branch
+    const bool in = ((unsigned)(wxl - cx) < (unsigned)g.px) & ((unsigned)(wyl - cy) < (unsigned)g.py) &
+                    ((unsigned)(wzl - cz) < (unsigned)g.pz);
     Wl += in ? w : 0.f;
   };
   // One step on the factors in (ut, uw), with the next step's factors loaded
@@ -318,12 +319,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NTB >= 4 ? 
     const int cxa = S.xya & 0xFFFF, cya = S.xya >> 16, cza = S.zpa & 0xFFFF, pa = S.zpa >> 16;
     const int cxb = S.xyb & 0xFFFF, cyb = S.xyb >> 16, czb = S.zpb & 0xFFFF, pb = S.zpb >> 16;
     add_w(cxa, cya, cza, S.wa);
-    if (S.kind == 2) add_w(cxb, cyb, czb, S.wb);
+    add_w(cxb, cyb, czb, S.wb);  // 0 for a single: W + 0 = W exactly
     // C^T row slot 4kq + r of this lane belongs to patch A, or to B (pairs, kq >= 2)
     const bool laneB = S.kind != 0 && kq >= 2;
     const bool lane_on = !laneB || S.kind == 2;
     const int cyl = laneB ? cyb : cya, czl = laneB ? czb : cza;
-    const bool myz = lane_on && yi >= cyl && yi < cyl + g.py && zi >= czl && zi < czl + g.pz;
+    const bool myz = lane_on & ((unsigned)(yi - cyl) < (unsigned)g.py) &
+                     ((unsigned)(zi - czl) < (unsigned)g.pz);
     const int nct = S.kind == 0 ? (pa + 15) >> 4 : 1;
     for (int nc = 0; nc < nct; ++nc) {
       if (nc > 0) load_fac(S, nc, ut, uw);  // rank above 16: further slot tiles
@@ -339,7 +341,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NTB >= 4 ? 
         for (int s = 0; s < KQ; ++s) ct = __builtin_amdgcn_mfma_f32_16x16x4f32(ut[s], xv[j][s], ct, 0, 0, 0);
         // a select, not a product, so that the mask does not depend on the
         // masked-out factor values
-        const bool keep = myz && (laneB ? inB : inA);
+        const bool keep = myz & (laneB ? inB : inA);
 #pragma unroll
         for (int r = 0; r < 4; ++r) ct[r] = keep ? ct[r] : 0.f;
 #pragma unroll
