@@ -446,6 +446,7 @@ __global__ __launch_bounds__(64) void k_eig64_mp(const double* __restrict__ gram
 
 constexpr int kRowGroup = 8;  // eigenvectors per pass (one lane each)
 constexpr int kHChunk = 8;    // reflectors staged through LDS per back-transform pass
+static_assert(kHChunk >= kRowGroup, "the twisted factorisation keeps |gamma| of each vector in s_h");
 
 // Number of eigenvalues smaller than x of the tridiagonal with diagonal ds and
 // squared off-diagonal e2s, both scaled so that |ds|, |e| <= 1. Division-free
