@@ -130,13 +130,17 @@ __device__ __forceinline__ void touch_range(int a0, int ext, int S, int p, int s
 //   D  += C * Uw^T   (K = slots, taken in the order slot 4q + s at step s, so
 //                    the registers of C are the A operand as they stand; steps
 //                    with s >= ceil(p/4) only meet empty slots and are skipped)
-// D (16 voxels x TF frames per slab) and W (weight sum per voxel) live in
-// registers; the wave writes out = mean + D / W once. Patches are visited in
-// increasing index, so W adds the weights in the order step 0 accumulates wsum
-// (step0_numpy_loop.py:36-39).
+// Two consecutive patches of rank <= 8 (compact layout, slots 0..7) share one
+// step: the second patch's slots take rows 8..15 of C^T (see run_step below).
+// D (16 voxels x TF frames per slab) and W (weight sum per voxel, one voxel per
+// lane) live in registers; the wave writes out = mean + D / W once. Patches
+// are visited in increasing index, so W adds the weights in the order step 0
+// accumulates wsum (step0_numpy_loop.py:36-39).
+// 128 VGPRs at T <= 48 give 4 waves per SIMD.
 // ---------------------------------------------------------------------------
 template <int NTB, int SX>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NTB >= 4 ? 1 : 4))) void k_recon_lr(const float* __restrict__ xc,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NTB >= 4 ? 1 : 4)))
+void k_recon_lr(const float* __restrict__ xc,
                                                   const float* __restrict__ fac,
                                                   const int* __restrict__ rank,
                                                   const float* __restrict__ weight,
