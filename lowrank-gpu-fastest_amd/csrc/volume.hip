@@ -44,6 +44,7 @@ int zero_fill(float* p, long long n, hipStream_t st) {
 // ---------------------------------------------------------------------------
 constexpr int kCentreVox = 64;
 
+template <bool VEC>
 __global__ __launch_bounds__(256) void k_centre(const float* __restrict__ vol,
                                                 float* __restrict__ xc,
                                                 float* __restrict__ mean, long long nvox,
@@ -54,33 +55,66 @@ __global__ __launch_bounds__(256) void k_centre(const float* __restrict__ vol,
   const int nv = (int)(nvox - v0 < kCentreVox ? nvox - v0 : kCentreVox);
   const int ne = nv * T;
   const float* src = vol + v0 * T;
-  for (int e = threadIdx.x; e < ne; e += blockDim.x) s_x[e] = src[e];
+  // VEC: T % 4 == 0 and 16-byte aligned rows, so a float4 never straddles two
+  // voxels and global and LDS traffic moves 16 bytes per lane
+  if (VEC) {
+    const float4* s4 = (const float4*)src;
+    float4* x4 = (float4*)s_x;
+    for (int e = threadIdx.x; e < ne / 4; e += blockDim.x) x4[e] = s4[e];
+  } else {
+    for (int e = threadIdx.x; e < ne; e += blockDim.x) s_x[e] = src[e];
+  }
   __syncthreads();
   if (threadIdx.x < nv) {
+    // four interleaved partial sums over t = 0, 4, 8, ... (+1, +2, +3)
     const float* r = s_x + threadIdx.x * T;
     float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
     int t = 0;
-    for (; t + 4 <= T; t += 4) {
-      s0 += r[t];
-      s1 += r[t + 1];
-      s2 += r[t + 2];
-      s3 += r[t + 3];
+    if (VEC) {
+      for (; t + 4 <= T; t += 4) {
+        const float4 q = *(const float4*)(r + t);
+        s0 += q.x;
+        s1 += q.y;
+        s2 += q.z;
+        s3 += q.w;
+      }
+    } else {
+      for (; t + 4 <= T; t += 4) {
+        s0 += r[t];
+        s1 += r[t + 1];
+        s2 += r[t + 2];
+        s3 += r[t + 3];
+      }
+      for (; t < T; ++t) s0 += r[t];
     }
-    for (; t < T; ++t) s0 += r[t];
     const float m = ((s0 + s1) + (s2 + s3)) / (float)T;
     s_m[threadIdx.x] = m;
     mean[v0 + threadIdx.x] = m;
   }
   __syncthreads();
   float* dst = xc + v0 * T;
-  for (int e = threadIdx.x; e < ne; e += blockDim.x) dst[e] = s_x[e] - s_m[e / T];
+  if (VEC) {
+    const float4* x4 = (const float4*)s_x;
+    float4* d4 = (float4*)dst;
+    for (int e = threadIdx.x; e < ne / 4; e += blockDim.x) {
+      const float m = s_m[(4 * e) / T];
+      const float4 q = x4[e];
+      d4[e] = make_float4(q.x - m, q.y - m, q.z - m, q.w - m);
+    }
+  } else {
+    for (int e = threadIdx.x; e < ne; e += blockDim.x) dst[e] = s_x[e] - s_m[e / T];
+  }
 }
 
 int launch_centre(const float* vol, float* xc, float* mean, long long nvox, int T,
                   hipStream_t st) {
   const unsigned blocks = (unsigned)((nvox + kCentreVox - 1) / kCentreVox);
   const size_t lds = (size_t)kCentreVox * (T + 1) * sizeof(float);
-  hipLaunchKernelGGL(k_centre, dim3(blocks), dim3(256), lds, st, vol, xc, mean, nvox, T);
+  const bool vec = T % 4 == 0 && (uintptr_t)vol % 16 == 0 && (uintptr_t)xc % 16 == 0;
+  if (vec)
+    hipLaunchKernelGGL(k_centre<true>, dim3(blocks), dim3(256), lds, st, vol, xc, mean, nvox, T);
+  else
+    hipLaunchKernelGGL(k_centre<false>, dim3(blocks), dim3(256), lds, st, vol, xc, mean, nvox, T);
   return launch_status("centre kernel");
 }
 
