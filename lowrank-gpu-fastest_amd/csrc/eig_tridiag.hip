@@ -733,39 +733,229 @@ __device__ __forceinline__ unsigned* lam_scratch(float* Pb, float* Fb, int T, in
   return Fb ? (unsigned*)(Fb + 2LL * TP * TP - 2 * TP) : (unsigned*)(Pb + (long long)T * T - 2 * T);
 }
 
+// Matrices a wavefront's 64 (matrix, eigenvalue) lanes can span.
+__host__ __device__ constexpr int vals_slots(int tp) { return tp == 16 ? 33 : tp == 32 ? 5 : 3; }
+
+// Eigenvalues of the tridiagonals left by k_tri_grid, packed: lane l of
+// wavefront w takes pair q = 64 w + l of the nb * T (matrix, eigenvalue) pairs,
+// matrix q / T, j = q % T-th largest eigenvalue, so no lane idles when T < 64.
+// Each lane keeps its matrix's scaled tridiagonal in registers (split, scaled
+// and bounded exactly as k_eig_vecs recomputes it). The lanes of one matrix in
+// the wavefront first count at n equal steps of its Gershgorin interval (an
+// n-section for n lanes), then each bisects its own bracket to 2^-33 of the
+// scaled norm and takes two Newton steps (see below). The eigenvalues go,
+// exactly, to the end of the patch's output block for k_eig_vecs, which runs
+// the MP rule. A matrix with a non-finite entry is left to k_eig_vecs.
 template <int TP>
-__global__ __launch_bounds__(64) void k_eig_vals(double* __restrict__ gram, int T, int N,
+__global__ __launch_bounds__(64) void k_eig_vals(const double* __restrict__ gram, int T,
                                                    long long nb, float* __restrict__ proj,
-                                                   float* __restrict__ fac,
-                                                   int* __restrict__ rank,
-                                                   float* __restrict__ weight,
-                                                   float* __restrict__ margin,
-                                                   float* __restrict__ lam_desc,
-                                                   int* __restrict__ info) {
-  __shared__ double s_d[TP], s_e[TP], s_e2[TP];
+                                                   float* __restrict__ fac) {
+  constexpr int NS = vals_slots(TP);
+  __shared__ double s_ds[NS][TP], s_e2s[NS][TP];  // for the substituting recount
+  __shared__ double s_pt[64];                     // n-section points
+  __shared__ int s_cnt[64];                       // eigenvalues below each point
+  __shared__ double s_lo[NS], s_hi[NS], s_bsc[NS];  // scaled interval, norm
+  __shared__ int s_fin[NS];                       // all entries finite
+
+  const int lane = threadIdx.x;
+  const long long npair = nb * T;
+  const long long q0 = (long long)blockIdx.x * 64;
+  if (q0 >= npair) return;
+  const bool live = q0 + lane < npair;
+  const long long q = live ? q0 + lane : npair - 1;
+  const long long b = q / T;
+  const int j = (int)(q - b * T);  // j-th largest
+  const int kk = T - 1 - j;        // kk-th smallest
+  const int slot = (int)(b - q0 / T);
+  // this matrix's lanes in the wavefront: [ls, ls + ns)
+  const long long mb0 = b * T, mb1 = (b + 1) * T < npair ? (b + 1) * T : npair;
+  const int ls = (int)((mb0 > q0 ? mb0 : q0) - q0);
+  const int ns = (int)((mb1 < q0 + 64 ? mb1 : q0 + 64) - q0) - ls;
+
+  // The wavefront's matrices one after another, lanes as rows: the tridiagonal
+  // (d in row T-2, e in row T-1), split where |e_i| <= 8u (|d_i| + |d_{i+1}|)
+  // (a backward perturbation of the order of the reduction's own rounding,
+  // after which an exactly repeated eigenvalue lives in separate blocks),
+  // Gershgorin bounds and norm, and the scaled form in LDS.
+  const long long bfirst = q0 / T, blast = (q0 + 63 < npair ? q0 + 63 : npair - 1) / T;
+  for (long long bs = bfirst; bs <= blast; ++bs) {
+    const int sl = (int)(bs - bfirst);
+    const double* Gs = gram + bs * (long long)T * T;
+    double dl = 0.0, el = 0.0;
+    bool fin = true;
+    if (lane < T) {
+      dl = Gs[(T - 2) * T + lane];
+      el = lane + 1 < T ? Gs[(T - 1) * T + lane] : 0.0;
+      fin = isfinite(dl) && isfinite(el);
+      s_ds[sl][lane] = dl;
+    }
+    const bool fin_all = __any(!fin) == 0;
+    __syncthreads();
+    if (lane + 1 < T) {
+      const double thr = 8.8817841970012523e-16 * (fabs(dl) + fabs(s_ds[sl][lane + 1]));
+      if (fabs(el) <= thr) el = 0.0;
+    }
+    s_e2s[sl][lane < TP ? lane : 0] = 0.0;  // scratch for the left neighbour's |e|
+    __syncthreads();
+    if (lane < T) s_e2s[sl][lane] = fabs(el);
+    __syncthreads();
+    double lo_g = INFINITY, hi_g = -INFINITY, emax2 = 0.0;
+    if (lane < T) {
+      const double elo = lane > 0 ? s_e2s[sl][lane - 1] : 0.0;
+      const double ehi = fabs(el);
+      lo_g = dl - elo - ehi;
+      hi_g = dl + elo + ehi;
+      emax2 = el * el;
+    }
+    lo_g = wave_min(lo_g);
+    hi_g = wave_max(hi_g);
+    emax2 = wave_max(emax2);
+    const double u = 1.1102230246251565e-16;
+    const double bnorm = fmax(fabs(lo_g), fabs(hi_g));
+    const double pivmin = 2.2250738585072014e-308 * fmax(1.0, emax2);
+    const double slack = 2.0 * u * bnorm * T + pivmin;
+    const double bsc = bnorm > 0.0 ? bnorm : 1.0;
+    const double inv = 1.0 / bsc;
+    __syncthreads();
+    if (lane < TP) {
+      // scaled, padded with 4 from row T on (a padded row multiplies every
+      // minor by 4 - x > 0 and couples to nothing, so it never changes a sign)
+      s_ds[sl][lane] = lane < T ? dl * inv : 4.0;
+      s_e2s[sl][lane] = lane + 1 < T ? (el * el) * (inv * inv) : 0.0;
+    }
+    if (lane == 0) {
+      s_lo[sl] = (lo_g - slack) * inv;
+      s_hi[sl] = (hi_g + slack) * inv;
+      s_bsc[sl] = bsc;
+      s_fin[sl] = fin_all ? 1 : 0;
+    }
+    __syncthreads();
+  }
+  double dr[TP], er[TP];
+#pragma unroll
+  for (int i = 0; i < TP; ++i) {
+    dr[i] = s_ds[slot][i];
+    er[i] = s_e2s[slot][i];
+  }
+  const bool finite = s_fin[slot] != 0;
+  const double bsc = s_bsc[slot];
+  // n-section of the Gershgorin interval over this matrix's ns lanes: lane
+  // ls + m counts at point m + 1 of ns equal steps (the last point is hi)
+  const double L = s_lo[slot], H = s_hi[slot];
+  {
+    const int m = lane - ls;
+    const double xm = m + 1 == ns ? H : fma((double)(m + 1) / (double)ns, H - L, L);
+    bool zero;
+    int cnt = sturm_count_fast<TP>(dr, er, xm, &zero);
+    if (zero) cnt = sturm_count3(s_ds[slot], s_e2s[slot], T, xm);
+    if (m + 1 == ns) cnt = T;
+    s_cnt[lane] = cnt;
+    s_pt[lane] = xm;
+  }
+  __syncthreads();
+  double lam = NAN;
+  if (live && finite) {
+    // bracket: the first section point whose count exceeds kk, and the one before
+    int lo_i = 0, n = ns;  // lower bound over s_cnt[ls .. ls + ns)
+    while (n > 0) {
+      const int half = n >> 1;
+      if (s_cnt[ls + lo_i + half] <= kk) {
+        lo_i += half + 1;
+        n -= half + 1;
+      } else {
+        n = half;
+      }
+    }
+    double lo = lo_i > 0 ? s_pt[ls + lo_i - 1] : L, hi = s_pt[ls + lo_i];
+    // Bisection to 2^-33 of the scaled norm brackets each eigenvalue (the MP
+    // rule needs ~2^-24 relative to lambda_max); two Newton steps on the
+    // characteristic polynomial, clamped to the bracket (a root near its edge
+    // makes the first step overshoot), then reach full precision for every
+    // eigenvalue separated from its neighbours. Copies of a cluster share one
+    // bracket and so take identical steps (the cluster test of k_eig_vecs
+    // relies on that); a vanishing derivative keeps the current point.
+    const double tol = 0x1p-33;
+    for (int it = 0; it < 96 && hi - lo > tol; ++it) {
+      const double x = 0.5 * (lo + hi);
+      bool zero;
+      int cnt = sturm_count_fast<TP>(dr, er, x, &zero);
+      if (zero) cnt = sturm_count3(s_ds[slot], s_e2s[slot], T, x);  // exact zero minor: rare
+      if (cnt > kk)
+        hi = x;
+      else
+        lo = x;
+    }
+    double x = 0.5 * (lo + hi);
+#pragma unroll
+    for (int st = 0; st < 2; ++st) {
+      const double xn = x - newton_corr_reg<TP>(dr, er, T, x);
+      x = isfinite(xn) ? fmin(fmax(xn, lo), hi) : x;
+    }
+    lam = x * bsc;
+  }
+  // the eigenvalue for k_eig_vecs, exactly (two 32-bit halves each); the
+  // matrix and index are derived again from the lane (fewer live registers)
+  if (live && finite) {
+    const long long qq = q0 + lane, bb = qq / T;
+    const int jj = (int)(qq - bb * T);
+    float* Pb = proj ? proj + bb * (long long)T * T : nullptr;
+    float* Fb = fac ? fac + bb * (2LL * TP * TP) : nullptr;
+    unsigned* sc = lam_scratch(Pb, Fb, T, TP);
+    const long long bits = __double_as_longlong(lam);
+    sc[2 * jj] = (unsigned)bits;
+    sc[2 * jj + 1] = (unsigned)(bits >> 32);
+  }
+}
+
+// Top-p eigenvectors of the tridiagonal left by k_tri_grid, at the eigenvalues
+// and rank left by k_eig_vals: twisted factorisation, Gram-Schmidt inside the
+// group, back-transform on the lane grid, factors or projector rows. Its own
+// kernel so that it runs at the occupancy of its few registers (the bisection
+// keeps the whole tridiagonal in registers): these phases are latency-bound.
+template <int TP>
+__global__ __launch_bounds__(64) void k_eig_vecs(const double* __restrict__ gram, int T, int N,
+                                                 long long nb, float* __restrict__ proj,
+                                                 float* __restrict__ fac,
+                                                 int* __restrict__ rank,
+                                                 float* __restrict__ weight,
+                                                 float* __restrict__ margin,
+                                                 float* __restrict__ lam_desc,
+                                                 int* __restrict__ info) {
+  constexpr int LD = TP + 1;
+  constexpr int R = TP / 8;  // register columns per lane of the 8 x 8 lane grid
+  __shared__ double s_d[TP], s_e[TP], s_e2[TP], s_tau[TP];
   __shared__ double s_lam[TP];
-  __shared__ double s_ds[TP], s_e2s[TP], s_dsfx[TP];
-  __shared__ float s_lamf[TP], s_ef[TP], s_rcf[TP];
-  __shared__ double s_sec[65];  // 64-section points
-  __shared__ int s_cnt[64];     // eigenvalues below each point
+  __shared__ double s_dm[kRowGroup * LD];
+  // the MP rule's arrays share s_dm, which the vectors use only after it
+  static_assert(kRowGroup * LD >= 3 * TP, "MP scratch fits in s_dm");
+  double* s_dsfx = s_dm;
+  float* s_lamf = (float*)(s_dm + TP);
+  float* s_ef = s_lamf + TP;
+  float* s_rcf = s_ef + TP;
+  __shared__ double s_z[kRowGroup * LD];  // D+ pivots, then the vector
+  __shared__ double s_h[kHChunk * TP];    // staged reflectors
 
   const long long b = blockIdx.x;
   if (b >= nb) return;
   const int lane = threadIdx.x;
-  double* Gb = gram + b * (long long)T * T;
-  // the tridiagonal form and the reflector scales left by k_tri_grid
-  bool finite = true;
-  if (lane < T) {
-    const double dl = Gb[(T - 2) * T + lane];
-    const double el = lane + 1 < T ? Gb[(T - 1) * T + lane] : 0.0;
-    s_d[lane] = dl;
-    s_e[lane] = el;
-    finite = isfinite(dl) && isfinite(el);
-  }
-  const bool all_finite = __any(!finite) == 0;
+  const bool row_ok = lane < T;
+  const int gr = lane >> 3, gc = lane & 7;  // lane-grid row and column
+  const double* Gb = gram + b * (long long)T * T;
   float* Pb = proj ? proj + b * (long long)T * T : nullptr;
   float* Fb = fac ? fac + b * (2LL * TP * TP) : nullptr;  // TP = 16*ceil(T/16)
-  if (!all_finite) {
+  bool finite = true;
+  unsigned lam_lo = 0u, lam_hi = 0u;
+  if (lane < T) {
+    // the eigenvalues left by k_eig_vals (largest first), loaded with the rest
+    const unsigned* sc = lam_scratch(Pb, Fb, T, TP);
+    lam_lo = sc[2 * lane];
+    lam_hi = sc[2 * lane + 1];
+    s_d[lane] = Gb[(T - 2) * T + lane];
+    s_e[lane] = lane + 1 < T ? Gb[(T - 1) * T + lane] : 0.0;
+    if (lane + 2 < T) s_tau[lane] = Gb[lane * T];
+    finite = isfinite(s_d[lane]) && isfinite(s_e[lane]);
+  }
+  if (__any(!finite)) {  // non-finite input: no rank, no projector, flagged
     if (Pb)
       for (int idx = lane; idx < T * T; idx += 64) Pb[idx] = 0.f;
     if (Fb) zero_factor_slots(Fb, 0, TP, lane);  // read by paired reconstruction
@@ -779,107 +969,10 @@ __global__ __launch_bounds__(64) void k_eig_vals(double* __restrict__ gram, int 
       for (int c = lane; c < T; c += 64) lam_desc[b * T + c] = NAN;
     return;
   }
-
-  __syncthreads();
-  // split the tridiagonal where |e_i| <= 8u (|d_i| + |d_{i+1}|): a backward
-  // perturbation of the order of the reduction's own rounding, after which an
-  // exactly repeated eigenvalue lives in separate blocks
-  if (lane + 1 < T) {
-    const double thr = 8.8817841970012523e-16 * (fabs(s_d[lane]) + fabs(s_d[lane + 1]));
-    if (fabs(s_e[lane]) <= thr) s_e[lane] = 0.0;
-  }
-  __syncthreads();
-
-  // ---- 2. eigenvalues by bisection --------------------------------------------
-  double lo_g = INFINITY, hi_g = -INFINITY, emax2 = 0.0;
   if (lane < T) {
-    const double el = lane > 0 ? fabs(s_e[lane - 1]) : 0.0;
-    const double er = fabs(s_e[lane]);
-    lo_g = s_d[lane] - el - er;
-    hi_g = s_d[lane] + el + er;
-    emax2 = s_e[lane] * s_e[lane];
-    s_e2[lane] = emax2;
-  }
-  lo_g = wave_min(lo_g);
-  hi_g = wave_max(hi_g);
-  emax2 = wave_max(emax2);
-  const double u = 1.1102230246251565e-16;
-  const double bnorm = fmax(fabs(lo_g), fabs(hi_g));
-  const double pivmin = 2.2250738585072014e-308 * fmax(1.0, emax2);
-  const double slack = 2.0 * u * bnorm * T + pivmin;
-  lo_g -= slack;
-  hi_g += slack;
-  const double bsc = bnorm > 0.0 ? bnorm : 1.0;
-  const double inv = 1.0 / bsc;
-  if (lane < T) {
-    s_ds[lane] = s_d[lane] * inv;
-    s_e2s[lane] = s_e2[lane] * (inv * inv);
-  }
-  __syncthreads();
-  {
-    // the scaled tridiagonal in registers (the rows of A are dead by now)
-    double dr[TP], er[TP];
-#pragma unroll
-    for (int i = 0; i < TP; ++i) {
-      dr[i] = i < T ? s_ds[i] : 4.0;
-      er[i] = i + 1 < T ? s_e2s[i] : 0.0;
-    }
-    // 64-section of the Gershgorin interval: lane j counts at point j + 1 of 64
-    // equal steps (point 0 = lo, point 64 = hi), one count for 6 bits of every
-    // bracket; an exact zero minor takes the substituting count
-    {
-      const double L = lo_g * inv, H = hi_g * inv;
-      const double xj = lane == 63 ? H : fma((double)(lane + 1) * 0x1p-6, H - L, L);
-      bool zero;
-      int cnt = sturm_count_fast<TP>(dr, er, xj, &zero);
-      if (zero) cnt = sturm_count3(s_ds, s_e2s, T, xj);
-      if (lane == 63) cnt = T;
-      s_cnt[lane] = cnt;
-      s_sec[lane + 1] = xj;
-      if (lane == 0) s_sec[0] = L;
-    }
-    __syncthreads();
-    if (lane < T) {
-      // Bisection to 2^-33 of the scaled norm brackets each eigenvalue (the MP
-      // rule needs ~2^-24 relative to lambda_max); two Newton steps on the
-      // characteristic polynomial, clamped to the bracket (a root near its edge
-      // makes the first step overshoot), then reach full precision for every
-      // eigenvalue separated from its neighbours. Copies of a cluster share
-      // one bracket and so take identical steps (the cluster test below relies
-      // on that); a vanishing derivative keeps the current point.
-      const int kk = T - 1 - lane;  // kk-th smallest
-      const double tol = 0x1p-33;   // of the scaled problem, |T| <= 1
-      double lo = s_sec[0], hi = s_sec[64];
-      {
-        // bracket from the 64-section: the last section point whose count is at
-        // most kk, and the next one (counts grow with the point)
-        int a = 0;  // s_cnt[a - 1] <= kk < s_cnt[a + 63]; find the first j with cnt_j > kk
-#pragma unroll
-        for (int step = 32; step >= 1; step >>= 1)
-          if (s_cnt[a + step - 1] <= kk) a += step;
-        lo = s_sec[a];
-        hi = s_sec[a + 1];
-      }
-      for (int it = 0; it < 96 && hi - lo > tol; ++it) {
-        const double x = 0.5 * (lo + hi);
-        bool zero;
-        int cnt = sturm_count_fast<TP>(dr, er, x, &zero);
-        if (zero) cnt = sturm_count3(s_ds, s_e2s, T, x);  // exact zero minor: rare
-        if (cnt > kk)
-          hi = x;
-        else
-          lo = x;
-      }
-      double x = 0.5 * (lo + hi);
-#pragma unroll
-      for (int st = 0; st < 2; ++st) {
-        const double xn = x - newton_corr_reg<TP>(dr, er, T, x);
-        x = isfinite(xn) ? fmin(fmax(xn, lo), hi) : x;
-      }
-      const double lam = x * bsc;
-      s_lam[lane] = lam;
-      s_lamf[lane] = (float)lam;
-    }
+    const double lam = __longlong_as_double((long long)lam_lo | ((long long)lam_hi << 32));
+    s_lam[lane] = lam;
+    s_lamf[lane] = (float)lam;
   }
   __syncthreads();
   if (lam_desc)
@@ -897,61 +990,14 @@ __global__ __launch_bounds__(64) void k_eig_vals(double* __restrict__ gram, int 
       if (info) info[b] = 0;
     }
   }
-  // the eigenvalues for k_eig_vecs, exactly (two 32-bit halves each)
-  if ((Pb || Fb) && p > 0 && lane < T) {
-    unsigned* sc = lam_scratch(Pb, Fb, T, TP);
-    const long long bits = __double_as_longlong(s_lam[lane]);
-    sc[2 * lane] = (unsigned)bits;
-    sc[2 * lane + 1] = (unsigned)(bits >> 32);
+  // the same split and scale as k_eig_vals (the data are the same)
+  if (lane + 1 < T) {
+    const double thr = 8.8817841970012523e-16 * (fabs(s_d[lane]) + fabs(s_d[lane + 1]));
+    if (fabs(s_e[lane]) <= thr) s_e[lane] = 0.0;
   }
-}
-
-// Top-p eigenvectors of the tridiagonal left by k_tri_grid, at the eigenvalues
-// and rank left by k_eig_vals: twisted factorisation, Gram-Schmidt inside the
-// group, back-transform on the lane grid, factors or projector rows. Its own
-// kernel so that it runs at the occupancy of its few registers (the bisection
-// keeps the whole tridiagonal in registers): these phases are latency-bound.
-template <int TP>
-__global__ __launch_bounds__(64) void k_eig_vecs(const double* __restrict__ gram, int T,
-                                                 long long nb, float* __restrict__ proj,
-                                                 float* __restrict__ fac,
-                                                 const int* __restrict__ rank) {
-  constexpr int LD = TP + 1;
-  constexpr int R = TP / 8;  // register columns per lane of the 8 x 8 lane grid
-  __shared__ double s_d[TP], s_e[TP], s_e2[TP], s_tau[TP];
-  __shared__ double s_lam[TP];
-  __shared__ double s_dm[kRowGroup * LD];
-  __shared__ double s_z[kRowGroup * LD];  // D+ pivots, then the vector
-  __shared__ double s_h[kHChunk * TP];    // staged reflectors
-
-  const long long b = blockIdx.x;
-  if (b >= nb) return;
-  const int lane = threadIdx.x;
-  const bool row_ok = lane < T;
-  const int gr = lane >> 3, gc = lane & 7;  // lane-grid row and column
-  const double* Gb = gram + b * (long long)T * T;
-  float* Pb = proj ? proj + b * (long long)T * T : nullptr;
-  float* Fb = fac ? fac + b * (2LL * TP * TP) : nullptr;  // TP = 16*ceil(T/16)
-  const int p = __builtin_amdgcn_readfirstlane(rank[b]);
-  if (p > 0) {
-    const unsigned* sc = lam_scratch(Pb, Fb, T, TP);
-    if (lane < T) {
-      s_d[lane] = Gb[(T - 2) * T + lane];
-      s_e[lane] = lane + 1 < T ? Gb[(T - 1) * T + lane] : 0.0;
-      if (lane + 2 < T) s_tau[lane] = Gb[lane * T];
-      s_lam[lane] = __longlong_as_double((long long)sc[2 * lane] |
-                                         ((long long)sc[2 * lane + 1] << 32));
-    }
-    __syncthreads();
-    // the same split and scale as k_eig_vals (the data are the same)
-    if (lane + 1 < T) {
-      const double thr = 8.8817841970012523e-16 * (fabs(s_d[lane]) + fabs(s_d[lane + 1]));
-      if (fabs(s_e[lane]) <= thr) s_e[lane] = 0.0;
-    }
-    __syncthreads();
-  }
+  __syncthreads();
   double lo_g = INFINITY, hi_g = -INFINITY, emax2 = 0.0;
-  if (p > 0 && lane < T) {
+  if (lane < T) {
     const double el = lane > 0 ? fabs(s_e[lane - 1]) : 0.0;
     const double er = fabs(s_e[lane]);
     lo_g = s_d[lane] - el - er;
@@ -1126,12 +1172,14 @@ static int launch_rows(double* gram, long long nb, int T, int N, float* proj, fl
   hipLaunchKernelGGL((k_tri_grid<TP>), dim3((unsigned)nb), dim3(64), 0, st, gram, T, nb);
   int rc = launch_status("tridiagonalisation kernel");
   if (rc) return rc;
-  hipLaunchKernelGGL((k_eig_vals<TP>), dim3((unsigned)nb), dim3(64), 0, st, gram, T, N, nb,
-                     proj, fac, rank, weight, margin, lam_desc, info);
+  // k_eig_vals hands the eigenvalues over in the projector or factor block
+  if (!proj && !fac) return set_err(MPPCA_EINVAL, "eig64: no projector or factor output");
+  const long long nwave = (nb * T + 63) / 64;
+  hipLaunchKernelGGL((k_eig_vals<TP>), dim3((unsigned)nwave), dim3(64), 0, st, gram, T, nb, proj,
+                     fac);
   if ((rc = launch_status("eigenvalue kernel"))) return rc;
-  if (!proj && !fac) return MPPCA_OK;
-  hipLaunchKernelGGL((k_eig_vecs<TP>), dim3((unsigned)nb), dim3(64), 0, st, gram, T, nb, proj,
-                     fac, rank);
+  hipLaunchKernelGGL((k_eig_vecs<TP>), dim3((unsigned)nb), dim3(64), 0, st, gram, T, N, nb, proj,
+                     fac, rank, weight, margin, lam_desc, info);
   return launch_status("eigenvector kernel");
 }
 
