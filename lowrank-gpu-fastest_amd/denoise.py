@@ -71,9 +71,12 @@ class Denoiser:
         self.margin = torch.empty(self.n_patches, dtype=torch.float32, device=dev)
         self.sweeps = torch.empty(self.n_patches, dtype=torch.int32, device=dev)
         self._pin_in = None
-        # host -> host pipeline: x-slabs and host copy threads
+        # host -> host pipeline: x-slabs and host copy threads (a sweep of 3-24
+        # slabs and 4-16 threads on the default volume: 3-4 slabs and 8 threads
+        # best; more slabs are slower because each slab's sliding-window Gram
+        # launch lasts a whole column walk however few columns it has)
         self.nslabs = 4
-        self.nthreads = max(1, min(4, os.cpu_count() or 1))
+        self.nthreads = max(1, min(8, os.cpu_count() or 1))
 
     def _geom_args(self):
         x, y, z, t = self.shape
