@@ -159,7 +159,11 @@ int mppca_normalize(const float* acc, const float* mean, const float* weight,
  * c = 4*(g / S2) + g % S2, with S2 = ceil(q/2) when p <= 8 (slots 0..7 only,
  * so that two such patches share one tile in the reconstruction) and
  * S2 = ceil(q/4) otherwise; other slots and frames t >= T are 0. Tiles
- * nc < max(1, ceil(p/16)) are written (a rank-0 patch gets a zero tile).
+ * nc < max(1, ceil(p/16)) are written (a rank-0 patch gets a zero tile);
+ * for T <= 48, 2T 32-bit words starting 2*TF words before the end of the
+ * block (TF = T rounded up to 16) also carry the eigenvalue hand-off between
+ * the solver's kernels, and keep it when p does not reach that tile. Nothing
+ * reads them afterwards.
  * ------------------------------------------------------------------------- */
 
 /* Floats of one patch's factor block for T frames (< 0 if T > 64). */
