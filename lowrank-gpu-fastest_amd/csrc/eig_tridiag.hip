@@ -102,7 +102,7 @@ __device__ __forceinline__ double lane_read64(double x, int l) {
   return __hiloint2double(hi, lo);
 }
 constexpr int kDppQuadXor1 = 0xB1, kDppQuadXor2 = 0x4E, kDppRowHalfMirror = 0x141,
-              kDppRowMirror = 0x140;
+              kDppRowMirror = 0x140, kDppRowRor8 = 0x128;
 
 __device__ __forceinline__ double wave_sum(double x) {
   x += dpp64<kDppQuadXor1>(x);
@@ -135,12 +135,21 @@ __device__ __forceinline__ double bperm64(double x, int src) {
   const int hi = __builtin_amdgcn_ds_bpermute(src << 2, __double2hiint(x));
   return __hiloint2double(hi, lo);
 }
-// Sum over the 8 consecutive lanes of an aligned group (every lane gets it).
+// Sum over the 8 consecutive lanes of an aligned group (every lane gets it,
+// bit-identical: each butterfly level adds the same two operands on both lanes).
 __device__ __forceinline__ double group8_sum(double x) {
   x += dpp64<kDppQuadXor1>(x);
   x += dpp64<kDppQuadXor2>(x);
   x += dpp64<kDppRowHalfMirror>(x);
   return x;
+}
+// Sum over the eight aligned 8-lane groups of a value that is the same on
+// every lane of its group (a rotation by 8 inside each 16-lane row pairs the
+// groups; the four rows are read back): one DPP level instead of a full
+// 64-lane reduction.
+__device__ __forceinline__ double groups_sum(double x) {
+  x += dpp64<kDppRowRor8>(x);
+  return (lane_read64(x, 0) + lane_read64(x, 16)) + (lane_read64(x, 32) + lane_read64(x, 48));
 }
 
 // 1/q to full precision: hardware estimate plus two Newton steps.
@@ -149,6 +158,18 @@ __device__ __forceinline__ double rcp64(double q) {
   r = fma(fma(-q, r, 1.0), r, r);
   r = fma(fma(-q, r, 1.0), r, r);
   return r;
+}
+
+// 1/sqrt(x) for normal x > 0: hardware estimate plus two Newton steps
+// y += y (1 - x y^2) / 2, each doubling the correct bits.
+__device__ __forceinline__ double rsq64(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+#pragma unroll
+  for (int it = 0; it < 2; ++it) {
+    const double e = fma(-(x * y), y, 1.0);
+    y = fma(0.5 * y, e, y);
+  }
+  return y;
 }
 
 // Number of eigenvalues of the tridiagonal (d, e) smaller than x.
@@ -483,20 +504,23 @@ __device__ __forceinline__ int sturm_count3(const double* ds, const double* e2s,
 // sign. The loops are unrolled over TP: no register is indexed at run time.
 //
 // Sturm count of sturm_count3 on the register-resident tridiagonal, without
-// the zero-minor substitution (the common case): *zero reports whether any minor came out exactly zero, in which case the
-// caller recounts that lane with the substituting count (sturm_count3 on the
-// same scaled tridiagonal in LDS; the two agree otherwise).
+// the zero-minor substitution. A zero minor p_i = 0 inside the sequence needs
+// none when e_i != 0: then p_{i+1} = -e_i^2 p_{i-1}, so (p_{i-1}, p_i, p_{i+1})
+// hold exactly one sign change whichever sign p_i is given, as with the
+// substitution. It matters only when e_i = 0 (a split) or at the last row, and
+// either way the last minor comes out zero (a zero followed by a zero coupling
+// zeroes every later minor, the padded rows included). So *zero reports a zero
+// last minor only, and the caller then recounts that lane with the
+// substituting count (sturm_count3 on the same scaled tridiagonal in LDS).
 template <int TP>
 __device__ __forceinline__ int sturm_count_fast(const double (&dr)[TP], const double (&er)[TP],
                                                 double x, bool* zero) {
   double pm = 1.0;
   double pc = dr[0] - x;
-  bool z = pc == 0.0;
   int c = pc < 0.0 ? 1 : 0;
 #pragma unroll
   for (int i = 1; i < TP; ++i) {
     const double pn = fma(dr[i] - x, pc, -er[i - 1] * pm);
-    z = z || pn == 0.0;
     c += (pn < 0.0) != (pc < 0.0) ? 1 : 0;
     pm = pc;
     pc = pn;
@@ -506,7 +530,7 @@ __device__ __forceinline__ int sturm_count_fast(const double (&dr)[TP], const do
       pm = __builtin_amdgcn_ldexp(pm, -ex);
     }
   }
-  *zero = z;
+  *zero = pc == 0.0;
   return c;
 }
 
@@ -640,19 +664,40 @@ __global__ __launch_bounds__(64) void k_tri_grid(double* __restrict__ gram, int 
       const int nb1 = mb + 1 < R ? mb + 1 : mb;  // column k+1 = 8(mb+1) when r0 = 7
       const double asrc = r0 == 7 ? a[mb][nb1] : a[mb][mb];
       const double alpha = lane_read64(asrc, 8 * r0 + ((r0 + 1) & 7));  // A[k][k+1]
+      // Index tests against k = 8 mb + r0 (r0 < 8) are static from block
+      // mb + 2 on (every column there is > k + 1) and, for rows > k, from
+      // block mb + 1 on: only the first one or two register blocks select.
       double part = 0.0;
 #pragma unroll
       for (int n = mb; n < R; ++n) {
         const double xv = a[mb][n];
-        part = gc + 8 * n > k + 1 ? fma(xv, xv, part) : part;
+        if (n >= mb + 2)
+          part = fma(xv, xv, part);
+        else
+          part = gc + 8 * n > k + 1 ? fma(xv, xv, part) : part;
       }
       const double ss = lane_read64(group8_sum(part), 8 * r0);
+      // dlarfg: beta = -sign(alpha) mu, mu = |(alpha, x)|; tau = (beta - alpha) / beta
+      // = 1 + |alpha| / mu; v scaled by 1 / (alpha - beta) = sign(alpha) / (|alpha| + mu).
+      // The operands are wave-uniform but live in VALU registers: the
+      // reciprocal square root and reciprocal estimates with Newton steps
+      // cost a third of IEEE sqrt and two divisions (tiny sums take those).
       double tk = 0.0, beta = alpha, sc = 0.0;
       if (ss > 0.0) {
-        const double mu = sqrt(alpha * alpha + ss);
+        const double s2 = fma(alpha, alpha, ss);
+        double mu, rmu;
+        if (s2 >= 0x1p-900) {
+          rmu = rsq64(s2);
+          mu = s2 * rmu;
+        } else {
+          mu = sqrt(s2);
+          rmu = 1.0 / mu;
+        }
+        const double aa = fabs(alpha);
         beta = alpha >= 0.0 ? -mu : mu;
-        tk = (beta - alpha) / beta;
-        sc = 1.0 / (alpha - beta);
+        tk = fma(aa, rmu, 1.0);
+        const double sa = rcp64(aa + mu);
+        sc = alpha >= 0.0 ? sa : -sa;
       }
       // v at this lane's columns gc + 8n and rows gr + 8m (row k of A lives in
       // grid row r0: column j in grid column j % 8, register column j / 8)
@@ -666,9 +711,14 @@ __global__ __launch_bounds__(64) void k_tri_grid(double* __restrict__ gram, int 
       for (int n = mb; n < R; ++n) {
         const double xc = bperm64(a[mb][n], 8 * r0 + gc);
         const double xr = bperm64(a[mb][n], 8 * r0 + gr);
-        const int j = gc + 8 * n, i = gr + 8 * n;
-        vc[n] = j == k + 1 ? 1.0 : (j > k + 1 ? xc * sc : 0.0);
-        vr[n] = i == k + 1 ? 1.0 : (i > k + 1 ? xr * sc : 0.0);
+        if (n >= mb + 2) {
+          vc[n] = xc * sc;
+          vr[n] = xr * sc;
+        } else {
+          const int j = gc + 8 * n, i = gr + 8 * n;
+          vc[n] = j == k + 1 ? 1.0 : (j > k + 1 ? xc * sc : 0.0);
+          vr[n] = i == k + 1 ? 1.0 : (i > k + 1 ? xr * sc : 0.0);
+        }
       }
       if (gr == r0) {  // reflector k (zero for indices <= k; column 0 carries tau_k)
 #pragma unroll
@@ -689,10 +739,11 @@ __global__ __launch_bounds__(64) void k_tri_grid(double* __restrict__ gram, int 
 #pragma unroll
           for (int n = mb; n < R; ++n) acc = fma(a[m][n], vc[n], acc);
           acc = group8_sum(acc);
-          pl[m] = gr + 8 * m > k ? tk * acc : 0.0;
+          pl[m] = (m >= mb + 1 || gr + 8 * m > k) ? tk * acc : 0.0;
           part2 = fma(pl[m], vr[m], part2);
         }
-        const double K = 0.5 * tk * wave_sum(gc == 0 ? part2 : 0.0);  // each row once
+        // part2 is the same on the 8 lanes of a grid row: each row once
+        const double K = 0.5 * tk * groups_sum(part2);
         double wr[R], wc[R];
 #pragma unroll
         for (int m = mb; m < R; ++m) wr[m] = pl[m] - K * vr[m];
