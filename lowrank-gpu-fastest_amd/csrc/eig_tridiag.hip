@@ -143,6 +143,22 @@ __device__ __forceinline__ double group8_sum(double x) {
   x += dpp64<kDppRowHalfMirror>(x);
   return x;
 }
+// Smallest (value, index) pair over the 8 consecutive lanes of an aligned
+// group, lowest index on equal values (every lane gets it; NaN never wins).
+template <int CTRL>
+__device__ __forceinline__ void argmin_step(double& v, int& i) {
+  const double ov = dpp64<CTRL>(v);
+  const int oi = __builtin_amdgcn_mov_dpp(i, CTRL, 0xF, 0xF, false);
+  if (ov < v || (ov == v && oi < i)) {
+    v = ov;
+    i = oi;
+  }
+}
+__device__ __forceinline__ void argmin8(double& v, int& i) {
+  argmin_step<kDppQuadXor1>(v, i);
+  argmin_step<kDppQuadXor2>(v, i);
+  argmin_step<kDppRowHalfMirror>(v, i);
+}
 // Sum over the eight aligned 8-lane groups of a value that is the same on
 // every lane of its group (a rotation by 8 inside each 16-lane row pairs the
 // groups; the four rows are read back): one DPP level instead of a full
@@ -964,7 +980,7 @@ __global__ __launch_bounds__(64) void k_eig_vals(const double* __restrict__ gram
 // kernel so that it runs at the occupancy of its few registers (the bisection
 // keeps the whole tridiagonal in registers): these phases are latency-bound.
 template <int TP>
-__global__ __launch_bounds__(64) void k_eig_vecs(const double* __restrict__ gram, int T, int N,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_eig_vecs(const double* __restrict__ gram, int T, int N,
                                                  long long nb, float* __restrict__ proj,
                                                  float* __restrict__ fac,
                                                  int* __restrict__ rank,
@@ -1071,61 +1087,107 @@ __global__ __launch_bounds__(64) void k_eig_vecs(const double* __restrict__ gram
     for (int c = 0; c < T; ++c) dst[c] = 0.f;
   for (int g0 = 0; g0 < p; g0 += kRowGroup) {
     const int gn = p - g0 < kRowGroup ? p - g0 : kRowGroup;
-    if (lane < gn) {
-      // Twisted factorisation of T - lambda I (z holds D+ first). Near a
-      // cluster the twisted vectors of different twist indices r are columns
-      // of the resolvent (T - lambda I)^-1, which span the cluster: the m-th
-      // copy of a (numerically) repeated eigenvalue takes the twist with the
-      // m-th smallest |gamma_r|, and Gram-Schmidt below separates the copies.
-      const int j = g0 + lane;
-      const double lam = s_lam[j];
-      double* z = s_z + lane * LD;
-      double* dm = s_dm + lane * LD;
-      double* gam = s_h + lane * TP;  // free until the back-transform
+    // Twisted factorisation of T - lambda I for vector g = gr of the group on
+    // the eight lanes c = gc of its grid row. Near a cluster the twisted
+    // vectors of different twist indices r are columns of the resolvent
+    // (T - lambda I)^-1, which span the cluster: the m-th copy of a
+    // (numerically) repeated eigenvalue takes the twist with the m-th smallest
+    // |gamma_r|, and Gram-Schmidt below separates the copies.
+    const bool vec_ok = gr < gn;
+    const int j = g0 + (vec_ok ? gr : 0);
+    const double lam = s_lam[j];
+    double* zf = s_z + gr * LD;   // forward pivots D+, then the vector
+    double* db = s_dm + gr * LD;  // backward pivots
+    double* rw = s_h + gr * TP;   // raw backward pivots, then the ratios (s_h is
+                                  // free until the back-transform)
+    // (a) the two stationary factorisations at once: lane c = 0 runs the
+    // forward one (clamped pivot i into zf[i], the last one unclamped), lane
+    // c = 1 the backward one (clamped pivot i + 1 into db[i + 1], the raw
+    // pivot i into rw[i]). One code path: per-lane pointers and a +-1 stride.
+    if (vec_ok && gc < 2) {
+      const bool fw = gc == 0;
+      double dq = fw ? s_d[0] - lam : s_d[T - 1] - lam;
+      if (!fw) rw[T - 1] = dq;
+      const int st = fw ? 1 : -1;
+      int ie = fw ? 0 : T - 2;                // coupling of this step
+      int idn = fw ? 1 : T - 2;               // diagonal of the next pivot
+      double* cd = fw ? zf : db + T - 1;      // clamped pivot of this step
+      double* rd = fw ? zf + 1 : rw + T - 2;  // raw next pivot
+      for (int t = 0; t + 1 < T; ++t) {
+        if (fabs(dq) < pvfloor) dq = dq < 0.0 ? -pvfloor : pvfloor;
+        *cd = dq;
+        dq = (s_d[idn] - lam) - s_e2[ie] * rcp64(dq);
+        *rd = dq;  // forward: overwritten by the next clamped pivot
+        cd += st;
+        rd += st;
+        ie += st;
+        idn += st;
+      }
+    }
+    __syncthreads();
+    // (b) |gamma_i| = |D+_i + D-_i - (d_i - lambda)| for i = c + 8n, and the
+    // m-th smallest over the group's 8 lanes (ties: lowest index, as a scan)
+    double gv[R];
+#pragma unroll
+    for (int n = 0; n < R; ++n) {
+      const int i = gc + 8 * n;
+      gv[n] = (vec_ok && i < T) ? fabs(zf[i] + rw[i] - (s_d[i] - lam)) : INFINITY;
+    }
+    int m = 0;
+    if (vec_ok) {
       const double ctol = 16.0 * u * bnorm + pivmin;
-      int m = 0;
       for (int i = 0; i < j; ++i) m += fabs(s_lam[i] - lam) <= ctol ? 1 : 0;
-      double dq = s_d[0] - lam;
-      for (int i = 0; i + 1 < T; ++i) {
-        if (fabs(dq) < pvfloor) dq = dq < 0.0 ? -pvfloor : pvfloor;
-        z[i] = dq;
-        dq = (s_d[i + 1] - lam) - s_e2[i] * rcp64(dq);
-      }
-      z[T - 1] = dq;
-      dq = s_d[T - 1] - lam;
-      gam[T - 1] = fabs(z[T - 1] + dq - (s_d[T - 1] - lam));
-      for (int i = T - 2; i >= 0; --i) {
-        if (fabs(dq) < pvfloor) dq = dq < 0.0 ? -pvfloor : pvfloor;
-        dm[i + 1] = dq;
-        dq = (s_d[i] - lam) - s_e2[i] * rcp64(dq);
-        gam[i] = fabs(z[i] + dq - (s_d[i] - lam));
-      }
-      int r = 0;
-      for (int sel = 0; sel <= m; ++sel) {  // m-th smallest |gamma| (ties: lowest index)
-        double best = INFINITY;
-        for (int i = 0; i < T; ++i)
-          if (gam[i] < best) {
-            best = gam[i];
-            r = i;
-          }
-        gam[r] = INFINITY;
-      }
-      double nrm = 1.0;
+    }
+    int r = 0;
+    for (int sel = 0; __any(vec_ok && sel <= m); ++sel) {
+      double bv = INFINITY;
+      int bi = 0x7fffffff;
+#pragma unroll
+      for (int n = 0; n < R; ++n)
+        if (gv[n] < bv) {
+          bv = gv[n];
+          bi = gc + 8 * n;
+        }
+      argmin8(bv, bi);
+      if (sel <= m && bi < T) r = bi;
+#pragma unroll
+      for (int n = 0; n < R; ++n)
+        if (gc + 8 * n == bi) gv[n] = INFINITY;
+    }
+    __syncthreads();  // rw is read above and rewritten below
+    // (c) the ratios z_i / z_{i+1} (i < r) and z_{i+1} / z_i (i >= r), in parallel
+#pragma unroll
+    for (int n = 0; n < R; ++n) {
+      const int i = gc + 8 * n;
+      if (vec_ok && i + 1 < T)
+        rw[i] = i < r ? -(s_e[i] * rcp64(zf[i])) : -(s_e[i] * rcp64(db[i + 1]));
+    }
+    __syncthreads();
+    // (d) z_r = 1, then one product per step: lane c = 0 walks down from r,
+    // lane c = 1 up; each sums its squares
+    double part = 0.0;
+    if (vec_ok && gc < 2) {
+      const bool dn = gc == 0;
+      const int cnt = dn ? r : T - 1 - r;
+      const int st = dn ? -1 : 1;
+      int ri = dn ? r - 1 : r;
+      double* zd = zf + (dn ? r - 1 : r + 1);
       double zn = 1.0;
-      for (int i = r - 1; i >= 0; --i) {
-        zn = -(s_e[i] * rcp64(z[i])) * zn;
-        z[i] = zn;
-        nrm = fma(zn, zn, nrm);
+      for (int t = 0; t < cnt; ++t) {
+        zn = rw[ri] * zn;
+        *zd = zn;
+        part = fma(zn, zn, part);
+        ri += st;
+        zd += st;
       }
-      z[r] = 1.0;
-      zn = 1.0;
-      for (int i = r; i + 1 < T; ++i) {
-        zn = -(s_e[i] * rcp64(dm[i + 1])) * zn;
-        z[i + 1] = zn;
-        nrm = fma(zn, zn, nrm);
-      }
-      const double sn = 1.0 / sqrt(nrm);
-      for (int i = 0; i < T; ++i) z[i] *= sn;
+      if (dn) zf[r] = 1.0;
+    }
+    const double sn = 1.0 / sqrt(1.0 + group8_sum(part));
+    __syncthreads();
+#pragma unroll
+    for (int n = 0; n < R; ++n) {
+      const int i = gc + 8 * n;
+      if (vec_ok && i < T) zf[i] *= sn;
     }
     __syncthreads();
     // modified Gram-Schmidt inside the group (guards near-equal eigenvalues)
