@@ -528,6 +528,9 @@ __device__ __forceinline__ int sturm_count3(const double* ds, const double* e2s,
 // zeroes every later minor, the padded rows included). So *zero reports a zero
 // last minor only, and the caller then recounts that lane with the
 // substituting count (sturm_count3 on the same scaled tridiagonal in LDS).
+// The pair is rescaled by a power of two (exact) every 16 rows: with |d_i|,
+// |e_i| <= 1 and |x| <= 1 + slack a minor grows by at most 1 + sqrt(2) per
+// row, so 16 rows stay below 2^21 of the last rescale.
 template <int TP>
 __device__ __forceinline__ int sturm_count_fast(const double (&dr)[TP], const double (&er)[TP],
                                                 double x, bool* zero) {
@@ -540,7 +543,7 @@ __device__ __forceinline__ int sturm_count_fast(const double (&dr)[TP], const do
     c += (pn < 0.0) != (pc < 0.0) ? 1 : 0;
     pm = pc;
     pc = pn;
-    if ((i & 7) == 0) {
+    if ((i & 15) == 0) {
       const int ex = __builtin_amdgcn_frexp_exp(fmax(fabs(pc), fabs(pm)));
       pc = __builtin_amdgcn_ldexp(pc, -ex);
       pm = __builtin_amdgcn_ldexp(pm, -ex);
@@ -553,7 +556,8 @@ __device__ __forceinline__ int sturm_count_fast(const double (&dr)[TP], const do
 // Newton correction f(x) / f'(x) of the characteristic polynomial
 // f = det(T_T - x I) of the leading T rows, from the minor recurrence over all
 // TP rows and its derivative dp_i = (d_i - x) dp_{i-1} - e_{i-1}^2 dp_{i-2} -
-// p_{i-1}, the four values rescaled together (the ratio is unchanged). The
+// p_{i-1}, the four values rescaled together every 16 rows (the ratio is
+// unchanged; the derivative adds at most a factor of 16 to the growth). The
 // k = TP - T padded rows make P = f (4 - x)^k, so f / f' = P / (P' + k P / (4 - x)).
 // NaN when the derivative vanishes.
 template <int TP>
@@ -569,7 +573,7 @@ __device__ __forceinline__ double newton_corr_reg(const double (&dr)[TP], const 
     pc = pn;
     dpm = dpc;
     dpc = dpn;
-    if ((i & 7) == 0) {
+    if ((i & 15) == 0) {
       const int ex = __builtin_amdgcn_frexp_exp(
           fmax(fmax(fabs(pc), fabs(pm)), fmax(fabs(dpc), fabs(dpm))));
       pc = __builtin_amdgcn_ldexp(pc, -ex);
@@ -1114,7 +1118,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
       double* cd = fw ? zf : db + T - 1;      // clamped pivot of this step
       double* rd = fw ? zf + 1 : rw + T - 2;  // raw next pivot
       for (int t = 0; t + 1 < T; ++t) {
-        if (fabs(dq) < pvfloor) dq = dq < 0.0 ? -pvfloor : pvfloor;
+        // |pivot| raised to pvfloor, sign kept (a pivot (d - lambda) - e^2 r
+        // that cancels exactly is +0, so this is the select of dlar1v)
+        dq = copysign(fmax(fabs(dq), pvfloor), dq);
         *cd = dq;
         dq = (s_d[idn] - lam) - s_e2[ie] * rcp64(dq);
         *rd = dq;  // forward: overwritten by the next clamped pivot
