@@ -216,8 +216,11 @@ int mppca_denoise_device(const float* vol, float* out, int X, int Y, int Z,
 /* Host -> host denoise with the PCIe transfers overlapped with the kernels:
  * the whole of step8_tuned.denoise (upload, denoise, download;
  * step8_tuned.py:116-146) as one blocking call. Returns once `out` holds the
- * result, equal to mppca_denoise_device's (bit for bit on the factor route,
- * which has no atomics; up to summation order on the projector route).
+ * result: mppca_denoise_device's up to rounding. The x-slab Gram launches
+ * split each column's z walk into runs whose prefixes restart at a window
+ * (FP64 rounding of the prefix differences; the same ranks), and the projector
+ * route adds float atomics. Repeated calls with the same arguments are bit for
+ * bit on the factor route.
  *   vol, out          host buffers (X, Y, Z, T) float32.
  *   workspace         device, >= mppca_workspace_bytes(...), 256-byte aligned.
  *   d_vol, d_out      device volume buffers (distinct).

@@ -15,6 +15,8 @@
 // S = double: v_mfma_f64_16x16x4f64 on exact float->double operands: every
 //             product is exact and sums carry 53 bits, so G matches the exact
 //             Gram of the float32 centred patch to ~1e-16 relative.
+#include <atomic>
+
 #include "common.h"
 
 namespace mppca {
@@ -161,6 +163,12 @@ int launch_gram(const float* xc, float* gram, const Geom& g, long long b0, long 
 // product is formed once per column instead of once per patch (3.1x fewer
 // MFMAs at 11^3 / stride 3); the difference of two FP64 prefixes costs about
 // u * Z / pz relative to the Gram entries. Planes in no window are skipped.
+// A launch with fewer columns than the GPU holds workgroups (an x-slab of the
+// host pipeline) splits each column's windows into nseg contiguous runs, one
+// workgroup each: a run walks only the planes of its own windows, from a
+// prefix of zero at its first window, so the GPU fills and the launch takes a
+// fraction of a whole walk (each run repeats up to pz - stride planes of its
+// neighbour). The differences come out the same up to rounding.
 // ---------------------------------------------------------------------------
 // Plane tile of the sliding walk in LDS: rows = footprint voxels padded with
 // zero rows to a multiple of 32 (8 chunks of 4), columns = frames padded with
@@ -182,7 +190,7 @@ __host__ __device__ constexpr size_t slide_lds(int ntb, int F) {
 template <int NTB>
 __global__ __launch_bounds__(64 * NTB * (NTB + 1) / 2 * slide_cols(NTB)) __attribute__((amdgpu_waves_per_eu(6))) void k_gram_slide(
     const float* __restrict__ xc, double* __restrict__ gram, Geom g, int ix0, int ncol,
-    int per_xcd) {
+    int nseg, int per_xcd) {
   constexpr int NBLK = NTB * (NTB + 1) / 2, NCOL = 64 * NBLK;  // threads per column
   constexpr int TP = slide_stride(NTB), TS = 16 * NTB, RPT = slide_rows_per_pass(NTB);
   static_assert(RPT * TS == NCOL, "staging grid must cover a column's threads");
@@ -192,10 +200,16 @@ __global__ __launch_bounds__(64 * NTB * (NTB + 1) / 2 * slide_cols(NTB)) __attri
   // XCD k walks columns [k * per_xcd, (k + 1) * per_xcd) * CPB and neighbouring
   // columns, whose footprints overlap in (px - stride) / px of their voxels,
   // share its L2
-  const int blk = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
+  // (the runs of one column are neighbours too)
+  const int unit = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
+  const int blk = unit / nseg, seg = unit - blk * nseg;
   const int h = threadIdx.x / NCOL, tid = threadIdx.x - h * NCOL;  // column slot, thread in it
   const int col_raw = blk * slide_cols(NTB) + h;
   if (blk * slide_cols(NTB) >= ncol) return;  // whole workgroup past the end
+  // this run's windows [w0, w1) and planes [z0, z1)
+  const int w0 = (int)((long long)seg * g.ncz / nseg), w1 = (int)((long long)(seg + 1) * g.ncz / nseg);
+  const int z0 = corner_of(w0, g.nsz, g.Z, g.pz, g.stride);
+  const int z1 = corner_of(w1 - 1, g.nsz, g.Z, g.pz, g.stride) + g.pz;
   // a slot past the last column repeats it without storing, to keep the barriers
   const bool writer = col_raw < ncol;
   const int col = writer ? col_raw : ncol - 1;
@@ -283,16 +297,16 @@ __global__ __launch_bounds__(64 * NTB * (NTB + 1) / 2 * slide_cols(NTB)) __attri
   // two accumulators (even and odd chunks) break the MFMA dependency chain;
   // the prefix is their sum
   doublex4 P0 = doublex4{0.0, 0.0, 0.0, 0.0}, P1 = P0;
-  int is = 0, ie = 0;  // next window to start, next window to end
-  fetch(0);
-  for (int z = 0; z < g.Z; ++z) {
-    if (is < g.ncz && corner_of(is, g.nsz, g.Z, g.pz, g.stride) == z) start(is++, P0 + P1);
+  int is = w0, ie = w0;  // next window to start, next window to end
+  fetch(z0);
+  for (int z = z0; z < z1; ++z) {
+    if (is < w1 && corner_of(is, g.nsz, g.Z, g.pz, g.stride) == z) start(is++, P0 + P1);
     const bool act = is > ie;  // plane z lies in a window
     const bool ends = ie < is && corner_of(ie, g.nsz, g.Z, g.pz, g.stride) + g.pz == z + 1;
     __syncthreads();  // every wave is done with plane z - 1
     put();
     __syncthreads();
-    if (z + 1 < g.Z) fetch(z + 1);  // in flight under this plane's MFMAs
+    if (z + 1 < z1) fetch(z + 1);  // in flight under this plane's MFMAs
     if (act) {
       for (int c = 0; c < R / 4; c += 4) {
         float a[4], b[4];
@@ -313,14 +327,44 @@ __global__ __launch_bounds__(64 * NTB * (NTB + 1) / 2 * slide_cols(NTB)) __attri
   }
 }
 
+constexpr int kMaxDevices = 64;
+
+// Workgroups of k_gram_slide<NTB> the current device holds at once (0 if the
+// query fails), cached per device.
+template <int NTB>
+static int slide_slots(size_t lds) {
+  constexpr int NTHR = 64 * NTB * (NTB + 1) / 2 * slide_cols(NTB);
+  static std::atomic<int> cache[kMaxDevices];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return 0;
+  int v = cache[dev].load(std::memory_order_relaxed);
+  if (v > 0) return v;
+  int per_cu = 0, ncu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_gram_slide<NTB>, NTHR, lds) !=
+          hipSuccess ||
+      hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    return 0;
+  v = per_cu * ncu;
+  cache[dev].store(v, std::memory_order_relaxed);
+  return v;
+}
+
 template <int NTB>
 static int launch_gram_slide_t(const float* xc, double* gram, const Geom& g, int ix0, int nix,
                                hipStream_t st) {
   constexpr int NTHR = 64 * NTB * (NTB + 1) / 2 * slide_cols(NTB);
+  const size_t lds = slide_lds(NTB, g.px * g.py);
   const int ncol = nix * g.ncy;
-  const int nblk = (ncol + slide_cols(NTB) - 1) / slide_cols(NTB), per_xcd = (nblk + 7) / 8;
-  hipLaunchKernelGGL((k_gram_slide<NTB>), dim3(8u * (unsigned)per_xcd), dim3(NTHR),
-                     slide_lds(NTB, g.px * g.py), st, xc, gram, g, ix0, ncol, per_xcd);
+  const int nblk = (ncol + slide_cols(NTB) - 1) / slide_cols(NTB);
+  // runs per column: as many as keep every workgroup resident at once
+  const int slots = slide_slots<NTB>(lds);
+  int nseg = slots > 0 ? slots / nblk : 1;
+  nseg = nseg < 1 ? 1 : nseg > g.ncz ? g.ncz : nseg;
+  const long long nunit = (long long)nblk * nseg;
+  if (nunit > (1LL << 30)) return set_err(MPPCA_EUNSUPPORTED, "gram (sliding): too many columns");
+  const int per_xcd = (int)((nunit + 7) / 8);
+  hipLaunchKernelGGL((k_gram_slide<NTB>), dim3(8u * (unsigned)per_xcd), dim3(NTHR), lds, st, xc,
+                     gram, g, ix0, ncol, nseg, per_xcd);
   return launch_status("gram (sliding) kernel");
 }
 

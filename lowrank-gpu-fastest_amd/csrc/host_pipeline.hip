@@ -10,9 +10,11 @@
 // contiguous index range) get their Gram matrices and eigensolves. The output
 // planes that no unfinished patch touches (x below the first unfinished
 // corner) are reconstructed and downloaded on the output stream while later
-// slabs upload and compute. Every kernel runs exactly as in the device-resident
-// path, so the result is bit-identical to mppca_denoise_device's (the factor
-// route has no atomics). Other routes run the whole volume at once.
+// slabs upload and compute. Every stage runs the device-resident path's
+// kernels; a slab's sliding-window Gram launch has few columns and so splits
+// their z walks into runs (the prefix restarts at a run's first window), which
+// leaves the result equal to mppca_denoise_device's up to FP64 rounding of the
+// Gram, with the same ranks. Other routes run the whole volume at once.
 #include <string.h>
 
 #include <atomic>
@@ -142,6 +144,33 @@ extern "C" int mppca_denoise_host(const float* vol, float* out, int X, int Y, in
   const long long per_ix = (long long)g.ncy * g.ncz;
   const long long ff = pipelined ? factor_floats(T) : 0;
   const int tw = recon_factor_tile_x();
+  // Reconstruction and download of the planes slab j finished, [x_done, x_hi).
+  // The download is a copy kernel on the CUs that slows a memory-bound kernel
+  // beside it many times over (the centring), so slab j's reconstruction is
+  // enqueued after slab j + 1's centring: the download then runs beside the
+  // Gram and eigensolver of slab j + 1.
+  auto finish_slab = [&](int j, int x_hi) -> int {
+    int r = MPPCA_OK;
+    if (x_hi > x_done) {
+      if ((r = launch_recon_factors(w.xc, w.fac, rk, w.weight, w.mean, d_out, g, x_done, x_hi,
+                                    sc)))
+        return r;
+      out_lo[j] = x_done;
+      out_hi[j] = x_hi;
+      x_done = x_hi;
+    }
+    if ((r = hip_ok(hipEventRecord(ev_rec.ev[j], sc), "event")) ||
+        (r = hip_ok(hipStreamWaitEvent(so, ev_rec.ev[j], 0), "wait")))
+      return r;
+    if (out_hi[j] > out_lo[j]) {
+      const size_t o = (size_t)out_lo[j] * plane, m = (size_t)(out_hi[j] - out_lo[j]) * plane;
+      if ((r = hip_ok(hipMemcpyAsync(dst_out + o, d_out + o, m * sizeof(float),
+                                     hipMemcpyDeviceToHost, so), "download")))
+        return r;
+    }
+    return hip_ok(hipEventRecord(ev_out.ev[j], so), "event");
+  };
+  int pend_hi = 0;  // planes the last slab finished (reconstructed after the next centring)
   for (int i = 0; i < K; ++i) {
     const size_t off = (size_t)xa[i] * plane, n = (size_t)(xa[i + 1] - xa[i]) * plane;
     if (in_copy) in_copy->wait(i);
@@ -155,43 +184,30 @@ extern "C" int mppca_denoise_host(const float* vol, float* out, int X, int Y, in
       if ((rc = run_device_path(d_vol, d_out, g, form, w, rank, margin, info, sc))) break;
       out_lo[0] = 0;
       out_hi[0] = X;
-    } else {
-      const long long vox0 = (long long)xa[i] * Y * Z, nv = (long long)(xa[i + 1] - xa[i]) * Y * Z;
-      if ((rc = launch_centre(d_vol + off, w.xc + off, w.mean + vox0, nv, T, sc))) break;
-      int ix_new = ix_done;
-      while (ix_new < g.ncx && corner_of(ix_new, g.nsx, X, px, stride) + px <= xa[i + 1]) ++ix_new;
-      if (ix_new > ix_done) {
-        const long long b0 = ix_done * per_ix, nb = (ix_new - ix_done) * per_ix;
-        if ((rc = launch_gram64(w.xc, (double*)w.gram + b0 * T * T, g, b0, nb, sc))) break;
-        if ((rc = launch_eig64_mp((double*)w.gram + b0 * T * T, nb, T, g.N, nullptr,
-                                  w.fac + b0 * ff, rk + b0, w.weight + b0, mg + b0, nullptr,
-                                  inf + b0, sc)))
-          break;
-        ix_done = ix_new;
-      }
-      // planes below the first unfinished corner (and uploaded) are final
-      int x_hi = ix_done == g.ncx ? X : corner_of(ix_done, g.nsx, X, px, stride);
-      if (x_hi > xa[i + 1]) x_hi = xa[i + 1];
-      if (x_hi != X) x_hi -= x_hi % tw;
-      if (x_hi > x_done) {
-        if ((rc = launch_recon_factors(w.xc, w.fac, rk, w.weight, w.mean, d_out, g, x_done, x_hi,
-                                       sc)))
-          break;
-        out_lo[i] = x_done;
-        out_hi[i] = x_hi;
-        x_done = x_hi;
-      }
+      x_done = X;
+      if ((rc = finish_slab(0, X))) break;
+      continue;
     }
-    if ((rc = hip_ok(hipEventRecord(ev_rec.ev[i], sc), "event")) ||
-        (rc = hip_ok(hipStreamWaitEvent(so, ev_rec.ev[i], 0), "wait")))
-      break;
-    if (out_hi[i] > out_lo[i]) {
-      const size_t o = (size_t)out_lo[i] * plane, m = (size_t)(out_hi[i] - out_lo[i]) * plane;
-      if ((rc = hip_ok(hipMemcpyAsync(dst_out + o, d_out + o, m * sizeof(float),
-                                      hipMemcpyDeviceToHost, so), "download")))
+    const long long vox0 = (long long)xa[i] * Y * Z, nv = (long long)(xa[i + 1] - xa[i]) * Y * Z;
+    if ((rc = launch_centre(d_vol + off, w.xc + off, w.mean + vox0, nv, T, sc))) break;
+    if (i > 0 && (rc = finish_slab(i - 1, pend_hi))) break;
+    int ix_new = ix_done;
+    while (ix_new < g.ncx && corner_of(ix_new, g.nsx, X, px, stride) + px <= xa[i + 1]) ++ix_new;
+    if (ix_new > ix_done) {
+      const long long b0 = ix_done * per_ix, nb = (ix_new - ix_done) * per_ix;
+      if ((rc = launch_gram64(w.xc, (double*)w.gram + b0 * T * T, g, b0, nb, sc))) break;
+      if ((rc = launch_eig64_mp((double*)w.gram + b0 * T * T, nb, T, g.N, nullptr,
+                                w.fac + b0 * ff, rk + b0, w.weight + b0, mg + b0, nullptr,
+                                inf + b0, sc)))
         break;
+      ix_done = ix_new;
     }
-    if ((rc = hip_ok(hipEventRecord(ev_out.ev[i], so), "event"))) break;
+    // planes below the first unfinished corner (and uploaded) are final
+    int x_hi = ix_done == g.ncx ? X : corner_of(ix_done, g.nsx, X, px, stride);
+    if (x_hi > xa[i + 1]) x_hi = xa[i + 1];
+    if (x_hi != X) x_hi -= x_hi % tw;
+    pend_hi = x_hi > x_done ? x_hi : x_done;
+    if (i + 1 == K && (rc = finish_slab(i, pend_hi))) break;
   }
   if (!rc && pipelined && x_done != X)
     rc = set_err(MPPCA_EINVAL, "denoise_host: planes [%d, %d) never finished", x_done, X);

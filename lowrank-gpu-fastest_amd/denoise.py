@@ -71,11 +71,11 @@ class Denoiser:
         self.margin = torch.empty(self.n_patches, dtype=torch.float32, device=dev)
         self.sweeps = torch.empty(self.n_patches, dtype=torch.int32, device=dev)
         self._pin_in = None
-        # host -> host pipeline: x-slabs and host copy threads (a sweep of 3-24
-        # slabs and 4-16 threads on the default volume: 3-4 slabs and 8 threads
-        # best; more slabs are slower because each slab's sliding-window Gram
-        # launch lasts a whole column walk however few columns it has)
-        self.nslabs = 4
+        # host -> host pipeline: x-slabs and host copy threads (a sweep of 4-12
+        # slabs and 8-16 threads on the default volume: 6 slabs and 8 threads
+        # best; more slabs are slower because every launch of a slab costs at
+        # least one pass of its longest workgroup, however few workgroups it has)
+        self.nslabs = 6
         self.nthreads = max(1, min(8, os.cpu_count() or 1))
 
     def _geom_args(self):

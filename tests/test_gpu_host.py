@@ -1,5 +1,11 @@
-"""Host -> host pipeline (mppca_denoise_host): bit-identical to the
-device-resident path for every slab count and route, through the C ABI."""
+"""Host -> host pipeline (mppca_denoise_host): the device-resident path's
+result for every slab count and route, through the C ABI.
+
+The pipeline's x-slab Gram launches have fewer columns than a whole-volume
+launch, so they split each column's z walk into runs (the prefix restarts at a
+run's first window): the Gram matrices agree up to FP64 rounding of the prefix
+differences, the ranks are the same and the volumes agree far below the
+parity tolerance (asserted here at 1e-9 of the largest value)."""
 
 import numpy as np
 import pytest
@@ -18,6 +24,13 @@ def _volume(shape, seed, rank=4):
     maps = rng.random(shape[:3] + (rank,))
     curves = rng.standard_normal((rank, shape[3]))
     return (maps @ curves + 0.05 * rng.standard_normal(shape)).astype(np.float32)
+
+
+def _assert_same(got, want, route):
+    # factor route: rounding of the Gram's prefix differences only; projector
+    # route: float atomics in the scatter as well
+    tol = 1e-9 if route == "factors" else 1e-6
+    np.testing.assert_allclose(got, want, rtol=0, atol=tol * np.abs(want).max())
 
 
 def _device_result(d, vol):
@@ -44,11 +57,11 @@ def test_host_pipeline_matches_device_path(pkg, shape, patch, stride, form):
         d.nslabs = nslabs
         print("host pipeline", shape, patch, stride, form, "nslabs", nslabs, flush=True)
         got = d(vol)
-        if d.route == "factors":  # no atomics anywhere: bit for bit
-            np.testing.assert_array_equal(got, want)
-        else:  # float atomics in the scatter: same up to summation order
-            np.testing.assert_allclose(got, want, rtol=0, atol=1e-6 * np.abs(want).max())
+        _assert_same(got, want, d.route)
         np.testing.assert_array_equal(d.info()["rank"], want_rank)
+        # the same call twice: bit for bit (same launches, no atomics on the factor route)
+        if d.route == "factors":
+            np.testing.assert_array_equal(d(vol), got)
 
 
 def test_host_pipeline_pageable_output_through_staging(pkg, lib):
@@ -67,7 +80,7 @@ def test_host_pipeline_pageable_output_through_staging(pkg, lib):
         d.vol_d.data_ptr(), d.out_d.data_ptr(), stage_in.data_ptr(), stage_out.data_ptr(),
         4, 3, *(s.cuda_stream for s in streams), None, None, None)
     pkg._lib.check(rc, "mppca_denoise_host")
-    np.testing.assert_array_equal(out, want)
+    _assert_same(out, want, "factors")
 
 
 def test_host_pipeline_one_stream(pkg, lib):
@@ -84,4 +97,4 @@ def test_host_pipeline_one_stream(pkg, lib):
         d.vol_d.data_ptr(), d.out_d.data_ptr(), pin_in.data_ptr(), None, 5, 2,
         None, None, None, None, None, None)
     pkg._lib.check(rc, "mppca_denoise_host")
-    np.testing.assert_array_equal(out.numpy(), want)
+    _assert_same(out.numpy(), want, "factors")

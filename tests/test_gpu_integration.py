@@ -4,8 +4,8 @@ The stub module is extracted from the document and executed in a fresh
 process without torch (numpy + ctypes + libamdhip64 + libmppca only), with a
 stand-in for the reference's ``common`` module that supplies the geometry
 globals (common.py:27-29). Its output must be bit-identical to the package's
-``denoise``: both enqueue the same kernels on the same inputs, and the path
-has no atomics.
+device-resident ``denoise`` (a CUDA tensor in): both enqueue the same kernels
+on the same inputs, and the path has no atomics.
 """
 
 import os
@@ -67,6 +67,8 @@ def test_reference_side_binding(pkg, tmp_path):
                        cwd=tmp_path, env=env, timeout=120, capture_output=True, text=True)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
     assert "torch loaded: False" in r.stdout
-    want = pkg.denoise(vol, patch=(5, 5, 5), stride=2)
+    import torch
+
+    want = pkg.denoise(torch.from_numpy(vol).cuda(), patch=(5, 5, 5), stride=2).cpu().numpy()
     np.testing.assert_array_equal(np.load(tmp_path / "out1.npy"), want)
     np.testing.assert_array_equal(np.load(tmp_path / "out2.npy"), want)
