@@ -7,9 +7,12 @@
 //    reflector H = I - tau v v^T, v_0 = 1, as LAPACK's dlarfg defines it).
 //    Lane i owns row k+1+i of the trailing block: p = tau A22 v,
 //    w = p - (tau/2)(p.v) v, A22 -= v w^T + w v^T.
-// 2. Every eigenvalue by bisection on Sturm counts of the LDL^T pivots
+// 2. Every eigenvalue from Sturm counts of the LDL^T pivots
 //    (q_i = d_i - x - e_{i-1}^2 / q_{i-1}; count of q_i < 0), lane j owns the
-//    j-th largest, to ~4 ulps of ||T||.
+//    j-th largest, to ~4 ulps of ||T||: bisection in k_eig64_mp; in the T <= 48
+//    kernels multisection while a bracket holds several eigenvalues, then
+//    Laguerre steps (Wilkinson, The Algebraic Eigenvalue Problem, 7.§60:
+//    monotone, cubic convergence for polynomials with real roots).
 // 3. MP rank p from the eigenvalues (step 0's arithmetic, common.h).
 // 4. Eigenvectors of the top p only (U and the noise vectors are never
 //    formed): twisted factorisation of T - lambda I (Parlett & Dhillon 2000):
@@ -513,76 +516,68 @@ __device__ __forceinline__ int sturm_count3(const double* ds, const double* e2s,
   return c;
 }
 
-// Register-resident forms for the register kernel. dr[i] holds the scaled
-// diagonal (padded with 4 from row T on), er[i] the squared scaled coupling of
-// rows i and i+1 (0 from row T-1 on). A padded row multiplies the minor by
-// 4 - x > 0 (|x| <= 1 + slack) and couples to nothing, so it never changes a
-// sign. The loops are unrolled over TP: no register is indexed at run time.
-//
-// Sturm count of sturm_count3 on the register-resident tridiagonal, without
-// the zero-minor substitution. A zero minor p_i = 0 inside the sequence needs
-// none when e_i != 0: then p_{i+1} = -e_i^2 p_{i-1}, so (p_{i-1}, p_i, p_{i+1})
-// hold exactly one sign change whichever sign p_i is given, as with the
-// substitution. It matters only when e_i = 0 (a split) or at the last row, and
-// either way the last minor comes out zero (a zero followed by a zero coupling
-// zeroes every later minor, the padded rows included). So *zero reports a zero
-// last minor only, and the caller then recounts that lane with the
-// substituting count (sturm_count3 on the same scaled tridiagonal in LDS).
-// The pair is rescaled by a power of two (exact) every 16 rows: with |d_i|,
-// |e_i| <= 1 and |x| <= 1 + slack a minor grows by at most 1 + sqrt(2) per
-// row, so 16 rows stay below 2^21 of the last rescale.
+// Sturm count of sturm_count3 together with f = det(T_T - x I) over all TP
+// rows and its first two derivatives, from the minor recurrence
+//   p_i = (d_i - x) p_{i-1} - e_{i-1}^2 p_{i-2},
+//   p'_i = (d_i - x) p'_{i-1} - e_{i-1}^2 p'_{i-2} - p_{i-1},
+//   p''_i = (d_i - x) p''_{i-1} - e_{i-1}^2 p''_{i-2} - 2 p'_{i-1}.
+// ds holds the scaled diagonal padded with 4 from row T on, e2 the squared
+// scaled couplings (0 from row T - 1 on): a padded row multiplies every minor by
+// 4 - x > 0 (|x| <= 1 + slack) and couples to nothing, so it changes no sign and
+// adds a root at 4 (the Laguerre step takes the degree TP).
+// - No zero-minor substitution. A zero minor p_i = 0 inside the sequence needs
+//   none when e_i != 0: then p_{i+1} = -e_i^2 p_{i-1}, so (p_{i-1}, p_i, p_{i+1})
+//   hold exactly one sign change whichever sign p_i is given, as with the
+//   substitution. It matters only when e_i = 0 (a split) or at the last row,
+//   and either way the last minor comes out zero (a zero followed by a zero
+//   coupling zeroes every later minor, the padded rows included). So *zero
+//   reports a zero last minor only, and the caller then recounts that lane with
+//   the substituting count (sturm_count3). A CPU emulation over 48 K counts
+//   with exact zeros, splits and diagonal matrices matched sturm_count3.
+// - The six values are rescaled by a power of two (exact; their ratios are
+//   what the Laguerre step uses) every 16 rows: with |d_i|, |e_i| <= 1 a minor
+//   grows by at most 1 + sqrt(2) per row, its derivatives by a further factor
+//   below 16 per order over 16 rows.
+// - The tridiagonal is read from LDS, two broadcast reads per row: in registers
+//   as well as the six recurrences it would take the kernel past 256 VGPRs.
 template <int TP>
-__device__ __forceinline__ int sturm_count_fast(const double (&dr)[TP], const double (&er)[TP],
-                                                double x, bool* zero) {
-  double pm = 1.0;
-  double pc = dr[0] - x;
+__device__ __forceinline__ int laguerre_eval(const double* __restrict__ ds,
+                                             const double* __restrict__ e2, double x,
+                                             double* f, double* f1, double* f2, bool* zero) {
+  double pm = 1.0, pc = ds[0] - x, dpm = 0.0, dpc = -1.0, ddpm = 0.0, ddpc = 0.0;
   int c = pc < 0.0 ? 1 : 0;
 #pragma unroll
   for (int i = 1; i < TP; ++i) {
-    const double pn = fma(dr[i] - x, pc, -er[i - 1] * pm);
+    const double t = ds[i] - x, ne = -e2[i - 1];
+    const double pn = fma(t, pc, ne * pm);
+    const double dpn = fma(t, dpc, fma(ne, dpm, -pc));
+    const double ddpn = fma(t, ddpc, fma(ne, ddpm, -2.0 * dpc));
     c += (pn < 0.0) != (pc < 0.0) ? 1 : 0;
-    pm = pc;
-    pc = pn;
-    if ((i & 15) == 0) {
-      const int ex = __builtin_amdgcn_frexp_exp(fmax(fabs(pc), fabs(pm)));
-      pc = __builtin_amdgcn_ldexp(pc, -ex);
-      pm = __builtin_amdgcn_ldexp(pm, -ex);
-    }
-  }
-  *zero = pc == 0.0;
-  return c;
-}
-
-// Newton correction f(x) / f'(x) of the characteristic polynomial
-// f = det(T_T - x I) of the leading T rows, from the minor recurrence over all
-// TP rows and its derivative dp_i = (d_i - x) dp_{i-1} - e_{i-1}^2 dp_{i-2} -
-// p_{i-1}, the four values rescaled together every 16 rows (the ratio is
-// unchanged; the derivative adds at most a factor of 16 to the growth). The
-// k = TP - T padded rows make P = f (4 - x)^k, so f / f' = P / (P' + k P / (4 - x)).
-// NaN when the derivative vanishes.
-template <int TP>
-__device__ __forceinline__ double newton_corr_reg(const double (&dr)[TP], const double (&er)[TP],
-                                                  int T, double x) {
-  double pm = 1.0, pc = dr[0] - x, dpm = 0.0, dpc = -1.0;
-#pragma unroll
-  for (int i = 1; i < TP; ++i) {
-    const double t = dr[i] - x;
-    const double pn = fma(t, pc, -er[i - 1] * pm);
-    const double dpn = fma(t, dpc, fma(-er[i - 1], dpm, -pc));
     pm = pc;
     pc = pn;
     dpm = dpc;
     dpc = dpn;
+    ddpm = ddpc;
+    ddpc = ddpn;
     if ((i & 15) == 0) {
-      const int ex = __builtin_amdgcn_frexp_exp(
-          fmax(fmax(fabs(pc), fabs(pm)), fmax(fabs(dpc), fabs(dpm))));
+      const int ex = __builtin_amdgcn_frexp_exp(fmax(fmax(fmax(fabs(pc), fabs(pm)),
+                                                          fmax(fabs(dpc), fabs(dpm))),
+                                                     fmax(fabs(ddpc), fabs(ddpm))));
       pc = __builtin_amdgcn_ldexp(pc, -ex);
       pm = __builtin_amdgcn_ldexp(pm, -ex);
       dpc = __builtin_amdgcn_ldexp(dpc, -ex);
       dpm = __builtin_amdgcn_ldexp(dpm, -ex);
+      ddpc = __builtin_amdgcn_ldexp(ddpc, -ex);
+      ddpm = __builtin_amdgcn_ldexp(ddpm, -ex);
+      // no load hoisted across: at most 16 rows of the tridiagonal in flight
+      __builtin_amdgcn_sched_barrier(0);
     }
   }
-  return pc / (dpc + (double)(TP - T) * pc / (4.0 - x));
+  *zero = pc == 0.0;
+  *f = pc;
+  *f1 = dpc;
+  *f2 = ddpc;
+  return c;
 }
 
 // MP rule of mp_rule (common.h) evaluated with every lane: the float32
@@ -637,7 +632,7 @@ __device__ void mp_rule_wave(const float* lam, float* e, float* rc, double* dsfx
 // three kernels on the 8 x 8 lane grid of a wavefront: k_tri_grid reduces each
 // matrix to tridiagonal form in registers; k_eig_vals finds its eigenvalues and
 // MP rank; k_eig_vecs its top-p eigenvectors. Each runs at the occupancy of its
-// own registers (the bisection holds the whole tridiagonal in every lane).
+// own registers.
 // Hand-off in the (clobbered) Gram buffer of each matrix:
 //   row k <= T-3   reflector v_k at columns k+1 .. T-1 (v_{k+1} = 1), tau_k
 //                  at column 0 (v_k vanishes there)
@@ -810,11 +805,10 @@ __host__ __device__ constexpr int vals_slots(int tp) { return tp == 16 ? 33 : tp
 // Eigenvalues of the tridiagonals left by k_tri_grid, packed: lane l of
 // wavefront w takes pair q = 64 w + l of the nb * T (matrix, eigenvalue) pairs,
 // matrix q / T, j = q % T-th largest eigenvalue, so no lane idles when T < 64.
-// Each lane keeps its matrix's scaled tridiagonal in registers (split, scaled
-// and bounded exactly as k_eig_vecs recomputes it). The lanes of one matrix in
-// the wavefront first count at n equal steps of its Gershgorin interval (an
-// n-section for n lanes), then each bisects its own bracket to 2^-33 of the
-// scaled norm and takes two Newton steps (see below). The eigenvalues go,
+// The wavefront's matrices are split, scaled and bounded in LDS exactly as
+// k_eig_vecs recomputes them. The lanes of one matrix in the wavefront first
+// count at n equal steps of its Gershgorin interval (an n-section for n lanes),
+// then search by multisection and Laguerre steps (see below). The eigenvalues go,
 // exactly, to the end of the patch's output block for k_eig_vecs, which runs
 // the MP rule. A matrix with a non-finite entry is left to k_eig_vecs.
 template <int TP>
@@ -827,6 +821,8 @@ __global__ __launch_bounds__(64) void k_eig_vals(const double* __restrict__ gram
   __shared__ int s_cnt[64];                       // eigenvalues below each point
   __shared__ double s_lo[NS], s_hi[NS], s_bsc[NS];  // scaled interval, norm
   __shared__ int s_fin[NS];                       // all entries finite
+  __shared__ double s_glo[64], s_ghi[64];         // search brackets, to find the groups
+  __shared__ int s_gkey[64];
 
   const int lane = threadIdx.x;
   const long long npair = nb * T;
@@ -902,12 +898,6 @@ __global__ __launch_bounds__(64) void k_eig_vals(const double* __restrict__ gram
     }
     __syncthreads();
   }
-  double dr[TP], er[TP];
-#pragma unroll
-  for (int i = 0; i < TP; ++i) {
-    dr[i] = s_ds[slot][i];
-    er[i] = s_e2s[slot][i];
-  }
   const bool finite = s_fin[slot] != 0;
   const double bsc = s_bsc[slot];
   // n-section of the Gershgorin interval over this matrix's ns lanes: lane
@@ -917,16 +907,38 @@ __global__ __launch_bounds__(64) void k_eig_vals(const double* __restrict__ gram
     const int m = lane - ls;
     const double xm = m + 1 == ns ? H : fma((double)(m + 1) / (double)ns, H - L, L);
     bool zero;
-    int cnt = sturm_count_fast<TP>(dr, er, xm, &zero);
+    double f, f1, f2;
+    int cnt = laguerre_eval<TP>(s_ds[slot], s_e2s[slot], xm, &f, &f1, &f2, &zero);
     if (zero) cnt = sturm_count3(s_ds[slot], s_e2s[slot], T, xm);
     if (m + 1 == ns) cnt = T;
     s_cnt[lane] = cnt;
     s_pt[lane] = xm;
   }
   __syncthreads();
-  double lam = NAN;
+  // ---- eigenvalue search ------------------------------------------------------
+  // Each lane starts from its n-section bracket [lo, hi] (clo, chi: the counts at
+  // its ends) and the wavefront iterates in step:
+  //  - lanes whose bracket holds several eigenvalues form a group with the other
+  //    lanes of the same matrix and bracket (contiguous lanes); a group of k
+  //    lanes counts at k equal steps inside it (a (k+1)-section), and each lane
+  //    keeps the part that holds its own eigenvalue;
+  //  - a lane whose bracket holds its eigenvalue alone (chi - clo == 1) takes
+  //    Laguerre steps toward it, on the side that the count at the current point
+  //    names. For a polynomial whose roots are all real that step lands between
+  //    the point and the nearest root on that side, here this eigenvalue, so it
+  //    converges monotonically (cubically near the root). It stops at a step
+  //    below 2^-46 of the scaled norm, the rounding level of the recurrences;
+  //  - a group whose bracket falls below 2^-33 of the norm is a cluster: its
+  //    lanes take the bracket's midpoint, one value for all of them (the cluster
+  //    test of k_eig_vecs relies on that).
+  // A CPU emulation (float64, the same recurrences) on matrices of the default
+  // workload took 8-10 iterations per matrix, against ~30 bisection steps and
+  // two Newton steps, to the same accuracy.
+  const double TPd = (double)TP;  // degree of the padded characteristic polynomial
+  double lo = L, hi = H, x = 0.0, lam = NAN;
+  int clo = 0, chi = T, mode = 2;  // 0: in a group, 1: Laguerre, 2: done
   if (live && finite) {
-    // bracket: the first section point whose count exceeds kk, and the one before
+    // the first section point whose count exceeds kk, and the one before
     int lo_i = 0, n = ns;  // lower bound over s_cnt[ls .. ls + ns)
     while (n > 0) {
       const int half = n >> 1;
@@ -937,33 +949,96 @@ __global__ __launch_bounds__(64) void k_eig_vals(const double* __restrict__ gram
         n = half;
       }
     }
-    double lo = lo_i > 0 ? s_pt[ls + lo_i - 1] : L, hi = s_pt[ls + lo_i];
-    // Bisection to 2^-33 of the scaled norm brackets each eigenvalue (the MP
-    // rule needs ~2^-24 relative to lambda_max); two Newton steps on the
-    // characteristic polynomial, clamped to the bracket (a root near its edge
-    // makes the first step overshoot), then reach full precision for every
-    // eigenvalue separated from its neighbours. Copies of a cluster share one
-    // bracket and so take identical steps (the cluster test of k_eig_vecs
-    // relies on that); a vanishing derivative keeps the current point.
-    const double tol = 0x1p-33;
-    for (int it = 0; it < 96 && hi - lo > tol; ++it) {
-      const double x = 0.5 * (lo + hi);
-      bool zero;
-      int cnt = sturm_count_fast<TP>(dr, er, x, &zero);
-      if (zero) cnt = sturm_count3(s_ds[slot], s_e2s[slot], T, x);  // exact zero minor: rare
-      if (cnt > kk)
-        hi = x;
-      else
-        lo = x;
+    if (lo_i > 0) {
+      lo = s_pt[ls + lo_i - 1];
+      clo = s_cnt[ls + lo_i - 1];
     }
-    double x = 0.5 * (lo + hi);
-#pragma unroll
-    for (int st = 0; st < 2; ++st) {
-      const double xn = x - newton_corr_reg<TP>(dr, er, T, x);
-      x = isfinite(xn) ? fmin(fmax(xn, lo), hi) : x;
-    }
-    lam = x * bsc;
+    hi = s_pt[ls + lo_i];
+    chi = s_cnt[ls + lo_i];
+    mode = chi - clo == 1 ? 1 : 0;
+    x = 0.5 * (lo + hi);
   }
+  for (int it = 0; it < 64 && __any(mode != 2); ++it) {
+    // groups: runs of lanes in mode 0 with the same matrix and bracket
+    __syncthreads();  // the previous iteration's reads of the point tables are done
+    s_glo[lane] = lo;
+    s_ghi[lane] = hi;
+    s_gkey[lane] = mode == 0 ? slot : -1 - lane;
+    __syncthreads();
+    const int pl = lane > 0 ? lane - 1 : 0;  // the lane before
+    const bool start =
+        lane == 0 || s_gkey[pl] != s_gkey[lane] || s_glo[pl] != lo || s_ghi[pl] != hi;
+    const unsigned long long S = __ballot(start);
+    const unsigned long long upto = lane == 63 ? ~0ull : (2ull << lane) - 1ull;
+    const int gs = 63 - __clzll(S & upto);  // this lane's group: [gs, ge)
+    const unsigned long long above = S & ~upto;
+    const int ge = above ? __ffsll(above) - 1 : 64;
+    const int k = ge - gs;
+    if (mode == 0) x = fma((double)(lane - gs + 1) / (double)(k + 1), hi - lo, lo);
+    double f, f1, f2;
+    bool zero;
+    int c = laguerre_eval<TP>(s_ds[slot], s_e2s[slot], x, &f, &f1, &f2, &zero);
+    if (zero) c = sturm_count3(s_ds[slot], s_e2s[slot], T, x);  // exact zero minor: rare
+    s_pt[lane] = x;
+    s_cnt[lane] = c;
+    __syncthreads();
+    if (mode == 0) {
+      // the group's first point whose count exceeds kk (points rise with the lane)
+      int a = 0, m = k;
+      while (m > 0) {
+        const int half = m >> 1;
+        if (s_cnt[gs + a + half] <= kk) {
+          a += half + 1;
+          m -= half + 1;
+        } else {
+          m = half;
+        }
+      }
+      if (a > 0) {
+        lo = s_pt[gs + a - 1];
+        clo = s_cnt[gs + a - 1];
+      }
+      if (a < k) {
+        hi = s_pt[gs + a];
+        chi = s_cnt[gs + a];
+      }
+      if (chi - clo == 1) {
+        mode = 1;
+        x = 0.5 * (lo + hi);
+      } else if (hi - lo <= 0x1p-33) {
+        lam = 0.5 * (lo + hi);
+        mode = 2;
+      }
+    } else if (mode == 1) {
+      if (c > kk) {
+        hi = x;
+        chi = c;
+      } else {
+        lo = x;
+        clo = c;
+      }
+      if (f == 0.0) {  // x is the eigenvalue
+        lam = x;
+        mode = 2;
+      } else {
+        const double G = f1 / f, Hs = G * G - f2 / f;
+        const double dsq = sqrt(fmax((TPd - 1.0) * (TPd * Hs - G * G), 0.0));
+        const double xn = x - TPd / (c <= kk ? G - dsq : G + dsq);
+        if (isfinite(xn) && fabs(xn - x) <= 0x1p-46) {
+          lam = fmin(fmax(xn, lo), hi);
+          mode = 2;
+        } else if (hi - lo <= 0x1p-46) {
+          lam = 0.5 * (lo + hi);
+          mode = 2;
+        } else {
+          x = isfinite(xn) ? fmin(fmax(xn, lo), hi) : 0.5 * (lo + hi);
+        }
+      }
+    }
+  }
+  if (mode == 0) lam = 0.5 * (lo + hi);  // the iteration cap (not reached in practice)
+  if (mode == 1) lam = x;
+  lam = live && finite ? lam * bsc : NAN;
   // the eigenvalue for k_eig_vecs, exactly (two 32-bit halves each); the
   // matrix and index are derived again from the lane (fewer live registers)
   if (live && finite) {
@@ -981,8 +1056,7 @@ __global__ __launch_bounds__(64) void k_eig_vals(const double* __restrict__ gram
 // Top-p eigenvectors of the tridiagonal left by k_tri_grid, at the eigenvalues
 // and rank left by k_eig_vals: twisted factorisation, Gram-Schmidt inside the
 // group, back-transform on the lane grid, factors or projector rows. Its own
-// kernel so that it runs at the occupancy of its few registers (the bisection
-// keeps the whole tridiagonal in registers): these phases are latency-bound.
+// kernel so that it runs at the occupancy of its few registers.
 template <int TP>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_eig_vecs(const double* __restrict__ gram, int T, int N,
                                                  long long nb, float* __restrict__ proj,
