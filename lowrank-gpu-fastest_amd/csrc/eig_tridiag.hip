@@ -11,8 +11,9 @@
 //    (q_i = d_i - x - e_{i-1}^2 / q_{i-1}; count of q_i < 0), lane j owns the
 //    j-th largest, to ~4 ulps of ||T||: bisection in k_eig64_mp; in the T <= 48
 //    kernels multisection while a bracket holds several eigenvalues, then
-//    Laguerre steps (Wilkinson, The Algebraic Eigenvalue Problem, 7.§60:
-//    monotone, cubic convergence for polynomials with real roots).
+//    Laguerre's method, which for a polynomial with only real roots converges
+//    monotonically (cubically near a simple root) to the nearest root on the
+//    side it is aimed at.
 // 3. MP rank p from the eigenvalues (step 0's arithmetic, common.h).
 // 4. Eigenvectors of the top p only (U and the noise vectors are never
 //    formed): twisted factorisation of T - lambda I (Parlett & Dhillon 2000):
