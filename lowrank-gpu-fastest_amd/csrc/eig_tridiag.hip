@@ -541,29 +541,32 @@ __device__ __forceinline__ int sturm_count3(const double* ds, const double* e2s,
 //   below 16 per order over 16 rows.
 // - The tridiagonal is read from LDS, two broadcast reads per row: in registers
 //   as well as the six recurrences it would take the kernel past 256 VGPRs.
-template <int TP>
+// COUNT and DERIV leave out the count (returned as 0) or the derivatives.
+template <int TP, bool COUNT, bool DERIV>
 __device__ __forceinline__ int laguerre_eval(const double* __restrict__ ds,
                                              const double* __restrict__ e2, double x,
                                              double* f, double* f1, double* f2, bool* zero) {
   double pm = 1.0, pc = ds[0] - x, dpm = 0.0, dpc = -1.0, ddpm = 0.0, ddpc = 0.0;
-  int c = pc < 0.0 ? 1 : 0;
+  int c = COUNT && pc < 0.0 ? 1 : 0;
 #pragma unroll
   for (int i = 1; i < TP; ++i) {
     const double t = ds[i] - x, ne = -e2[i - 1];
     const double pn = fma(t, pc, ne * pm);
-    const double dpn = fma(t, dpc, fma(ne, dpm, -pc));
-    const double ddpn = fma(t, ddpc, fma(ne, ddpm, -2.0 * dpc));
-    c += (pn < 0.0) != (pc < 0.0) ? 1 : 0;
+    if (DERIV) {
+      const double dpn = fma(t, dpc, fma(ne, dpm, -pc));
+      const double ddpn = fma(t, ddpc, fma(ne, ddpm, -2.0 * dpc));
+      dpm = dpc;
+      dpc = dpn;
+      ddpm = ddpc;
+      ddpc = ddpn;
+    }
+    if (COUNT) c += (pn < 0.0) != (pc < 0.0) ? 1 : 0;
     pm = pc;
     pc = pn;
-    dpm = dpc;
-    dpc = dpn;
-    ddpm = ddpc;
-    ddpc = ddpn;
     if ((i & 15) == 0) {
-      const int ex = __builtin_amdgcn_frexp_exp(fmax(fmax(fmax(fabs(pc), fabs(pm)),
-                                                          fmax(fabs(dpc), fabs(dpm))),
-                                                     fmax(fabs(ddpc), fabs(ddpm))));
+      const double mp = fmax(fabs(pc), fabs(pm));
+      const int ex = __builtin_amdgcn_frexp_exp(
+          DERIV ? fmax(mp, fmax(fmax(fabs(dpc), fabs(dpm)), fmax(fabs(ddpc), fabs(ddpm)))) : mp);
       pc = __builtin_amdgcn_ldexp(pc, -ex);
       pm = __builtin_amdgcn_ldexp(pm, -ex);
       dpc = __builtin_amdgcn_ldexp(dpc, -ex);
@@ -909,7 +912,7 @@ __global__ __launch_bounds__(64) void k_eig_vals(const double* __restrict__ gram
     const double xm = m + 1 == ns ? H : fma((double)(m + 1) / (double)ns, H - L, L);
     bool zero;
     double f, f1, f2;
-    int cnt = laguerre_eval<TP>(s_ds[slot], s_e2s[slot], xm, &f, &f1, &f2, &zero);
+    int cnt = laguerre_eval<TP, true, false>(s_ds[slot], s_e2s[slot], xm, &f, &f1, &f2, &zero);
     if (zero) cnt = sturm_count3(s_ds[slot], s_e2s[slot], T, xm);
     if (m + 1 == ns) cnt = T;
     s_cnt[lane] = cnt;
@@ -978,8 +981,17 @@ __global__ __launch_bounds__(64) void k_eig_vals(const double* __restrict__ gram
     if (mode == 0) x = fma((double)(lane - gs + 1) / (double)(k + 1), hi - lo, lo);
     double f, f1, f2;
     bool zero;
-    int c = laguerre_eval<TP>(s_ds[slot], s_e2s[slot], x, &f, &f1, &f2, &zero);
-    if (zero) c = sturm_count3(s_ds[slot], s_e2s[slot], T, x);  // exact zero minor: rare
+    int c;
+    if (__any(mode == 0)) {
+      c = laguerre_eval<TP, true, true>(s_ds[slot], s_e2s[slot], x, &f, &f1, &f2, &zero);
+      if (zero) c = sturm_count3(s_ds[slot], s_e2s[slot], T, x);  // exact zero minor: rare
+    } else {
+      // no groups left: in an isolating bracket the count at x is clo or clo + 1,
+      // and f = det(T - x I) has the sign (-1)^count (the padded roots lie above)
+      laguerre_eval<TP, false, true>(s_ds[slot], s_e2s[slot], x, &f, &f1, &f2, &zero);
+      c = (f < 0.0) == ((clo & 1) != 0) ? clo : clo + 1;
+      if (f == 0.0) c = sturm_count3(s_ds[slot], s_e2s[slot], T, x);  // no sign: rare
+    }
     s_pt[lane] = x;
     s_cnt[lane] = c;
     __syncthreads();
@@ -1011,28 +1023,32 @@ __global__ __launch_bounds__(64) void k_eig_vals(const double* __restrict__ gram
         mode = 2;
       }
     } else if (mode == 1) {
-      if (c > kk) {
-        hi = x;
-        chi = c;
-      } else {
-        lo = x;
-        clo = c;
-      }
-      if (f == 0.0) {  // x is the eigenvalue
+      if (f == 0.0 && x > lo && x < hi) {  // x is the one eigenvalue of the bracket
         lam = x;
         mode = 2;
       } else {
-        const double G = f1 / f, Hs = G * G - f2 / f;
-        const double dsq = sqrt(fmax((TPd - 1.0) * (TPd * Hs - G * G), 0.0));
-        const double xn = x - TPd / (c <= kk ? G - dsq : G + dsq);
-        if (isfinite(xn) && fabs(xn - x) <= 0x1p-46) {
-          lam = fmin(fmax(xn, lo), hi);
-          mode = 2;
-        } else if (hi - lo <= 0x1p-46) {
-          lam = 0.5 * (lo + hi);
-          mode = 2;
+        if (c > kk) {
+          hi = x;
+          chi = c;
         } else {
-          x = isfinite(xn) ? fmin(fmax(xn, lo), hi) : 0.5 * (lo + hi);
+          lo = x;
+          clo = c;
+        }
+        if (f == 0.0) {  // an end of the bracket that is another eigenvalue
+          x = 0.5 * (lo + hi);
+        } else {
+          const double G = f1 / f, Hs = G * G - f2 / f;
+          const double dsq = sqrt(fmax((TPd - 1.0) * (TPd * Hs - G * G), 0.0));
+          const double xn = x - TPd / (c <= kk ? G - dsq : G + dsq);
+          if (isfinite(xn) && fabs(xn - x) <= 0x1p-46) {
+            lam = fmin(fmax(xn, lo), hi);
+            mode = 2;
+          } else if (hi - lo <= 0x1p-46) {
+            lam = 0.5 * (lo + hi);
+            mode = 2;
+          } else {
+            x = isfinite(xn) ? fmin(fmax(xn, lo), hi) : 0.5 * (lo + hi);
+          }
         }
       }
     }
