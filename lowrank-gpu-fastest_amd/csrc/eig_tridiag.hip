@@ -587,7 +587,10 @@ __device__ __forceinline__ int laguerre_eval(const double* __restrict__ ds,
 // MP rule of mp_rule (common.h) evaluated with every lane: the float32
 // eigenvalue scaling and the float32 suffix sums keep step 0's sequential order
 // (lane 0), then lane i evaluates inequality i. Same results as mp_rule.
-__device__ void mp_rule_wave(const float* lam, float* e, float* rc, double* dsfx, int T, int N,
+// lam, e, rc and dsfx are disjoint (the serial suffix sums then pipeline their
+// LDS loads instead of waiting out each one).
+__device__ void mp_rule_wave(const float* __restrict__ lam, float* __restrict__ e,
+                             float* __restrict__ rc, double* __restrict__ dsfx, int T, int N,
                              MpErr err, int lane, int* p_out, float* margin_out) {
   const int M = N < T - 1 ? N : T - 1;
   const int n = N > T - 1 ? N : T - 1;
@@ -603,6 +606,7 @@ __device__ void mp_rule_wave(const float* lam, float* e, float* rc, double* dsfx
     float run = e[M - 1];
     rc[M - 1] = run;
     double drun = dsfx[M - 1];
+#pragma unroll 8
     for (int i = M - 2; i >= 0; --i) {
       run = run + e[i];
       rc[i] = run;
