@@ -521,7 +521,7 @@ __device__ __forceinline__ int sturm_count3(const double* ds, const double* e2s,
 // rows and its first two derivatives, from the minor recurrence
 //   p_i = (d_i - x) p_{i-1} - e_{i-1}^2 p_{i-2},
 //   p'_i = (d_i - x) p'_{i-1} - e_{i-1}^2 p'_{i-2} - p_{i-1},
-//   p''_i = (d_i - x) p''_{i-1} - e_{i-1}^2 p''_{i-2} - 2 p'_{i-1}.
+//   q_i = (d_i - x) q_{i-1} - e_{i-1}^2 q_{i-2} - p'_{i-1}   (q = p'' / 2).
 // ds holds the scaled diagonal padded with 4 from row T on, e2 the squared
 // scaled couplings (0 from row T - 1 on): a padded row multiplies every minor by
 // 4 - x > 0 (|x| <= 1 + slack) and couples to nothing, so it changes no sign and
@@ -554,7 +554,7 @@ __device__ __forceinline__ int laguerre_eval(const double* __restrict__ ds,
     const double pn = fma(t, pc, ne * pm);
     if (DERIV) {
       const double dpn = fma(t, dpc, fma(ne, dpm, -pc));
-      const double ddpn = fma(t, ddpc, fma(ne, ddpm, -2.0 * dpc));
+      const double ddpn = fma(t, ddpc, fma(ne, ddpm, -dpc));
       dpm = dpc;
       dpc = dpn;
       ddpm = ddpc;
@@ -580,7 +580,7 @@ __device__ __forceinline__ int laguerre_eval(const double* __restrict__ ds,
   *zero = pc == 0.0;
   *f = pc;
   *f1 = dpc;
-  *f2 = ddpc;
+  *f2 = 2.0 * ddpc;
   return c;
 }
 
