@@ -439,6 +439,51 @@ def test_eig64_degenerate(pkg, lib):
     assert info[2] == -2 and rank[2] == 0
 
 
+@pytest.mark.parametrize("t", [48, 17, 33])
+def test_eig64_structured(pkg, lib, t):
+    """Matrices that stay split after the Householder reduction: zero couplings
+    (splits, exact zero minors in the Sturm counts), exactly repeated
+    eigenvalues (clusters that never isolate), exact zero eigenvalues and a
+    multiple of the identity. Eigenvalues against numpy in float64, ranks
+    against the MP rule on them."""
+    rng = np.random.default_rng(33)
+    n_vox = 1331
+    mats = []
+    d = np.sort(rng.uniform(0.5, 2.0, t))[::-1]
+    d[:3] = [400.0, 150.0, 60.0]
+    mats.append(np.diag(d))                                   # every coupling zero
+    rep = np.full(t, 1.25)
+    rep[:4] = [300.0, 300.0, 80.0, 80.0]                      # repeated signal and noise
+    mats.append(np.diag(rep))
+    blk = np.zeros((t, t))
+    h = t // 2
+    for a, b in ((0, h), (h, t)):                             # two blocks, one split
+        x = rng.standard_normal((n_vox, b - a)) * 0.05
+        x[:, 0] += rng.standard_normal(n_vox) * (3.0 if a == 0 else 1.5)
+        blk[a:b, a:b] = x.T @ x
+    mats.append(blk)
+    zer = np.diag(np.concatenate([[500.0, 90.0], rng.uniform(0.8, 1.2, t - 5), [0.0, 0.0, 0.0]]))
+    mats.append(zer)                                          # exact zero eigenvalues
+    mats.append(np.eye(t) * 2.5)                              # one cluster of t copies
+    G = np.stack(mats)
+    proj, rank, w, mg, lam, info = _eig64(pkg, lib, G, n_vox)
+    assert np.all(info == 0)
+    ref_l = np.linalg.eigvalsh(G)[:, ::-1]
+    nrm = np.abs(ref_l).max(axis=1, keepdims=True)
+    # copies of a repeated eigenvalue never isolate: their bracket is resolved
+    # to 2^-33 of the norm (isolated ones to full precision)
+    assert np.all(np.abs(lam - ref_l) <= 4 * EPS32 * np.abs(ref_l) + 2.0**-33 * nrm)
+    want = np.array([_mp_reference(np.sort(lam[i]).astype(np.float32), n_vox, t)
+                     for i in range(len(mats))])
+    np.testing.assert_array_equal(rank, want)
+    np.testing.assert_array_equal(w, (1.0 / (2.0 + rank)).astype(np.float32))
+    # projectors of the well-separated signal subspaces
+    for i in (0, 2, 3):
+        vl, vv = np.linalg.eigh(G[i])
+        vp = vv[:, ::-1][:, :rank[i]]
+        np.testing.assert_allclose(proj[i], vp @ vp.T, rtol=0, atol=1e-6)
+
+
 @pytest.mark.parametrize("n_vox,t", [(1331, 48), (125, 16), (343, 30), (60, 17), (40, 64),
                                      (200, 40)])
 def test_eig64_factors_match_projector(pkg, lib, n_vox, t):
