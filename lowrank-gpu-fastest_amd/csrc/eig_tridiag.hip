@@ -940,8 +940,8 @@ __global__ __launch_bounds__(64) void k_eig_vals(const double* __restrict__ gram
   //    lanes take the bracket's midpoint, one value for all of them (the cluster
   //    test of k_eig_vecs relies on that).
   // A CPU emulation (float64, the same recurrences) on matrices of the default
-  // workload took 8-10 iterations per matrix, against ~30 bisection steps and
-  // two Newton steps, to the same accuracy.
+  // workload took 8-10 iterations per matrix; bisection to the same accuracy
+  // takes ~30 counts and two Newton steps.
   const double TPd = (double)TP;  // degree of the padded characteristic polynomial
   double lo = L, hi = H, x = 0.0, lam = NAN;
   int clo = 0, chi = T, mode = 2;  // 0: in a group, 1: Laguerre, 2: done
