@@ -327,45 +327,366 @@ __global__ __launch_bounds__(64 * NTB * (NTB + 1) / 2 * slide_cols(NTB)) __attri
   }
 }
 
+// ---------------------------------------------------------------------------
+// Sliding-window Gram on int8 MFMA: the walk of k_gram_slide (the prefix P(z)
+// over a column's planes, -P stored at a window's first plane, P plus the
+// value read back after its last) with every plane's products formed exactly
+// in integers.
+//  - A plane (footprint voxels x frames of one column) is scaled by 2^s,
+//    s = 46 - e where every |x| < 2^e, and rounded to integers X, |X| < 2^46:
+//    at most 2^(e-47) <= 2^-46 * max|x| off per value.
+//  - X has six signed 8-bit limbs, X = sum_a L_a 2^(8a), L_a in [-128, 127]:
+//    the bytes of X + 0x808080808080, less 128 each. They are the low six
+//    bytes of fma(x, 2^s, 1.5 * 2^52 + 0x808080808080), whose mantissa holds
+//    that sum once rounded to an integer.
+//  - sum_v X_i X_j = sum_{a,b} 2^(8(a+b)) sum_v L_a,i L_b,j. The 26 limb
+//    pairs with a + b >= 4 run on v_mfma_i32_16x16x64_i8, exact in one
+//    32-bit accumulator per weight w = a + b (at most 6 pairs x 128 voxels x
+//    2^14 < 2^24 per plane). The 10 pairs below add up to less than 2^40.1
+//    against products up to 2^92 and are left out: at most 2^-51.9 * 2^(2e)
+//    per voxel.
+//  - The seven weight sums are combined in FP64 (Horner in 2^8) and added
+//    to the prefix scaled by 2^(32 - 2s). The planes of a window enter its
+//    entries once: a Gram entry is off by its planes' quantisation (at most
+//    2^-46 max|x| (|x_i| + |x_j|) + 2^-49 max|x|^2 per voxel) plus the FP64
+//    rounding of the prefix difference.
+// A block's plane costs 2 x 26 int8 MFMAs of 16 cycles where FP64 takes 32 of
+// 64. A plane holding a non-finite value makes the prefix NaN, which flags
+// every later window of the run as k_gram_slide's NaN products do.
+// Plane tiles in LDS: one per limb, a row of R = F rounded up to 64 bytes per
+// frame (zero past the footprint and past T); the 16-byte chunk c of frame f
+// sits at chunk c ^ (f mod R/16), so that the 16 frames of one MFMA operand
+// read fall on distinct banks.
+// ---------------------------------------------------------------------------
+typedef int intx4 __attribute__((ext_vector_type(4)));
+constexpr int kLimbs = 6;                          // signed 8-bit limbs per value
+constexpr int kLimbW0 = 4;                         // lightest limb pair weight formed
+constexpr int kLimbNW = 2 * kLimbs - 1 - kLimbW0;  // weights 4 .. 10
+constexpr int kLimbMaxLoads = 16;                  // plane floats each thread stages (one chunk)
+__host__ __device__ constexpr int limb_rows(int F) { return (F + 63) / 64 * 64; }
+// per column: two sets of kLimbs tiles, the footprint offsets (padded to the
+// tile rows), two sets of per-wavefront plane maxima
+__host__ __device__ constexpr size_t limb_col_bytes(int ntb, int F) {
+  return ((size_t)2 * kLimbs * 16 * ntb * limb_rows(F) +
+          4 * (size_t)(limb_rows(F) + ntb * (ntb + 1)) + 15) / 16 * 16;
+}
+__host__ __device__ constexpr size_t limb_lds(int ntb, int F) {
+  return (size_t)slide_cols(ntb) * limb_col_bytes(ntb, F);
+}
+
+template <int NTB>
+__global__ __launch_bounds__(64 * NTB * (NTB + 1) / 2 * slide_cols(NTB)) void k_gram_limb(
+    const float* __restrict__ xc, double* __restrict__ gram, Geom g, int ix0, int ncol,
+    int nseg, int per_xcd) {
+  constexpr int NBLK = NTB * (NTB + 1) / 2, NCOL = 64 * NBLK;  // threads per column
+  constexpr int TS = 16 * NTB, QS = slide_rows_per_pass(NTB);
+  static_assert(QS * TS == NCOL, "staging grid must cover a column's threads");
+  extern __shared__ __align__(16) unsigned char s_lb[];
+  const int T = g.T, F = g.px * g.py, R = limb_rows(F);
+  const int TB = TS * R;  // bytes per limb tile
+  const int cmask = R / 16 - 1;
+  // columns and runs as k_gram_slide
+  const int unit = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
+  const int blk = unit / nseg, seg = unit - blk * nseg;
+  const int h = threadIdx.x / NCOL, tid = threadIdx.x - h * NCOL;
+  const int col_raw = blk * slide_cols(NTB) + h;
+  if (blk * slide_cols(NTB) >= ncol) return;
+  const int w0 = (int)((long long)seg * g.ncz / nseg), w1 = (int)((long long)(seg + 1) * g.ncz / nseg);
+  const int z0 = corner_of(w0, g.nsz, g.Z, g.pz, g.stride);
+  const int z1 = corner_of(w1 - 1, g.nsz, g.Z, g.pz, g.stride) + g.pz;
+  const bool writer = col_raw < ncol;
+  const int col = writer ? col_raw : ncol - 1;
+  unsigned char* s_t = s_lb + h * limb_col_bytes(NTB, F);  // 2 x kLimbs tiles
+  unsigned* s_rowb = (unsigned*)(s_t + 2 * kLimbs * TB);   // byte offset of footprint row n
+  unsigned* s_wmax = s_rowb + R;                           // 2 x per-wavefront plane maxima
+  const int ix = ix0 + col / g.ncy, iy = col % g.ncy;
+  const int cx = corner_of(ix, g.nsx, g.X, g.px, g.stride);
+  const int cy = corner_of(iy, g.nsy, g.Y, g.py, g.stride);
+  for (int n = tid; n < R; n += NCOL) {  // rows past the footprint repeat its last
+    const int nn = n < F ? n : F - 1, dx = nn / g.py, dy = nn - dx * g.py;
+    s_rowb[n] = (unsigned)((dx * g.Y + dy) * g.Z) * (unsigned)T * 4u;
+  }
+
+  // staging role: thread -> frame st, chunk sc of 16 voxels (value k: voxel
+  // 16 sc + k); a column's threads cover every chunk (checked by the launcher)
+  const int st = tid % TS, sc = tid / TS;
+  const bool st_ok = st < T, sc_ok = sc < R / 16;
+  // (cx, cy, 0): the same for a whole wavefront (a column's threads are whole
+  // wavefronts), kept in scalar registers
+  const unsigned long long cb64 = (unsigned long long)(xc + (((long long)cx * g.Y + cy) * g.Z) * T);
+  const char* cb = (const char*)((unsigned long long)(unsigned)__builtin_amdgcn_readfirstlane((int)(cb64 >> 32)) << 32 |
+                                 (unsigned)__builtin_amdgcn_readfirstlane((int)cb64));
+  const unsigned sto = 4u * (unsigned)(st_ok ? st : T - 1);
+  // two planes in registers: plane z in sa when z - z0 is even, else in sb
+  float sa[kLimbMaxLoads], sb[kLimbMaxLoads];
+  unsigned vmask = 0;  // bit k: value k is a footprint voxel (frames past T: zero scale)
+  // the chunk's row offsets come from LDS at each fetch (a chunk past the
+  // tile rows, which converts nothing, reads chunk 0's)
+  const intx4* rob = (const intx4*)(s_rowb + 16 * (sc_ok ? sc : 0));
+  auto fetch = [&](float(&stage)[kLimbMaxLoads], int z) {
+    const char* cbz = cb + (size_t)z * 4u * (unsigned)T;
+#pragma unroll
+    for (int k4 = 0; k4 < kLimbMaxLoads / 4; ++k4) {
+      const intx4 o = rob[k4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) stage[4 * k4 + j] = *(const float*)(cbz + ((unsigned)o[j] + sto));
+    }
+  };
+
+  // compute role: one wavefront per 16 x 16 block (tb1, tb2) of the upper triangle
+  const int lane = threadIdx.x & 63, i = lane & 15, kq = lane >> 4;
+  int tb1 = 0, tb2 = 0;
+  {
+    int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    for (int t1 = 0; t1 < NTB; ++t1) {
+      if (w < NTB - t1) {
+        tb1 = t1;
+        tb2 = t1 + w;
+        break;
+      }
+      w -= NTB - t1;
+    }
+  }
+  const bool diag = tb1 == tb2;
+  const int t2 = tb2 * 16 + i;
+  const unsigned char* opA = s_t + (tb1 * 16 + i) * R;  // frame tb1*16 + i of limb tile 0
+  const unsigned char* opB = s_t + (tb2 * 16 + i) * R;
+  const int sw = i & cmask;  // chunk swizzle of frames tb*16 + i
+  double* Gc = gram + (long long)col * g.ncz * T * T;
+  // block entries of this lane: register r holds row tb1*16 + 4 kq + r, column t2
+  const bool ent_ok = writer && t2 < T;
+  auto entry = [&](int iz, int r) -> double* {
+    return Gc + (long long)iz * T * T + (long long)(tb1 * 16 + 4 * kq + r) * T + t2;
+  };
+  double P[4] = {0.0, 0.0, 0.0, 0.0};
+  auto start = [&](int iz) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+      if (ent_ok && tb1 * 16 + 4 * kq + r < T) *entry(iz, r) = -P[r];
+  };
+  auto finish = [&](int iz) {
+    // the stores of -P precede these loads in this wave's program order
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int ti = tb1 * 16 + 4 * kq + r;
+      if (ent_ok && ti < T) {
+        double* e = entry(iz, r);
+        const double v = P[r] + *e;
+        *e = v;
+        if (!diag) Gc[(long long)iz * T * T + (long long)t2 * T + ti] = v;
+      }
+    }
+  };
+  __syncthreads();  // offset table written
+#pragma unroll
+  for (int k = 0; k < kLimbMaxLoads; ++k) vmask |= (sc_ok && 16 * sc + k < F) ? 1u << k : 0u;
+  const bool vfull = vmask == 0xffffu;
+
+  // the plane's largest |x| over this wavefront's staged values, as bits (a
+  // non-finite value compares above every finite one), into slot set z & 1.
+  // Every staged value is a footprint value of the plane (the clamped rows
+  // and frames repeat real ones).
+  auto plane_max = [&](const float(&stage)[kLimbMaxLoads], int z) {
+    unsigned m = 0;
+#pragma unroll
+    for (int k = 0; k < kLimbMaxLoads; ++k) m = max(m, __float_as_uint(stage[k]) & 0x7fffffffu);
+    // pairs, quads, 8-lane halves, 16-lane rows (DPP), then the four rows
+    m = max(m, (unsigned)__builtin_amdgcn_update_dpp((int)m, (int)m, 0xb1, 0xf, 0xf, false));
+    m = max(m, (unsigned)__builtin_amdgcn_update_dpp((int)m, (int)m, 0x4e, 0xf, 0xf, false));
+    m = max(m, (unsigned)__builtin_amdgcn_update_dpp((int)m, (int)m, 0x141, 0xf, 0xf, false));
+    m = max(m, (unsigned)__builtin_amdgcn_update_dpp((int)m, (int)m, 0x140, 0xf, 0xf, false));
+    m = max(max((unsigned)__builtin_amdgcn_readlane((int)m, 0), (unsigned)__builtin_amdgcn_readlane((int)m, 16)),
+            max((unsigned)__builtin_amdgcn_readlane((int)m, 32), (unsigned)__builtin_amdgcn_readlane((int)m, 48)));
+    if (lane == 0) s_wmax[(z & 1) * NBLK + (tid >> 6)] = m;
+  };
+  constexpr double kMagic = 0x1.8p52 + (double)0x808080808080LL;
+
+  // Software pipeline over the run's planes, one barrier per plane. Step z
+  // issues the MFMAs of plane z - 1 (tile set (z - 1) & 1), converts plane z
+  // into tile set z & 1 while they run, loads plane z + 2 into the registers
+  // plane z leaves, and reduces the maximum of plane z + 1, loaded a step ago.
+  int is = w0, ie = w0;  // next window to start, next window to end
+  double sc2 = 0.0;      // 2^(32 - 2s) of plane z - 1 (NaN: non-finite plane)
+  auto step = [&](int z, float(&cur)[kLimbMaxLoads], float(&nxt)[kLimbMaxLoads])
+      __attribute__((always_inline)) {
+    // tile set (z - 1) & 1 complete, set z & 1 free, maxima of plane z written
+    __syncthreads();
+    const int zm = z - 1;
+    bool act = false, ends = false;
+    if (z > z0) {
+      if (is < w1 && corner_of(is, g.nsz, g.Z, g.pz, g.stride) == zm) start(is++);
+      act = is > ie;  // plane zm lies in a window
+      ends = ie < is && corner_of(ie, g.nsz, g.Z, g.pz, g.stride) + g.pz == zm + 1;
+    }
+    intx4 acc[kLimbNW];
+    if (act) {
+      const unsigned char* tA = opA + (zm & 1) * kLimbs * TB;
+      const unsigned char* tB = opB + (zm & 1) * kLimbs * TB;
+      // limb pairs (a, b), a + b >= kLimbW0, weight w = a + b - kLimbW0; the
+      // first pair of each weight (a == 0 or b == 5) starts from zero
+      auto half = [&](int hf, bool first) __attribute__((always_inline)) {
+        const int co = ((4 * hf + kq) ^ sw) << 4;
+        intx4 bv[kLimbs];
+#pragma unroll
+        for (int b = 0; b < kLimbs; ++b) bv[b] = *(const intx4*)(tB + b * TB + co);
+#pragma unroll
+        for (int a = 0; a < kLimbs; ++a) {
+          intx4 av;
+          if (diag)
+            av = bv[a];
+          else
+            av = *(const intx4*)(tA + a * TB + co);
+#pragma unroll
+          for (int b = (kLimbW0 - a > 0 ? kLimbW0 - a : 0); b < kLimbs; ++b) {
+            const int w = a + b - kLimbW0;
+            const bool zero = first && (a == 0 || b == kLimbs - 1);
+            acc[w] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
+                av, bv[b], zero ? intx4{0, 0, 0, 0} : acc[w], 0, 0, 0);
+          }
+        }
+      };
+      half(0, true);
+      if (R > 64) half(1, false);
+    }
+    double sc2_next = 0.0;
+    if (z < z1) {
+      unsigned M = s_wmax[(z & 1) * NBLK];
+#pragma unroll
+      for (int w = 1; w < NBLK; ++w) M = max(M, s_wmax[(z & 1) * NBLK + w]);
+      const bool bad = M >= 0x7f800000u;
+      const int e = bad ? 0 : __builtin_amdgcn_frexp_expf(__uint_as_float(M));  // max|x| < 2^e
+      const double scl = st_ok ? __builtin_amdgcn_ldexp(1.0, 46 - e) : 0.0;  // frames past T: 0
+      sc2_next = bad ? __builtin_nan("") : __builtin_amdgcn_ldexp(1.0, 32 - 2 * (46 - e));
+      unsigned char* tz = s_t + (z & 1) * kLimbs * TB;
+      if (sc_ok) {
+        if (!vfull) {  // voxels past the footprint convert to zero
+#pragma unroll
+          for (int k = 0; k < kLimbMaxLoads; ++k) cur[k] = ((vmask >> k) & 1u) ? cur[k] : 0.f;
+        }
+        unsigned L[kLimbs][4];  // limb a of voxels 16 sc + 4 d .. + 3 in word d
+#pragma unroll
+        for (int d = 0; d < 4; ++d) {
+          unsigned lo[4], hi[4];
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const int k = 4 * d + j;
+            const unsigned long long b =
+                (unsigned long long)__double_as_longlong(fma((double)cur[k], scl, kMagic));
+            lo[j] = (unsigned)b;
+            hi[j] = (unsigned)(b >> 32);
+          }
+          // byte transpose: limb a of the four voxels into one word
+          const unsigned t01l = __builtin_amdgcn_perm(lo[1], lo[0], 0x05010400u);
+          const unsigned t01h = __builtin_amdgcn_perm(lo[1], lo[0], 0x07030602u);
+          const unsigned t23l = __builtin_amdgcn_perm(lo[3], lo[2], 0x05010400u);
+          const unsigned t23h = __builtin_amdgcn_perm(lo[3], lo[2], 0x07030602u);
+          const unsigned u01 = __builtin_amdgcn_perm(hi[1], hi[0], 0x05010400u);
+          const unsigned u23 = __builtin_amdgcn_perm(hi[3], hi[2], 0x05010400u);
+          L[0][d] = __builtin_amdgcn_perm(t23l, t01l, 0x05040100u) ^ 0x80808080u;
+          L[1][d] = __builtin_amdgcn_perm(t23l, t01l, 0x07060302u) ^ 0x80808080u;
+          L[2][d] = __builtin_amdgcn_perm(t23h, t01h, 0x05040100u) ^ 0x80808080u;
+          L[3][d] = __builtin_amdgcn_perm(t23h, t01h, 0x07060302u) ^ 0x80808080u;
+          L[4][d] = __builtin_amdgcn_perm(u23, u01, 0x05040100u) ^ 0x80808080u;
+          L[5][d] = __builtin_amdgcn_perm(u23, u01, 0x07060302u) ^ 0x80808080u;
+        }
+        // one 16-byte chunk per limb; eight consecutive frames hit eight distinct chunks
+        unsigned char* dst = tz + st * R + ((sc ^ (st & cmask)) << 4);
+#pragma unroll
+        for (int a = 0; a < kLimbs; ++a)
+          *(intx4*)(dst + a * TB) = intx4{(int)L[a][0], (int)L[a][1], (int)L[a][2], (int)L[a][3]};
+      }
+      if (z + 2 < z1) fetch(cur, z + 2);  // in flight for two steps
+    }
+    if (act) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        double t = (double)acc[kLimbNW - 1][r];
+#pragma unroll
+        for (int w = kLimbNW - 2; w >= 0; --w) t = fma(t, 256.0, (double)acc[w][r]);
+        P[r] = fma(t, sc2, P[r]);
+      }
+    }
+    if (ends) finish(ie++);
+    sc2 = sc2_next;
+    if (z + 1 < z1) plane_max(nxt, z + 1);
+  };
+  fetch(sa, z0);
+  if (z0 + 1 < z1) fetch(sb, z0 + 1);
+  plane_max(sa, z0);
+  for (int z = z0; z <= z1; z += 2) {
+    step(z, sa, sb);
+    if (z + 1 <= z1) step(z + 1, sb, sa);
+  }
+}
+
 constexpr int kMaxDevices = 64;
 
-// Workgroups of k_gram_slide<NTB> the current device holds at once (0 if the
-// query fails), cached per device.
-template <int NTB>
-static int slide_slots(size_t lds) {
-  constexpr int NTHR = 64 * NTB * (NTB + 1) / 2 * slide_cols(NTB);
-  static std::atomic<int> cache[kMaxDevices];
+// Workgroups of a sliding kernel (fn, nthr threads, lds bytes) the current
+// device holds at once (0 if the query fails), cached per device and LDS size
+// in `cache` (lds * 2^16 + count).
+static int resident_slots(const void* fn, int nthr, size_t lds, std::atomic<long long>* cache) {
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return 0;
-  int v = cache[dev].load(std::memory_order_relaxed);
-  if (v > 0) return v;
+  const long long v = cache[dev].load(std::memory_order_relaxed);
+  if (v > 0 && (size_t)(v >> 16) == lds) return (int)(v & 0xffff);
   int per_cu = 0, ncu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_gram_slide<NTB>, NTHR, lds) !=
-          hipSuccess ||
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, nthr, lds) != hipSuccess ||
       hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
     return 0;
-  v = per_cu * ncu;
-  cache[dev].store(v, std::memory_order_relaxed);
-  return v;
+  const int n = per_cu * ncu;
+  if (n <= 0 || n > 0xffff) return 0;
+  cache[dev].store((long long)lds << 16 | n, std::memory_order_relaxed);
+  return n;
+}
+
+// Runs per column of a sliding launch over ncol columns (cols per workgroup):
+// as many as keep every workgroup resident at once. Sets *nblk and *per_xcd.
+static int slide_runs(const Geom& g, int ncol, int cols, int slots, int* nblk, int* per_xcd) {
+  *nblk = (ncol + cols - 1) / cols;
+  int nseg = slots > 0 ? slots / *nblk : 1;
+  nseg = nseg < 1 ? 1 : nseg > g.ncz ? g.ncz : nseg;
+  const long long nunit = (long long)*nblk * nseg;
+  if (nunit > (1LL << 30)) return 0;
+  *per_xcd = (int)((nunit + 7) / 8);
+  return nseg;
 }
 
 template <int NTB>
 static int launch_gram_slide_t(const float* xc, double* gram, const Geom& g, int ix0, int nix,
                                hipStream_t st) {
   constexpr int NTHR = 64 * NTB * (NTB + 1) / 2 * slide_cols(NTB);
+  static std::atomic<long long> cache[kMaxDevices];
   const size_t lds = slide_lds(NTB, g.px * g.py);
+  int rc = prepare_lds((const void*)k_gram_slide<NTB>, lds);
+  if (rc != MPPCA_OK) return rc;
   const int ncol = nix * g.ncy;
-  const int nblk = (ncol + slide_cols(NTB) - 1) / slide_cols(NTB);
-  // runs per column: as many as keep every workgroup resident at once
-  const int slots = slide_slots<NTB>(lds);
-  int nseg = slots > 0 ? slots / nblk : 1;
-  nseg = nseg < 1 ? 1 : nseg > g.ncz ? g.ncz : nseg;
-  const long long nunit = (long long)nblk * nseg;
-  if (nunit > (1LL << 30)) return set_err(MPPCA_EUNSUPPORTED, "gram (sliding): too many columns");
-  const int per_xcd = (int)((nunit + 7) / 8);
+  int nblk = 0, per_xcd = 0;
+  const int nseg = slide_runs(g, ncol, slide_cols(NTB),
+                              resident_slots((const void*)k_gram_slide<NTB>, NTHR, lds, cache),
+                              &nblk, &per_xcd);
+  if (nseg == 0) return set_err(MPPCA_EUNSUPPORTED, "gram (sliding): too many columns");
   hipLaunchKernelGGL((k_gram_slide<NTB>), dim3(8u * (unsigned)per_xcd), dim3(NTHR), lds, st, xc,
                      gram, g, ix0, ncol, nseg, per_xcd);
   return launch_status("gram (sliding) kernel");
+}
+
+template <int NTB>
+static int launch_gram_limb_t(const float* xc, double* gram, const Geom& g, int ix0, int nix,
+                              hipStream_t st) {
+  constexpr int NTHR = 64 * NTB * (NTB + 1) / 2 * slide_cols(NTB);
+  static std::atomic<long long> cache[kMaxDevices];
+  const size_t lds = limb_lds(NTB, g.px * g.py);
+  int rc = prepare_lds((const void*)k_gram_limb<NTB>, lds);
+  if (rc != MPPCA_OK) return rc;
+  const int ncol = nix * g.ncy;
+  int nblk = 0, per_xcd = 0;
+  const int nseg = slide_runs(g, ncol, slide_cols(NTB),
+                              resident_slots((const void*)k_gram_limb<NTB>, NTHR, lds, cache),
+                              &nblk, &per_xcd);
+  if (nseg == 0) return set_err(MPPCA_EUNSUPPORTED, "gram (int8 sliding): too many columns");
+  hipLaunchKernelGGL((k_gram_limb<NTB>), dim3(8u * (unsigned)per_xcd), dim3(NTHR), lds, st, xc,
+                     gram, g, ix0, ncol, nseg, per_xcd);
+  return launch_status("gram (int8 sliding) kernel");
 }
 
 // The sliding kernel applies to launches over whole x-corner columns whose
@@ -379,11 +700,29 @@ static bool slide_fits(const Geom& g) {
   return (long long)g.px * g.Y * g.Z * g.T * 4 < (1LL << 31);
 }
 
+// The int8 kernel takes the sliding launches whose plane fits its staging
+// registers (one 16-voxel chunk of one frame per thread) and the LDS.
+static bool limb_fits(const Geom& g) {
+  if (g.T > 64 || (long long)g.px * g.Y * g.Z * g.T * 4 >= (1LL << 31)) return false;
+  const int ntb = ntb_of(g.T), F = g.px * g.py;
+  return limb_rows(F) / 16 <= slide_rows_per_pass(ntb) && limb_lds(ntb, F) <= 160 * 1024;
+}
+
 int launch_gram64(const float* xc, double* gram, const Geom& g, long long b0, long long nb,
                   hipStream_t st) {
-  // whole x-corner columns: the sliding-window kernel
+  // whole x-corner columns: a sliding-window kernel
   const long long per_ix = (long long)g.ncy * g.ncz;
-  if (nb > 0 && b0 % per_ix == 0 && nb % per_ix == 0 && slide_fits(g)) {
+  const bool cols = nb > 0 && b0 % per_ix == 0 && nb % per_ix == 0;
+  if (cols && limb_fits(g)) {
+    const int ix0 = (int)(b0 / per_ix), nix = (int)(nb / per_ix);
+    switch (ntb_of(g.T)) {
+      case 1: return launch_gram_limb_t<1>(xc, gram, g, ix0, nix, st);
+      case 2: return launch_gram_limb_t<2>(xc, gram, g, ix0, nix, st);
+      case 3: return launch_gram_limb_t<3>(xc, gram, g, ix0, nix, st);
+      default: return launch_gram_limb_t<4>(xc, gram, g, ix0, nix, st);
+    }
+  }
+  if (cols && slide_fits(g)) {
     const int ix0 = (int)(b0 / per_ix), nix = (int)(nb / per_ix);
     switch (ntb_of(g.T)) {
       case 1: return launch_gram_slide_t<1>(xc, gram, g, ix0, nix, st);

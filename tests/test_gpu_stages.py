@@ -341,11 +341,34 @@ def _slide_bound(xc, patch, stride):
     return np.stack(out)
 
 
+def _limb_bound(xc, patch, stride):
+    """Quantisation bound of the int8-limb sliding Gram: each plane of a
+    column's footprint is rounded to integers at 2^-46 of its largest |x|
+    (M), and limb pairs worth at most 2^-49 M^2 per voxel are left out, so
+    entry (t, s) of a window is off by at most
+    sum_z 2^-46 M_z (S_z[t] + S_z[s]) + F 2^-48 M_z^2, S_z = sum of |x| over
+    the plane's voxels (the square of the rounding error included)."""
+    import oracle as o
+
+    px, py, pz = patch
+    A = np.abs(xc.astype(np.float64))
+    out = []
+    for i, j, k in o.patch_corners(xc.shape, patch, stride):
+        F = A[i:i + px, j:j + py, k:k + pz]          # px, py, pz, T
+        M = F.max(axis=(0, 1, 3))                    # per plane
+        S = F.sum(axis=(0, 1))                       # pz, T
+        b = 2.0 ** -46 * (M[:, None, None] * (S[:, :, None] + S[:, None, :])).sum(0)
+        out.append(b + px * py * 2.0 ** -48 * (M ** 2).sum())
+    return np.stack(out)
+
+
 def test_gram64_exact(pkg, lib):
-    """FP64 MFMA Gram: products of float32 values are exact, sums carry 53 bits.
-    T > 64 (per-patch sums) is held to the patch-sum bound, T <= 64 (sliding
-    window over each x-y column) to the prefix-difference bound; a launch over
-    part of the patches (not whole columns) takes the per-patch kernel."""
+    """FP64 Gram. T > 64 (per-patch FP64 MFMA sums: products of float32 values
+    are exact, sums carry 53 bits) is held to the patch-sum bound; T <= 64
+    (sliding window over each x-y column, on int8 limbs of each plane rounded
+    at 2^-46 of its largest value, or on FP64 MFMA) to the prefix-difference
+    bound plus the quantisation bound; a launch over part of the patches (not
+    whole columns) takes the per-patch kernel."""
     torch = _torch()
     rng = np.random.default_rng(8)
     for shape, patch, stride in [((13, 13, 13, 48), (11, 11, 11), 2), ((9, 8, 8, 100), (4, 4, 4), 2),
@@ -362,7 +385,8 @@ def test_gram64_exact(pkg, lib):
         A = np.abs(X)
         bound = np.einsum("bnt,bns->bts", A, A) * X.shape[1] * 2.0 ** -52
         if t <= 64:
-            bound = np.maximum(bound, _slide_bound(xc, patch, stride) + np.abs(G64) * 2.0 ** -52)
+            bound = np.maximum(bound, _slide_bound(xc, patch, stride) + _limb_bound(xc, patch, stride)
+                               + np.abs(G64) * 2.0 ** -52)
         dx = _dev(xc)
         for b0, nb in ((0, P), (1, P - 2)):  # whole volume; a range of partial columns
             g = torch.full((nb, t, t), float("nan"), dtype=torch.float64, device="cuda")
