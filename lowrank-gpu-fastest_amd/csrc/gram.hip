@@ -370,27 +370,31 @@ __host__ __device__ constexpr size_t limb_col_bytes(int ntb, int F) {
   return ((size_t)2 * kLimbs * 16 * ntb * limb_rows(F) +
           4 * (size_t)(limb_rows(F) + ntb * (ntb + 1)) + 15) / 16 * 16;
 }
+// columns per workgroup as k_gram_slide (one per workgroup, two sharing a CU,
+// measured 1.5x slower at NTB = 3)
+__host__ __device__ constexpr int limb_cols(int ntb) { return slide_cols(ntb); }
 __host__ __device__ constexpr size_t limb_lds(int ntb, int F) {
-  return (size_t)slide_cols(ntb) * limb_col_bytes(ntb, F);
+  return (size_t)limb_cols(ntb) * limb_col_bytes(ntb, F);
 }
 
-template <int NTB>
-__global__ __launch_bounds__(64 * NTB * (NTB + 1) / 2 * slide_cols(NTB)) void k_gram_limb(
+template <int NTB, int RH>
+__global__ __launch_bounds__(64 * NTB * (NTB + 1) / 2 * limb_cols(NTB)) void k_gram_limb(
     const float* __restrict__ xc, double* __restrict__ gram, Geom g, int ix0, int ncol,
     int nseg, int per_xcd) {
   constexpr int NBLK = NTB * (NTB + 1) / 2, NCOL = 64 * NBLK;  // threads per column
   constexpr int TS = 16 * NTB, QS = slide_rows_per_pass(NTB);
   static_assert(QS * TS == NCOL, "staging grid must cover a column's threads");
   extern __shared__ __align__(16) unsigned char s_lb[];
-  const int T = g.T, F = g.px * g.py, R = limb_rows(F);
+  const int T = g.T, F = g.px * g.py;
+  constexpr int R = 64 * RH;  // tile row bytes: the footprint rounded up to 64 voxels
   const int TB = TS * R;  // bytes per limb tile
   const int cmask = R / 16 - 1;
   // columns and runs as k_gram_slide
   const int unit = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
   const int blk = unit / nseg, seg = unit - blk * nseg;
   const int h = threadIdx.x / NCOL, tid = threadIdx.x - h * NCOL;
-  const int col_raw = blk * slide_cols(NTB) + h;
-  if (blk * slide_cols(NTB) >= ncol) return;
+  const int col_raw = blk * limb_cols(NTB) + h;
+  if (blk * limb_cols(NTB) >= ncol) return;
   const int w0 = (int)((long long)seg * g.ncz / nseg), w1 = (int)((long long)(seg + 1) * g.ncz / nseg);
   const int z0 = corner_of(w0, g.nsz, g.Z, g.pz, g.stride);
   const int z1 = corner_of(w1 - 1, g.nsz, g.Z, g.pz, g.stride) + g.pz;
@@ -413,9 +417,10 @@ __global__ __launch_bounds__(64 * NTB * (NTB + 1) / 2 * slide_cols(NTB)) void k_
   const bool st_ok = st < T, sc_ok = sc < R / 16;
   // (cx, cy, 0): the same for a whole wavefront (a column's threads are whole
   // wavefronts), kept in scalar registers
-  const unsigned long long cb64 = (unsigned long long)(xc + (((long long)cx * g.Y + cy) * g.Z) * T);
-  const char* cb = (const char*)((unsigned long long)(unsigned)__builtin_amdgcn_readfirstlane((int)(cb64 >> 32)) << 32 |
-                                 (unsigned)__builtin_amdgcn_readfirstlane((int)cb64));
+  const unsigned long long cbo = (unsigned long long)(((long long)cx * g.Y + cy) * g.Z) * T * 4u;
+  const char* cb = (const char*)xc +
+                   ((unsigned long long)(unsigned)__builtin_amdgcn_readfirstlane((int)(cbo >> 32)) << 32 |
+                    (unsigned)__builtin_amdgcn_readfirstlane((int)cbo));
   const unsigned sto = 4u * (unsigned)(st_ok ? st : T - 1);
   // two planes in registers: plane z in sa when z - z0 is even, else in sb
   float sa[kLimbMaxLoads], sb[kLimbMaxLoads];
@@ -518,85 +523,93 @@ __global__ __launch_bounds__(64 * NTB * (NTB + 1) / 2 * slide_cols(NTB)) void k_
       act = is > ie;  // plane zm lies in a window
       ends = ie < is && corner_of(ie, g.nsz, g.Z, g.pz, g.stride) + g.pz == zm + 1;
     }
-    intx4 acc[kLimbNW];
-    if (act) {
-      const unsigned char* tA = opA + (zm & 1) * kLimbs * TB;
-      const unsigned char* tB = opB + (zm & 1) * kLimbs * TB;
-      // limb pairs (a, b), a + b >= kLimbW0, weight w = a + b - kLimbW0; the
-      // first pair of each weight (a == 0 or b == 5) starts from zero
-      auto half = [&](int hf, bool first) __attribute__((always_inline)) {
-        const int co = ((4 * hf + kq) ^ sw) << 4;
-        intx4 bv[kLimbs];
+    if (!vfull) {  // voxels past the footprint convert to zero
 #pragma unroll
-        for (int b = 0; b < kLimbs; ++b) bv[b] = *(const intx4*)(tB + b * TB + co);
-#pragma unroll
-        for (int a = 0; a < kLimbs; ++a) {
-          intx4 av;
-          if (diag)
-            av = bv[a];
-          else
-            av = *(const intx4*)(tA + a * TB + co);
-#pragma unroll
-          for (int b = (kLimbW0 - a > 0 ? kLimbW0 - a : 0); b < kLimbs; ++b) {
-            const int w = a + b - kLimbW0;
-            const bool zero = first && (a == 0 || b == kLimbs - 1);
-            acc[w] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
-                av, bv[b], zero ? intx4{0, 0, 0, 0} : acc[w], 0, 0, 0);
-          }
-        }
-      };
-      half(0, true);
-      if (R > 64) half(1, false);
+      for (int k = 0; k < kLimbMaxLoads; ++k) cur[k] = ((vmask >> k) & 1u) ? cur[k] : 0.f;
     }
-    double sc2_next = 0.0;
-    if (z < z1) {
-      unsigned M = s_wmax[(z & 1) * NBLK];
+    // Branch-free from here to the tile writes, so that the conversion of
+    // plane z interleaves with the MFMAs of plane zm: the MFMAs also run at
+    // the first step (on a tile set nothing has written; the sums are
+    // dropped) and the conversion at the last (into a set nothing reads).
+    unsigned M = s_wmax[(z & 1) * NBLK];
 #pragma unroll
-      for (int w = 1; w < NBLK; ++w) M = max(M, s_wmax[(z & 1) * NBLK + w]);
-      const bool bad = M >= 0x7f800000u;
-      const int e = bad ? 0 : __builtin_amdgcn_frexp_expf(__uint_as_float(M));  // max|x| < 2^e
-      const double scl = st_ok ? __builtin_amdgcn_ldexp(1.0, 46 - e) : 0.0;  // frames past T: 0
-      sc2_next = bad ? __builtin_nan("") : __builtin_amdgcn_ldexp(1.0, 32 - 2 * (46 - e));
-      unsigned char* tz = s_t + (z & 1) * kLimbs * TB;
-      if (sc_ok) {
-        if (!vfull) {  // voxels past the footprint convert to zero
+    for (int w = 1; w < NBLK; ++w) M = max(M, s_wmax[(z & 1) * NBLK + w]);
+    const bool bad = M >= 0x7f800000u;
+    const int e = bad ? 0 : __builtin_amdgcn_frexp_expf(__uint_as_float(M));  // max|x| < 2^e
+    const double scl = st_ok ? __builtin_amdgcn_ldexp(1.0, 46 - e) : 0.0;      // frames past T: 0
+    const double sc2_next = bad ? __builtin_nan("") : __builtin_amdgcn_ldexp(1.0, 32 - 2 * (46 - e));
+    const unsigned char* tA = opA + (zm & 1) * kLimbs * TB;
+    const unsigned char* tB = opB + (zm & 1) * kLimbs * TB;
+    unsigned L[kLimbs][4];  // limb a of voxels 16 sc + 4 d .. + 3 in word d
+    auto convert = [&](int d) __attribute__((always_inline)) {
+      unsigned lo[4], hi[4];
 #pragma unroll
-          for (int k = 0; k < kLimbMaxLoads; ++k) cur[k] = ((vmask >> k) & 1u) ? cur[k] : 0.f;
-        }
-        unsigned L[kLimbs][4];  // limb a of voxels 16 sc + 4 d .. + 3 in word d
-#pragma unroll
-        for (int d = 0; d < 4; ++d) {
-          unsigned lo[4], hi[4];
-#pragma unroll
-          for (int j = 0; j < 4; ++j) {
-            const int k = 4 * d + j;
-            const unsigned long long b =
-                (unsigned long long)__double_as_longlong(fma((double)cur[k], scl, kMagic));
-            lo[j] = (unsigned)b;
-            hi[j] = (unsigned)(b >> 32);
-          }
-          // byte transpose: limb a of the four voxels into one word
-          const unsigned t01l = __builtin_amdgcn_perm(lo[1], lo[0], 0x05010400u);
-          const unsigned t01h = __builtin_amdgcn_perm(lo[1], lo[0], 0x07030602u);
-          const unsigned t23l = __builtin_amdgcn_perm(lo[3], lo[2], 0x05010400u);
-          const unsigned t23h = __builtin_amdgcn_perm(lo[3], lo[2], 0x07030602u);
-          const unsigned u01 = __builtin_amdgcn_perm(hi[1], hi[0], 0x05010400u);
-          const unsigned u23 = __builtin_amdgcn_perm(hi[3], hi[2], 0x05010400u);
-          L[0][d] = __builtin_amdgcn_perm(t23l, t01l, 0x05040100u) ^ 0x80808080u;
-          L[1][d] = __builtin_amdgcn_perm(t23l, t01l, 0x07060302u) ^ 0x80808080u;
-          L[2][d] = __builtin_amdgcn_perm(t23h, t01h, 0x05040100u) ^ 0x80808080u;
-          L[3][d] = __builtin_amdgcn_perm(t23h, t01h, 0x07060302u) ^ 0x80808080u;
-          L[4][d] = __builtin_amdgcn_perm(u23, u01, 0x05040100u) ^ 0x80808080u;
-          L[5][d] = __builtin_amdgcn_perm(u23, u01, 0x07060302u) ^ 0x80808080u;
-        }
-        // one 16-byte chunk per limb; eight consecutive frames hit eight distinct chunks
-        unsigned char* dst = tz + st * R + ((sc ^ (st & cmask)) << 4);
-#pragma unroll
-        for (int a = 0; a < kLimbs; ++a)
-          *(intx4*)(dst + a * TB) = intx4{(int)L[a][0], (int)L[a][1], (int)L[a][2], (int)L[a][3]};
+      for (int j = 0; j < 4; ++j) {
+        const unsigned long long bits =
+            (unsigned long long)__double_as_longlong(fma((double)cur[4 * d + j], scl, kMagic));
+        lo[j] = (unsigned)bits;
+        hi[j] = (unsigned)(bits >> 32);
       }
-      if (z + 2 < z1) fetch(cur, z + 2);  // in flight for two steps
+      // byte transpose: limb a of the four voxels into one word
+      const unsigned t01l = __builtin_amdgcn_perm(lo[1], lo[0], 0x05010400u);
+      const unsigned t01h = __builtin_amdgcn_perm(lo[1], lo[0], 0x07030602u);
+      const unsigned t23l = __builtin_amdgcn_perm(lo[3], lo[2], 0x05010400u);
+      const unsigned t23h = __builtin_amdgcn_perm(lo[3], lo[2], 0x07030602u);
+      const unsigned u01 = __builtin_amdgcn_perm(hi[1], hi[0], 0x05010400u);
+      const unsigned u23 = __builtin_amdgcn_perm(hi[3], hi[2], 0x05010400u);
+      L[0][d] = __builtin_amdgcn_perm(t23l, t01l, 0x05040100u) ^ 0x80808080u;
+      L[1][d] = __builtin_amdgcn_perm(t23l, t01l, 0x07060302u) ^ 0x80808080u;
+      L[2][d] = __builtin_amdgcn_perm(t23h, t01h, 0x05040100u) ^ 0x80808080u;
+      L[3][d] = __builtin_amdgcn_perm(t23h, t01h, 0x07060302u) ^ 0x80808080u;
+      L[4][d] = __builtin_amdgcn_perm(u23, u01, 0x05040100u) ^ 0x80808080u;
+      L[5][d] = __builtin_amdgcn_perm(u23, u01, 0x07060302u) ^ 0x80808080u;
+    };
+    intx4 acc[kLimbNW];
+    // limb pairs (a, b), a + b >= kLimbW0, weight w = a + b - kLimbW0; the
+    // first pair of each weight (a == 0 or b == 5) starts from zero. The
+    // conversion of plane z runs in four pieces between groups of MFMAs
+    // (in order, by scheduling barriers), its VALU work under their 16-cycle
+    // matrix-pipe slots.
+    constexpr int NPAIR = 26 * RH;
+    int pi = 0, piece = 0;
+#pragma unroll
+    for (int hf = 0; hf < RH; ++hf) {
+      const int co = ((4 * hf + kq) ^ sw) << 4;
+      intx4 av[kLimbs], bv[kLimbs];
+#pragma unroll
+      for (int l = 0; l < kLimbs; ++l) {
+        bv[l] = *(const intx4*)(tB + l * TB + co);
+        av[l] = *(const intx4*)(tA + l * TB + co);
+      }
+#pragma unroll
+      for (int a = 0; a < kLimbs; ++a)
+#pragma unroll
+        for (int b = (kLimbW0 - a > 0 ? kLimbW0 - a : 0); b < kLimbs; ++b) {
+          const int w = a + b - kLimbW0;
+          const bool zero = hf == 0 && (a == 0 || b == kLimbs - 1);
+          acc[w] = __builtin_amdgcn_mfma_i32_16x16x64_i8(av[a], bv[b],
+                                                         zero ? intx4{0, 0, 0, 0} : acc[w], 0, 0, 0);
+          ++pi;
+          if (piece < 4 && pi * 4 >= (piece + 1) * NPAIR - NPAIR / 2) {
+            __builtin_amdgcn_sched_barrier(0);
+            convert(piece++);
+            __builtin_amdgcn_sched_barrier(0);
+          }
+        }
     }
+#pragma unroll
+    for (; piece < 4; ++piece) convert(piece);
+#pragma unroll
+    for (int a = 0; a < kLimbs; ++a)
+      asm volatile("" ::"v"(L[a][0]), "v"(L[a][1]), "v"(L[a][2]), "v"(L[a][3]));
+    if (z < z1 && sc_ok) {
+      // one 16-byte chunk per limb; eight consecutive frames hit eight distinct chunks
+      unsigned char* dst = s_t + (z & 1) * kLimbs * TB + st * R + ((sc ^ (st & cmask)) << 4);
+#pragma unroll
+      for (int a = 0; a < kLimbs; ++a)
+        *(intx4*)(dst + a * TB) = intx4{(int)L[a][0], (int)L[a][1], (int)L[a][2], (int)L[a][3]};
+    }
+    if (z + 2 < z1) fetch(cur, z + 2);  // in flight for two steps
     if (act) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
@@ -670,21 +683,21 @@ static int launch_gram_slide_t(const float* xc, double* gram, const Geom& g, int
   return launch_status("gram (sliding) kernel");
 }
 
-template <int NTB>
+template <int NTB, int RH>
 static int launch_gram_limb_t(const float* xc, double* gram, const Geom& g, int ix0, int nix,
                               hipStream_t st) {
-  constexpr int NTHR = 64 * NTB * (NTB + 1) / 2 * slide_cols(NTB);
+  constexpr int NTHR = 64 * NTB * (NTB + 1) / 2 * limb_cols(NTB);
   static std::atomic<long long> cache[kMaxDevices];
   const size_t lds = limb_lds(NTB, g.px * g.py);
-  int rc = prepare_lds((const void*)k_gram_limb<NTB>, lds);
+  int rc = prepare_lds((const void*)k_gram_limb<NTB, RH>, lds);
   if (rc != MPPCA_OK) return rc;
   const int ncol = nix * g.ncy;
   int nblk = 0, per_xcd = 0;
-  const int nseg = slide_runs(g, ncol, slide_cols(NTB),
-                              resident_slots((const void*)k_gram_limb<NTB>, NTHR, lds, cache),
+  const int nseg = slide_runs(g, ncol, limb_cols(NTB),
+                              resident_slots((const void*)k_gram_limb<NTB, RH>, NTHR, lds, cache),
                               &nblk, &per_xcd);
   if (nseg == 0) return set_err(MPPCA_EUNSUPPORTED, "gram (int8 sliding): too many columns");
-  hipLaunchKernelGGL((k_gram_limb<NTB>), dim3(8u * (unsigned)per_xcd), dim3(NTHR), lds, st, xc,
+  hipLaunchKernelGGL((k_gram_limb<NTB, RH>), dim3(8u * (unsigned)per_xcd), dim3(NTHR), lds, st, xc,
                      gram, g, ix0, ncol, nseg, per_xcd);
   return launch_status("gram (int8 sliding) kernel");
 }
@@ -715,11 +728,16 @@ int launch_gram64(const float* xc, double* gram, const Geom& g, long long b0, lo
   const bool cols = nb > 0 && b0 % per_ix == 0 && nb % per_ix == 0;
   if (cols && limb_fits(g)) {
     const int ix0 = (int)(b0 / per_ix), nix = (int)(nb / per_ix);
+    const bool two = limb_rows(g.px * g.py) > 64;  // tile rows of 128 voxels
     switch (ntb_of(g.T)) {
-      case 1: return launch_gram_limb_t<1>(xc, gram, g, ix0, nix, st);
-      case 2: return launch_gram_limb_t<2>(xc, gram, g, ix0, nix, st);
-      case 3: return launch_gram_limb_t<3>(xc, gram, g, ix0, nix, st);
-      default: return launch_gram_limb_t<4>(xc, gram, g, ix0, nix, st);
+      case 1: return launch_gram_limb_t<1, 1>(xc, gram, g, ix0, nix, st);
+      case 2: return launch_gram_limb_t<2, 1>(xc, gram, g, ix0, nix, st);
+      case 3:
+        return two ? launch_gram_limb_t<3, 2>(xc, gram, g, ix0, nix, st)
+                   : launch_gram_limb_t<3, 1>(xc, gram, g, ix0, nix, st);
+      default:
+        return two ? launch_gram_limb_t<4, 2>(xc, gram, g, ix0, nix, st)
+                   : launch_gram_limb_t<4, 1>(xc, gram, g, ix0, nix, st);
     }
   }
   if (cols && slide_fits(g)) {
