@@ -44,7 +44,9 @@ import __graft_entry__  # noqa: E402
 
 METRIC = ("ms/volume host->host (96x96x48x48, 11³, stride 3) + p50/p99 latency "
           "at fixed load")
-PEAK = {"hbm": (8000.0, "GB/s"), "fp32": (157.3, "TFLOP/s"), "fp64": (78.6, "TFLOP/s")}
+PEAK = {"hbm": (8000.0, "GB/s"), "fp32": (157.3, "TFLOP/s"), "fp64": (78.6, "TFLOP/s"),
+        "int8": (5000.0, "TOP/s")}
+LIMB_PAIRS = 26  # int8 limb pairs the sliding Gram forms per product (gram.hip)
 # kernel-symbol prefix of each stage, as rocprofv3 reports it
 # kernel-symbol prefixes of each stage, as rocprofv3 reports them (the FP64
 # eigensolver for T <= 48 is three kernels: tridiagonalisation, eigenvalues +
@@ -59,15 +61,27 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def gram_sliding(shape, patch, form):
-    """Whether the FP64 Gram runs as the sliding window over x-y columns: the
-    plain form at T <= 64 with a footprint of at most 16 staging passes of
-    2 (ceil(T/16) + 1) rows (the launcher's conditions in gram.hip)."""
+def gram_route(shape, patch, form):
+    """Which kernel forms the FP64 Gram (the launcher's conditions in gram.hip):
+    "limb" (sliding window on int8 limbs: T <= 64, each thread one 16-voxel
+    chunk of the footprint rounded up to 64, two tile sets within 160 KiB of
+    LDS), "slide" (sliding window on FP64 MFMA: at most 16 staging passes of
+    2 (ceil(T/16) + 1) rows, 96 KiB of LDS) or "patch" (per-patch sums)."""
     x, y, z, t = shape
     ntb = (t + 15) // 16
     f = patch[0] * patch[1]
-    return (form == "plain" and t <= 64 and -(-f // (2 * (ntb + 1))) <= 16
-            and patch[0] * y * z * t * 4 < 2 ** 31)
+    if form != "plain" or t > 64 or patch[0] * y * z * t * 4 >= 2 ** 31:
+        return "patch"
+    rows = -(-f // 64) * 64
+    cols = 1 if ntb >= 4 else 2 if ntb == 3 else 4
+    col_bytes = -(-(2 * 6 * 16 * ntb * rows + 4 * (rows + ntb * (ntb + 1))) // 16) * 16
+    if rows // 16 <= 2 * (ntb + 1) and cols * col_bytes <= 160 * 1024:
+        return "limb"
+    srows = -(-f // 32) * 32
+    stride = ntb * 16 if ntb % 2 else ntb * 16 + 16
+    if -(-f // (2 * (ntb + 1))) <= 16 and cols * (srows * stride + f) * 4 <= 96 * 1024:
+        return "slide"
+    return "patch"
 
 
 def stage_launches(stage, shape, form):
@@ -92,6 +106,7 @@ def stage_work(stage, shape, patch, stride, n_patches, ranks, form, route):
     gram      flops: symmetric rank-N update per patch (syrk)   N*T*(T+1) * P
                      sliding window: each covered plane of each
                      x-y column's footprint once                F*T*(T+1) * Zc * columns
+                     int8 limbs: that count once per limb pair  26 * F*T*(T+1) * Zc * columns
     eigh_mp   flops: tridiagonalisation + p back-transformed
                      vectors per patch (LAPACK counts)          sum_b 4/3*T^3 + 2*T^2*p_b
     recon     flops: factors: (Xc V_p) V_p^T per patch           sum_b 4*N*T*p_b
@@ -106,11 +121,13 @@ def stage_work(stage, shape, patch, stride, n_patches, ranks, form, route):
     if stage == "centre":
         return "hbm", 4.0 * (2 * nvox * t + nvox), "hbm"
     if stage == "gram":
-        if gram_sliding(shape, patch, form):
+        gr = gram_route(shape, patch, form)
+        if gr != "patch":
             ncz = len(set(range(0, z - patch[2] + 1, stride)) | {z - patch[2]})
             ncol = n_patches // ncz
             f = patch[0] * patch[1]
-            return "mfma", float(f) * t * (t + 1) * covered_planes(z, patch[2], stride) * ncol, fp
+            work = float(f) * t * (t + 1) * covered_planes(z, patch[2], stride) * ncol
+            return ("mfma", LIMB_PAIRS * work, "int8") if gr == "limb" else ("mfma", work, fp)
         return "mfma", float(n) * t * (t + 1) * n_patches, fp
     if stage == "eigh_mp":
         return "mfma", 4.0 / 3.0 * t ** 3 * n_patches + 2.0 * t * t * psum, fp
