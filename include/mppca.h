@@ -42,7 +42,9 @@ extern "C" {
 
 /* Arithmetic forms of the whole path (mppca_workspace_bytes,
  * mppca_denoise_device). Both read and write float32 volumes.
- *   PLAIN: Gram on FP64 MFMA (exact products of the float32 centred data),
+ *   PLAIN: FP64 Gram (int8 MFMA on 46-bit fixed-point limbs of each plane
+ *          for the sliding window over x-y columns, else FP64 MFMA on exact
+ *          products of the float32 centred data),
  *          FP64 eigensolver (Householder + bisection + twisted factorisation),
  *          FP32 MFMA reconstruction from the rank-p factors (T <= 64; above
  *          that from the projector). Matches step 0 to ~1e-7.
@@ -79,8 +81,12 @@ int mppca_gram(const float* xc, float* gram, int X, int Y, int Z, int T,
                int px, int py, int pz, int stride, int64_t b0, int64_t nb,
                void* stream);
 
-/* As mppca_gram with FP64 accumulation: every product of two float32 values
- * is exact in FP64 (v_mfma_f64_16x16x4f64), gram[b] is T x T doubles.
+/* As mppca_gram with FP64 accumulation; gram[b] is T x T doubles. A launch
+ * over whole x-corner columns (b0 and nb multiples of ncy * ncz) with
+ * T <= 64 slides a window along each x-y column: each plane's values are
+ * rounded to integers at 2^-46 of the plane's largest |x| and their products
+ * summed exactly in int8-limb MFMA (v_mfma_i32_16x16x64_i8) before the FP64
+ * prefix. Other launches sum exact FP64 products (v_mfma_f64_16x16x4f64).
  * Replaces gram_kernel (step8_tuned.py:32-56) in the PLAIN form. */
 int mppca_gram64(const float* xc, double* gram, int X, int Y, int Z, int T,
                  int px, int py, int pz, int stride, int64_t b0, int64_t nb,
