@@ -396,6 +396,43 @@ def test_gram64_exact(pkg, lib):
             assert np.all(np.abs(g.cpu().numpy() - G64[b0:b0 + nb]) <= bound[b0:b0 + nb])
 
 
+def test_gram64_nonfinite(pkg, lib):
+    """A NaN or an infinity in the volume makes the Gram of every patch that
+    holds it non-finite (the eigensolver flags such matrices). Patches of
+    other x-y columns, and windows of the same column that end before its
+    plane, keep their bounds."""
+    torch = _torch()
+    rng = np.random.default_rng(11)
+    for shape, patch, stride, bad in [((13, 13, 16, 48), (11, 11, 5), 2, np.nan),
+                                      ((10, 9, 12, 17), (5, 3, 4), 2, np.inf),
+                                      ((8, 8, 10, 64), (4, 3, 3), 3, np.nan)]:
+        print("gram64 nonfinite", shape, patch, stride, bad, flush=True)
+        xc = (rng.standard_normal(shape) * rng.uniform(0.5, 2.0)).astype(np.float32)
+        v = (shape[0] // 2, shape[1] // 2, shape[2] // 2)
+        xc[v + (shape[3] // 3,)] = bad
+        P, t = pkg.num_patches(shape, patch, stride), shape[3]
+        import oracle as o
+        corners = o.patch_corners(shape, patch, stride)
+        ok = xc.copy()
+        ok[v + (shape[3] // 3,)] = 0.0
+        X = _gather(ok, patch, stride).astype(np.float64)
+        G64 = np.einsum("bnt,bns->bts", X, X)
+        bound = (_slide_bound(ok, patch, stride) + _limb_bound(ok, patch, stride)
+                 + np.abs(G64) * 2.0 ** -52)
+        g = torch.full((P, t, t), 0.0, dtype=torch.float64, device="cuda")
+        pkg._lib.check(lib.mppca_gram64(_dev(xc).data_ptr(), g.data_ptr(), *shape, *patch, stride,
+                                        0, P, _stream()), "gram64")
+        _sync()
+        got = g.cpu().numpy()
+        px, py, pz = patch
+        for b, (i, j, k) in enumerate(corners):
+            inx = i <= v[0] < i + px and j <= v[1] < j + py
+            if inx and k <= v[2] < k + pz:
+                assert not np.isfinite(got[b]).all(), (b, (i, j, k))
+            elif not inx or k + pz <= v[2]:
+                assert np.all(np.abs(got[b] - G64[b]) <= bound[b]), (b, (i, j, k))
+
+
 def _eig64(pkg, lib, G, n_vox):
     torch = _torch()
     b, t, _ = G.shape
