@@ -355,8 +355,9 @@ __global__ __launch_bounds__(64 * NTB * (NTB + 1) / 2 * slide_cols(NTB)) __attri
 // every later window of the run as k_gram_slide's NaN products do.
 // Plane tiles in LDS: one per limb, a row of R = F rounded up to 64 bytes per
 // frame (zero past the footprint and past T); the 16-byte chunk c of frame f
-// sits at chunk c ^ (f mod R/16), so that the 16 frames of one MFMA operand
-// read fall on distinct banks.
+// sits at chunk c ^ (f mod R/16), so that eight consecutive frames of an MFMA
+// operand read or of a staging write touch eight distinct chunks (no bank
+// conflicts).
 // ---------------------------------------------------------------------------
 typedef int intx4 __attribute__((ext_vector_type(4)));
 constexpr int kLimbs = 6;                          // signed 8-bit limbs per value
