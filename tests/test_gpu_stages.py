@@ -374,7 +374,9 @@ def test_gram64_exact(pkg, lib):
     for shape, patch, stride in [((13, 13, 13, 48), (11, 11, 11), 2), ((9, 8, 8, 100), (4, 4, 4), 2),
                                  ((10, 9, 8, 17), (5, 3, 4), 2), ((8, 8, 8, 128), (5, 5, 5), 3),
                                  ((9, 7, 30, 33), (3, 4, 5), 3), ((8, 9, 23, 64), (4, 3, 3), 5),
-                                 ((6, 5, 40, 2), (2, 2, 7), 1)]:
+                                 ((6, 5, 40, 2), (2, 2, 7), 1),
+                                 # T = 24 with an 81-voxel footprint: the FP64 sliding kernel
+                                 ((12, 12, 10, 24), (9, 9, 3), 2)]:
         print("gram64", shape, patch, stride, flush=True)
         vol = (rng.random(shape[:3] + (3,)) @ rng.standard_normal((3, shape[3]))
                + 0.05 * rng.standard_normal(shape)).astype(np.float32)
