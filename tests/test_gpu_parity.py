@@ -67,6 +67,7 @@ def test_golden_volumes(pkg, golden):
     ((14, 13, 12, 17), (3, 5, 4), 2, 3, 0.05),      # odd T, anisotropic patch
     ((12, 12, 12, 64), (7, 7, 7), 2, 6, 0.03),
     ((11, 11, 11, 20), (11, 11, 11), 3, 4, 0.05),   # one patch = the whole volume
+    ((16, 16, 14, 56), (9, 9, 9), 3, 5, 0.04),      # T in 49..64, 81-voxel footprint
 ])
 def test_configs(pkg, shape, patch, stride, rank, noise):
     vol, _ = pkg.make_lowrank_volume(shape, rank, 2.0, noise, 11)
