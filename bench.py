@@ -65,7 +65,7 @@ def gram_route(shape, patch, form):
     """Which kernel forms the FP64 Gram (the launcher's conditions in gram.hip):
     "limb" (sliding window on int8 limbs: T <= 64, each thread one 16-voxel
     chunk of the footprint rounded up to 64, two tile sets within 160 KiB of
-    LDS), "slide" (sliding window on FP64 MFMA: at most 16 staging passes of
+    LDS; T in 17..32 may take the 48-frame kernel), "slide" (sliding window on FP64 MFMA: at most 16 staging passes of
     2 (ceil(T/16) + 1) rows, 96 KiB of LDS) or "patch" (per-patch sums)."""
     x, y, z, t = shape
     ntb = (t + 15) // 16
@@ -73,10 +73,16 @@ def gram_route(shape, patch, form):
     if form != "plain" or t > 64 or patch[0] * y * z * t * 4 >= 2 ** 31:
         return "patch"
     rows = -(-f // 64) * 64
-    cols = 1 if ntb >= 4 else 2 if ntb == 3 else 4
-    col_bytes = -(-(2 * 6 * 16 * ntb * rows + 4 * (rows + ntb * (ntb + 1))) // 16) * 16
-    if rows // 16 <= 2 * (ntb + 1) and cols * col_bytes <= 160 * 1024:
+
+    def limb_fits(nb):
+        cols = 1 if nb >= 4 else 2 if nb == 3 else 4
+        col_bytes = -(-(2 * 6 * 16 * nb * rows + 4 * (rows + nb * (nb + 1))) // 16) * 16
+        return rows // 16 <= 2 * (nb + 1) and cols * col_bytes <= 160 * 1024
+
+    # T in 17..32 with a footprint over 64 voxels runs the 48-frame kernel
+    if limb_fits(ntb) or (ntb == 2 and limb_fits(3)):
         return "limb"
+    cols = 1 if ntb >= 4 else 2 if ntb == 3 else 4
     srows = -(-f // 32) * 32
     stride = ntb * 16 if ntb % 2 else ntb * 16 + 16
     if -(-f // (2 * (ntb + 1))) <= 16 and cols * (srows * stride + f) * 4 <= 96 * 1024:

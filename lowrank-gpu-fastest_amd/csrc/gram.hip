@@ -714,12 +714,19 @@ static bool slide_fits(const Geom& g) {
   return (long long)g.px * g.Y * g.Z * g.T * 4 < (1LL << 31);
 }
 
-// The int8 kernel takes the sliding launches whose plane fits its staging
-// registers (one 16-voxel chunk of one frame per thread) and the LDS.
-static bool limb_fits(const Geom& g) {
-  if (g.T > 64 || (long long)g.px * g.Y * g.Z * g.T * 4 >= (1LL << 31)) return false;
-  const int ntb = ntb_of(g.T), F = g.px * g.py;
-  return limb_rows(F) / 16 <= slide_rows_per_pass(ntb) && limb_lds(ntb, F) <= 160 * 1024;
+// Frame blocks of the int8 kernel that takes a sliding launch (0: none): its
+// plane must fit the staging (one 16-voxel chunk of one frame per thread)
+// and the LDS. T in 17..32 with a footprint of 65..128 voxels runs the
+// 48-frame kernel, its frames past T zero (the blocks past T store nothing).
+static int limb_ntb(const Geom& g) {
+  if (g.T > 64 || (long long)g.px * g.Y * g.Z * g.T * 4 >= (1LL << 31)) return 0;
+  const int F = g.px * g.py;
+  auto fits = [&](int ntb) {
+    return limb_rows(F) / 16 <= slide_rows_per_pass(ntb) && limb_lds(ntb, F) <= 160 * 1024;
+  };
+  const int ntb = ntb_of(g.T);
+  if (fits(ntb)) return ntb;
+  return ntb == 2 && fits(3) ? 3 : 0;
 }
 
 int launch_gram64(const float* xc, double* gram, const Geom& g, long long b0, long long nb,
@@ -727,10 +734,11 @@ int launch_gram64(const float* xc, double* gram, const Geom& g, long long b0, lo
   // whole x-corner columns: a sliding-window kernel
   const long long per_ix = (long long)g.ncy * g.ncz;
   const bool cols = nb > 0 && b0 % per_ix == 0 && nb % per_ix == 0;
-  if (cols && limb_fits(g)) {
+  const int lntb = cols ? limb_ntb(g) : 0;
+  if (lntb > 0) {
     const int ix0 = (int)(b0 / per_ix), nix = (int)(nb / per_ix);
     const bool two = limb_rows(g.px * g.py) > 64;  // tile rows of 128 voxels
-    switch (ntb_of(g.T)) {
+    switch (lntb) {
       case 1: return launch_gram_limb_t<1, 1>(xc, gram, g, ix0, nix, st);
       case 2: return launch_gram_limb_t<2, 1>(xc, gram, g, ix0, nix, st);
       case 3:

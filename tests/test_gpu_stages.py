@@ -375,8 +375,10 @@ def test_gram64_exact(pkg, lib):
                                  ((10, 9, 8, 17), (5, 3, 4), 2), ((8, 8, 8, 128), (5, 5, 5), 3),
                                  ((9, 7, 30, 33), (3, 4, 5), 3), ((8, 9, 23, 64), (4, 3, 3), 5),
                                  ((6, 5, 40, 2), (2, 2, 7), 1),
-                                 # T = 24 with an 81-voxel footprint: the FP64 sliding kernel
-                                 ((12, 12, 10, 24), (9, 9, 3), 2)]:
+                                 # T = 24 with an 81-voxel footprint: the 48-frame int8 kernel
+                                 ((12, 12, 10, 24), (9, 9, 3), 2),
+                                 # T = 56 with a 156-voxel footprint: the FP64 sliding kernel
+                                 ((15, 14, 10, 56), (13, 12, 3), 2)]:
         print("gram64", shape, patch, stride, flush=True)
         vol = (rng.random(shape[:3] + (3,)) @ rng.standard_normal((3, shape[3]))
                + 0.05 * rng.standard_normal(shape)).astype(np.float32)
