@@ -2,13 +2,17 @@
 
 Workload: float32 volume 96x96x48x48 (seed 1, rank-6 signal, maps smoothed at
 sigma 4 voxels, noise N(0, 0.03^2)), 11^3 patches, stride 3 -> 12,600 patches.
-A step is one denoise of that volume with the input already resident in HBM
-(``value`` / ``ms_per_step``). The host->host time of the reference's harness
-(upload + denoise + download, common.py:129-143) and its p50/p99 over a fixed
-number of back-to-back calls are reported beside it, never as ``value``.
+A step is one host->host denoise of that volume (``value`` / ``ms_per_step``):
+a NumPy volume in pageable host memory goes in, the denoised NumPy volume
+comes out (upload, kernels, download), timed by wall clock as the reference's
+harness does (common.py:129-143). Beside it: best / p50 / p99 of the calls,
+the device-resident time (input already in HBM), per-kernel times and their
+roofline, a fixed-load serving run, the harness's checks against step 0
+(rel. error, RMSE, rank agreement; common.py:145-161) and the step-0 CPU
+baseline at 1 BLAS thread and on all cores.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
-                    [--no-traffic]
+                    [--no-traffic] [--no-validate]
 
 Per-stage kernel times come from HIP events on the launch stream: each stage is
 queued behind a spacer fill, so its start event fires as the stage's kernel
@@ -148,24 +152,51 @@ def stage_work(stage, shape, patch, stride, n_patches, ranks, form, route):
     raise KeyError(stage)
 
 
-def cpu_baseline(vol, patch, stride, budget_s=15.0):
-    """Step-0 oracle on this host, 1 BLAS thread, over a bounded patch sample."""
+def host_cpu():
+    """CPU model, usable cores and numpy's BLAS (the reference's CPU numbers
+    depend on all three: run_all.sh:8, micro/cpu_threads.py:1-19)."""
+    model = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:  # pragma: no cover
+        cores = os.cpu_count() or 1
+    blas = None
+    try:
+        from threadpoolctl import threadpool_info
+
+        for lib in threadpool_info():
+            if lib.get("user_api") == "blas":
+                blas = f"{lib.get('internal_api')} {lib.get('version')}"
+                break
+    except Exception:  # pragma: no cover
+        pass
+    return {"model": model, "cores": cores, "os_cpu_count": os.cpu_count(), "blas": blas}
+
+
+def cpu_baseline(vol, patch, stride, budget_s=10.0, threads=1):
+    """Step-0 oracle on this host with ``threads`` BLAS threads, over a bounded
+    sample of the volume's patches (random order), scaled to the volume."""
     import oracle
     from oracle.step0 import patch_corners
 
-    try:
-        from threadpoolctl import threadpool_limits
-    except ImportError:  # pragma: no cover
-        threadpool_limits = None
+    from threadpoolctl import threadpool_limits
+
     corners = patch_corners(vol.shape, patch, stride)
     order = np.random.default_rng(0).permutation(len(corners))
     px, py, pz = patch
     out = np.zeros_like(vol)
     wsum = np.zeros(vol.shape[:3] + (1,), dtype=vol.dtype)
-    ctx = threadpool_limits(1) if threadpool_limits else None
     done = 0
-    t0 = time.perf_counter()
-    try:
+    with threadpool_limits(threads, user_api="blas"):
+        t0 = time.perf_counter()
         for b in order:
             i, j, k = corners[b]
             sl = np.s_[i:i + px, j:j + py, k:k + pz]
@@ -176,22 +207,47 @@ def cpu_baseline(vol, patch, stride, budget_s=15.0):
             done += 1
             if done >= 64 and time.perf_counter() - t0 > budget_s:
                 break
-    finally:
-        if ctx is not None:
-            ctx.unpatch() if hasattr(ctx, "unpatch") else ctx.__exit__(None, None, None)
-    el = time.perf_counter() - t0
+        el = time.perf_counter() - t0
     per_patch = el / done
     return {
         "value": round(per_patch * len(corners) * 1e3, 1),
         "unit": "ms/volume",
-        "cores": 1,
+        "cores": threads,
         "kind": "port",
         "sample": (f"{done} of {len(corners)} patches (random order, seed 0) of the same "
-                   f"volume through the NumPy step-0 restatement, 1 BLAS thread, "
+                   f"volume through the NumPy step-0 restatement, {threads} BLAS thread(s), "
                    f"{el:.1f} s; scaled to the full volume; final division excluded"),
         "us_per_patch": round(per_patch * 1e6, 1),
-        "cpu": platform.processor() or platform.machine(),
     }
+
+
+def validate(vol, clean, res, info, patch, stride, workers):
+    """The reference harness's checks (common.py:145-161) on the host->host
+    result, outside the timed region: relative L2 error against step 0 (the
+    oracle, run over ``workers`` processes, which also times the whole volume
+    on all cores), RMSE of the noisy and denoised volumes against the clean
+    signal, and the per-patch rank agreement (mismatches are allowed only where
+    the GPU's MP margin is below the eigenvalue error bound; the error is then
+    also stated against step 0 with those patches' ranks forced)."""
+    import oracle
+
+    def rmse(x):
+        return float(np.sqrt(np.mean((x.astype(np.float64) - clean) ** 2)))
+
+    t0 = time.perf_counter()
+    ref, ranks, _ = oracle.denoise_parallel(vol, patch, stride, workers=workers)
+    oracle_s = time.perf_counter() - t0
+    rel = float(np.linalg.norm(res - ref) / np.linalg.norm(ref))
+    bad = np.flatnonzero(info["rank"] != ranks)
+    out = {"rel_err_vs_step0": rel, "rmse_noisy": rmse(vol), "rmse_denoised": rmse(res),
+           "n_patches": int(len(ranks)), "rank_mismatches": int(bad.size),
+           "rank_mismatches_outside_bound": int((info["margin"][bad] >= 1.0).sum())}
+    if bad.size:
+        ref_f, _, _ = oracle.denoise_parallel(vol, patch, stride, workers=workers,
+                                              force_ranks=info["rank"])
+        out["rel_err_vs_step0_same_ranks"] = float(np.linalg.norm(res - ref_f)
+                                                  / np.linalg.norm(ref_f))
+    return out, oracle_s
 
 
 # ---- HBM traffic: rocprofv3 --pmc child runs ---------------------------------
@@ -324,8 +380,9 @@ def main():
     ap.add_argument("--workload", default="default")
     ap.add_argument("--form", default="plain", choices=("plain", "fp32"))
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--no-traffic", action="store_true")
+    ap.add_argument("--no-validate", action="store_true")
     ap.add_argument("--latency-calls", type=int, default=30)
     ap.add_argument("--serve-rate", type=float, default=100.0,
                     help="Poisson arrival rate (volumes/s) of the fixed-load serving run")
@@ -353,27 +410,49 @@ def main():
 
     pkg = __graft_entry__.load_package()
     log(f"generating workload {args.workload}")
-    vol, _, patch, stride = pkg.workload_volume(args.workload)
+    vol, clean, patch, stride = pkg.workload_volume(args.workload)
     shape = vol.shape
     d = pkg.get_denoiser(shape, patch, stride, form=args.form)
     P = d.n_patches
+    K, W = args.steps, args.warmup
+    log(f"{shape} patch {patch} stride {stride}: {P} patches, route {d.route}; warmup {W}")
+
+    # ---- timed: host -> host denoise (upload, kernels, download), K steps ---
+    # (the reference harness's timing, common.py:129-143: wall clock around
+    # the whole call; the first call allocates the pinned staging buffers)
+    for _ in range(max(W, 1)):
+        res = d(vol)
+    torch.cuda.synchronize()
+    lat = []
+    t_all = time.perf_counter()
+    for _ in range(K):
+        t0 = time.perf_counter()
+        res = d(vol)
+        lat.append((time.perf_counter() - t0) * 1e3)
+    torch.cuda.synchronize()
+    h2h_ms = (time.perf_counter() - t_all) * 1e3 / K
+    info = d.info()
+    log(f"host->host: {h2h_ms:.3f} ms/volume over {K} calls")
+    for _ in range(max(0, args.latency_calls - K)):  # more calls for the tail
+        t0 = time.perf_counter()
+        d(vol)
+        lat.append((time.perf_counter() - t0) * 1e3)
+    lat = np.array(lat)
+    log(f"host->host: best {lat.min():.2f} p50 {np.percentile(lat, 50):.2f} "
+        f"p99 {np.percentile(lat, 99):.2f} ms")
+
+    # ---- device-resident denoise (input already in HBM), K steps -----------
     vol_d = torch.from_numpy(vol).cuda()
     out_d = torch.empty_like(vol_d)
-    K, W = args.steps, args.warmup
-
-    log(f"{shape} patch {patch} stride {stride}: {P} patches, route {d.route}; warmup {W}")
     for _ in range(W):
         d.run_device(vol_d, out_d)
     torch.cuda.synchronize()
-
-    # ---- timed: device-resident denoise, K steps ---------------------------
     t0 = time.perf_counter()
     for _ in range(K):
         d.run_device(vol_d, out_d)
     torch.cuda.synchronize()
     dev_ms = (time.perf_counter() - t0) * 1e3 / K
     log(f"device-resident: {dev_ms:.3f} ms/volume")
-    info = d.info()
 
     # ---- per-stage kernel durations, roofline of the dominant kernel --------
     kernel_ms = stage_times(d, vol_d, out_d, max(K, 5))
@@ -398,21 +477,20 @@ def main():
         ach = a / (ms * 1e-3) / (1e9 if u == "GB/s" else 1e12)
         kernels[n] = {"ms": round(ms, 4), "bound": b, "achieved": round(ach, 3), "unit": u,
                       "frac": round(ach / pv, 4)}
+        if n == "gram":
+            # the survey's algorithmic count, 2 N T^2 P (the dense per-patch FP32
+            # product the reference forms), against the FP32 MFMA peak: an
+            # effective rate, since the sliding window forms each plane's products
+            # once per column and the int8 limbs are not FP32 arithmetic
+            alg = 2.0 * float(np.prod(patch)) * shape[3] ** 2 * P
+            eff = alg / (ms * 1e-3) / 1e12
+            kernels[n]["effective_vs_dense_fp32"] = {"flops": alg, "achieved": round(eff, 3),
+                                                     "unit": "TFLOP/s", "peak": PEAK["fp32"][0],
+                                                     "frac": round(eff / PEAK["fp32"][0], 4)}
         if traffic and n in traffic:
             kernels[n]["hbm_bytes"] = round(traffic[n]["fetch_bytes"] + traffic[n]["write_bytes"])
     if traffic and dom in traffic:
         roofline["traffic"] = round(traffic[dom]["fetch_bytes"] + traffic[dom]["write_bytes"])
-
-    # ---- host -> host, the reference harness's timing ------------------------
-    res = d(vol)  # first call allocates the pinned staging buffers
-    lat = []
-    for _ in range(max(args.latency_calls, K)):
-        t0 = time.perf_counter()
-        res = d(vol)
-        lat.append((time.perf_counter() - t0) * 1e3)
-    lat = np.array(lat)
-    log(f"host->host: best {lat.min():.2f} p50 {np.percentile(lat, 50):.2f} "
-        f"p99 {np.percentile(lat, 99):.2f} ms")
 
     # ---- serving: Poisson arrivals of this volume at a fixed rate -----------
     srv_res = None
@@ -441,26 +519,32 @@ def main():
         log(f"serving at {args.serve_rate:.0f}/s: p50 {srv_res['p50_ms']:.2f} p99 "
             f"{srv_res['p99_ms']:.2f} ms; burst {burst_vps:.1f} volumes/s")
 
+    cpu = host_cpu()
+    workers = max(1, min(16, cpu["cores"]))
     line = {
         "metric": METRIC,
-        "value": round(dev_ms, 4),
+        "value": round(h2h_ms, 4),
         "unit": "ms/volume",
         "n_gpus": 1,
         "steps": K,
         "warmup": W,
-        "ms_per_step": round(dev_ms, 4),
+        "ms_per_step": round(h2h_ms, 4),
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32" if args.form == "fp32" else "f32+f64",
         "data": "synthetic (rank-6 + noise generator, seed 1)",
         "config": {"workload": "96x96x48x48 float32, 11^3 patches, stride 3, 12600 patches",
-                   "form": args.form, "route": d.route, "input": "HBM-resident"},
-        "volumes_per_s": round(1e3 / dev_ms, 2),
-        "host_to_host_ms": {"best": round(float(lat.min()), 3),
+                   "form": args.form, "route": d.route,
+                   "timed": "host->host: NumPy volume in pageable memory to NumPy result "
+                            "(upload, denoise, download, status check)"},
+        "volumes_per_s": round(1e3 / h2h_ms, 2),
+        "host_to_host_ms": {"mean": round(h2h_ms, 4), "best": round(float(lat.min()), 3),
                             "p50": round(float(np.percentile(lat, 50)), 3),
                             "p99": round(float(np.percentile(lat, 99)), 3),
                             "calls": int(len(lat))},
+        "device_resident_ms": round(dev_ms, 4),
+        "device_resident_volumes_per_s": round(1e3 / dev_ms, 2),
         "serving_at_fixed_load": srv_res,
         "kernels": kernels,
         "roofline": roofline,
@@ -471,10 +555,28 @@ def main():
         "ranks": {"min": int(info["rank"].min()), "max": int(info["rank"].max()),
                   "mean": round(float(info["rank"].mean()), 3),
                   "margin_below_bound": int((info["margin"] < 1).sum())},
+        "host_cpu": cpu,
     }
+    oracle_s = None
+    if not args.no_validate:
+        log(f"validation against step 0 ({workers} oracle processes)")
+        line["validation"], oracle_s = validate(vol, clean, res, info, patch, stride, workers)
+        log(f"rel err vs step 0 {line['validation']['rel_err_vs_step0']:.2e}, rank mismatches "
+            f"{line['validation']['rank_mismatches']}")
     if not args.no_cpu_baseline:
-        log(f"cpu baseline (step-0 oracle, 1 thread, ~{args.cpu_budget:.0f} s)")
-        line["cpu_baseline"] = cpu_baseline(vol, patch, stride, args.cpu_budget)
+        log(f"cpu baseline (step-0 oracle, 1 BLAS thread, ~{args.cpu_budget:.0f} s)")
+        cb = cpu_baseline(vol, patch, stride, args.cpu_budget, threads=1)
+        log(f"cpu baseline ({cpu['cores']} BLAS threads, ~{args.cpu_budget:.0f} s)")
+        cb_all = cpu_baseline(vol, patch, stride, args.cpu_budget, threads=cpu["cores"])
+        cb["all_cores_blas"] = {k: cb_all[k] for k in ("value", "cores", "us_per_patch", "sample")}
+        if oracle_s is not None:
+            cb["all_cores_processes"] = {
+                "value": round(oracle_s * 1e3, 1), "cores": workers,
+                "sample": (f"the whole volume through the step-0 restatement over {workers} "
+                           "worker processes of 1 BLAS thread each (the validation run, "
+                           "pool start-up included)")}
+        cb["cpu"] = cpu["model"]
+        line["cpu_baseline"] = cb
     del res
     print(json.dumps(line), flush=True)
 

@@ -18,13 +18,14 @@ from .phantom import (  # noqa: F401
     workload_volume,
     MIXED_GEOMETRIES,
 )
-from .denoise import FORMS, Denoiser, denoise, get_denoiser  # noqa: F401
+from .denoise import FORMS, Denoiser, SolverStatusError, denoise, get_denoiser  # noqa: F401
 from .serve import AdmissionError, Server, device_bytes, poisson_run  # noqa: F401
 
 __all__ = [
     "denoise",
     "Denoiser",
     "get_denoiser",
+    "SolverStatusError",
     "FORMS",
     "patch_corners",
     "axis_corners",

@@ -81,6 +81,10 @@ static int launch_recon_t(const float* xc, const float* proj, const float* weigh
                    const Geom& g, long long b0, long long nb, hipStream_t st) {
   const size_t lds =
       (size_t)NTB * 16 * NTB * 16 * sizeof(float) + (size_t)(g.N + kOffPad) * sizeof(int);
+  // NTB = 8 always needs more than 64 KiB (the 128 x 128 projector alone fills it),
+  // NTB = 7 once N > 3824
+  const int rc = prepare_lds((const void*)k_recon<NTB>, lds);
+  if (rc) return rc;
   hipLaunchKernelGGL((k_recon<NTB>), dim3((unsigned)nb), dim3(256), lds, st, xc, proj, weight, acc,
                      g, b0, nb);
   return launch_status("recon kernel");

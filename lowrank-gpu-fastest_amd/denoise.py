@@ -18,6 +18,7 @@ from __future__ import annotations
 
 import os
 import threading
+import warnings
 
 import numpy as np
 
@@ -29,6 +30,16 @@ from .geometry import check_geometry, num_patches
 # "fp32":  the reference's all-FP32 arithmetic (FP32 MFMA Gram, FP32 Jacobi).
 FORMS = ("plain", "fp32")
 FACTOR_MAX_T = 64
+
+
+class SolverStatusError(RuntimeError):
+    """Some patch's eigensolve failed: NaN / Inf in the patch (status -2) or
+    Jacobi sweeps that did not converge (status -1). ``status`` holds the
+    per-patch codes."""
+
+    def __init__(self, msg, status):
+        super().__init__(msg)
+        self.status = status
 
 
 class Denoiser:
@@ -188,8 +199,32 @@ class Denoiser:
         mark(None)
         return out_d
 
+    # -- per-patch status --------------------------------------------------
+    def check_status(self, on_bad="raise"):
+        """Check the per-patch solver status of the last call (a LAPACK-style
+        ``info``: -1 the Jacobi sweeps did not converge, -2 a non-finite patch).
+
+        ``on_bad``: "raise" (:class:`SolverStatusError`), "warn" or "ignore".
+        Waits for the device work of the last call. Step 0 raises on such
+        input too (``numpy.linalg.svd`` does not converge on NaN or Inf).
+        """
+        if on_bad == "ignore":
+            return
+        if on_bad not in ("raise", "warn"):
+            raise ValueError(f"on_bad must be 'raise', 'warn' or 'ignore', not {on_bad!r}")
+        worst = int(self.sweeps.min().item())
+        if worst >= 0:
+            return
+        st = self.sweeps.cpu().numpy()
+        nan, nconv = int((st == -2).sum()), int((st == -1).sum())
+        msg = (f"{nan} patch(es) hold NaN or Inf, {nconv} did not converge "
+               f"(of {self.n_patches}; first at patch {int(np.flatnonzero(st < 0)[0])})")
+        if on_bad == "raise":
+            raise SolverStatusError(msg, st)
+        warnings.warn(msg, RuntimeWarning, stacklevel=3)
+
     # -- host -> host -------------------------------------------------------
-    def __call__(self, vol) -> np.ndarray:
+    def __call__(self, vol, on_bad="raise") -> np.ndarray:
         torch = self.torch
         a = vol.numpy() if isinstance(vol, torch.Tensor) else np.asarray(vol)
         if a.dtype != np.float32:
@@ -213,6 +248,7 @@ class Denoiser:
             self.rank.data_ptr(), self.margin.data_ptr(), self.sweeps.data_ptr(),
         )
         _lib.check(rc, "mppca_denoise_host")
+        self.check_status(on_bad)
         return host.numpy()
 
     def info(self) -> dict:
@@ -242,14 +278,17 @@ def get_denoiser(shape, patch=(11, 11, 11), stride=3, device=None, form="plain")
     return d
 
 
-def denoise(vol, patch=(11, 11, 11), stride=3, *, form="plain", return_info=False):
+def denoise(vol, patch=(11, 11, 11), stride=3, *, form="plain", return_info=False,
+            on_bad="raise"):
     """MP-PCA denoise a float32 (X, Y, Z, T) volume on the GPU.
 
     NumPy in -> NumPy out (host -> host); a CUDA tensor in -> a new CUDA
     tensor out. ``form`` is one of :data:`FORMS`. ``return_info`` adds a dict
     with the per-patch rank, MP margin and solver status. Raises on invalid
     input; never falls back to a CPU path. A patch holding NaN or Inf gets
-    rank 0 and status -2; the NaN reaches the output, as in any float path.
+    rank 0 and status -2 (the NaN reaches the output, as in any float path);
+    the per-patch status is checked after the call and ``on_bad`` ("raise",
+    the default, "warn" or "ignore") says what a negative status does.
     """
     try:
         import torch
@@ -257,13 +296,15 @@ def denoise(vol, patch=(11, 11, 11), stride=3, *, form="plain", return_info=Fals
         raise ImportError("PyTorch (ROCm) is required by the HIP denoiser") from e
     if isinstance(vol, torch.Tensor) and vol.is_cuda:
         d = get_denoiser(tuple(vol.shape), patch, stride, device=vol.device, form=form)
-        out = torch.empty_like(vol)
+        # a fresh contiguous output whatever the input's strides
+        out = torch.empty(vol.shape, dtype=vol.dtype, device=vol.device)
         d.run_device(vol.contiguous(), out)
+        d.check_status(on_bad)
         res = out
     else:
         shape = tuple(vol.shape)
         d = get_denoiser(shape, patch, stride, form=form)
-        res = d(vol)
+        res = d(vol, on_bad=on_bad)
     if return_info:
         return res, d.info()
     return res

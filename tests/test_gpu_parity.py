@@ -75,6 +75,25 @@ def test_configs(pkg, shape, patch, stride, rank, noise):
     _parity(out, info, vol, patch, stride)
 
 
+@pytest.mark.parametrize("shape,patch,stride,rank,noise", [
+    # the projector route (T > 64): per-patch FP64 Gram, FP64 eigensolver in LDS,
+    # atomic scatter of w * Xc P_b, gathered normalisation
+    ((17, 17, 17, 96), (11, 11, 11), 3, 8, 0.03),     # the T = 96 variant's patch
+    ((15, 15, 15, 100), (9, 9, 9), 3, 6, 0.05),       # mixed shape c: 9^3, stride 3
+    ((12, 12, 12, 128), (5, 5, 5), 2, 6, 0.04),       # mixed shape f: N = 125 < T - 1
+    ((13, 12, 11, 65), (5, 4, 6), 2, 5, 0.05),        # first T past the factor route
+    # the other mixed geometries at reduced X, Y, Z (their patch, stride and T)
+    ((16, 16, 14, 30), (7, 7, 7), 2, 5, 0.06),        # shape a: 7^3 / 2, T = 30
+    ((14, 14, 12, 64), (7, 7, 7), 2, 6, 0.04),        # shape d: 7^3 / 2, T = 64
+    ((23, 23, 19, 24), (11, 11, 11), 4, 4, 0.07),     # shape e: 11^3 / 4, T = 24
+])
+def test_mixed_geometries(pkg, shape, patch, stride, rank, noise):
+    """Every patch / stride / T of the mixed-shape workload against step 0."""
+    vol, _ = pkg.make_lowrank_volume(shape, rank, 2.0, noise, 12)
+    out, info = pkg.denoise(vol, patch, stride, return_info=True)
+    _parity(out, info, vol, patch, stride)
+
+
 def test_reference_phantom_baseline_1000(pkg):
     """The reference's own phantom: ~1000 baseline, sigma 4 (common.py:47-77)."""
     vol, _ = pkg.make_phantom((24, 22, 20, 24), rank=5, seed=0)
@@ -85,6 +104,14 @@ def test_reference_phantom_baseline_1000(pkg):
 def test_default_workload_full_size(pkg):
     """96x96x48x48, 11^3 patches, stride 3 (12,600 patches) against step 0."""
     vol, _, patch, stride = pkg.workload_volume("default")
+    out, info = pkg.denoise(vol, patch, stride, return_info=True)
+    _parity(out, info, vol, patch, stride, parallel=True)
+
+
+def test_t96_workload_full_size(pkg):
+    """96x96x48x96, 11^3 patches, stride 3 (the T = 96 variant, 12,600 patches,
+    projector route) against step 0."""
+    vol, _, patch, stride = pkg.workload_volume("t96")
     out, info = pkg.denoise(vol, patch, stride, return_info=True)
     _parity(out, info, vol, patch, stride, parallel=True)
 
@@ -157,3 +184,36 @@ def test_bad_input_raises(pkg):
         pkg.denoise(np.zeros((8, 8, 8, 6), dtype=np.float32), (9, 3, 3), 2)
     with pytest.raises(ValueError):
         pkg.denoise(np.zeros((8, 8, 8, 200), dtype=np.float32), (3, 3, 3), 2)
+
+
+def test_nonfinite_input_status(pkg):
+    """A NaN in the volume flags its patches (status -2): the host checks the
+    status after the call and raises by default (step 0's SVD raises on such
+    input), warns or passes on request; the device-tensor path checks too."""
+    import torch
+
+    vol, _ = pkg.make_lowrank_volume((14, 13, 12, 16), 3, 2.0, 0.05, 4)
+    vol[6, 6, 6, 5] = np.nan
+    with pytest.raises(pkg.SolverStatusError) as e:
+        pkg.denoise(vol, (5, 5, 5), 2)
+    assert (e.value.status == -2).any() and (e.value.status >= -2).all()
+    with pytest.warns(RuntimeWarning):
+        out = pkg.denoise(vol, (5, 5, 5), 2, on_bad="warn")
+    assert np.isnan(out[6, 6, 6]).all()
+    out, info = pkg.denoise(vol, (5, 5, 5), 2, on_bad="ignore", return_info=True)
+    assert np.isfinite(out[0, 0, 0]).all() and (info["sweeps"] == -2).any()
+    with pytest.raises(pkg.SolverStatusError):
+        pkg.denoise(torch.from_numpy(vol).cuda(), (5, 5, 5), 2)
+
+
+def test_device_path_permuted_input(pkg):
+    """A non-contiguous CUDA input gets a contiguous output of its shape."""
+    import torch
+
+    vol, _ = pkg.make_lowrank_volume((12, 12, 10, 12), 3, 2.0, 0.05, 6)
+    t = torch.from_numpy(np.ascontiguousarray(vol.transpose(1, 0, 2, 3))).cuda().permute(1, 0, 2, 3)
+    assert not t.is_contiguous()
+    dev = pkg.denoise(t, (5, 5, 5), 2)
+    assert dev.is_contiguous() and tuple(dev.shape) == vol.shape
+    host = pkg.denoise(vol, (5, 5, 5), 2)
+    assert np.linalg.norm(dev.cpu().numpy() - host) / np.linalg.norm(host) <= 1e-6

@@ -202,10 +202,16 @@ def test_mp_threshold_bit_exact(pkg, lib, n_vox, t):
                                   (1.0 / (2.0 + want.astype(np.float64))).astype(np.float32))
 
 
-def test_recon_and_normalize_vs_fp64(pkg, lib):
+@pytest.mark.parametrize("shape,patch,stride", [
+    ((13, 12, 11, 24), (5, 4, 6), 2),     # NTB = 2
+    ((13, 12, 11, 80), (5, 4, 6), 2),     # NTB = 5
+    ((13, 12, 11, 100), (5, 4, 6), 2),    # NTB = 7
+    ((13, 12, 11, 128), (5, 4, 6), 2),    # NTB = 8: over 64 KiB of LDS
+    ((17, 16, 16, 100), (16, 16, 15), 1),  # NTB = 7 with N = 3840: over 64 KiB
+])
+def test_recon_and_normalize_vs_fp64(pkg, lib, shape, patch, stride):
     torch = _torch()
     rng = np.random.default_rng(4)
-    shape, patch, stride = (13, 12, 11, 24), (5, 4, 6), 2
     t = shape[3]
     vol = rng.standard_normal(shape).astype(np.float32)
     mean = vol.mean(-1, dtype=np.float64).astype(np.float32)
@@ -236,6 +242,8 @@ def test_recon_and_normalize_vs_fp64(pkg, lib):
     ref_out = mean[..., None] + ref_acc / wsum[..., None]
     np.testing.assert_allclose(out.cpu().numpy(), ref_out, rtol=0,
                                atol=4 * EPS32 * (np.abs(ref_out).max() + scale))
+    if t > 64:
+        return
     # fused factor form on P_b = q q^T: same result, no zeroed accumulator needed
     fused = torch.full_like(acc, float("nan"))
     fac = _dev(_factor_blocks([q[b].astype(np.float32) for b in range(P)], wts, t))
