@@ -251,6 +251,32 @@ int mppca_denoise_host(const float* vol, float* out, int X, int Y, int Z, int T,
                        void* s_compute, void* s_in, void* s_out,
                        int32_t* rank, float* margin, int32_t* info);
 
+/* ---------------------------------------------------------------------------
+ * Whole path without any allocation
+ * ------------------------------------------------------------------------- */
+
+/* Bytes of the static device scratch that mppca_denoise_device_noalloc uses
+ * as its workspace (64 MiB). */
+#define MPPCA_NOALLOC_SCRATCH_BYTES (64ll << 20)
+int64_t mppca_noalloc_scratch_bytes(void);
+
+/* mppca_denoise_device with the workspace taken from a static device array of
+ * MPPCA_NOALLOC_SCRATCH_BYTES bytes inside the library instead of a caller
+ * buffer: no hipMalloc, hipFree or other allocation on any call. The
+ * reference has no counterpart; this entry point is for callers that cannot
+ * allocate device memory (code under stream capture, tight serving loops,
+ * harnesses that forbid hipMalloc). Same arguments, results and status codes
+ * as mppca_denoise_device without (workspace, workspace_bytes); a geometry
+ * whose workspace (mppca_workspace_bytes) exceeds the scratch returns
+ * MPPCA_ENOSPACE before anything is enqueued. Each GPU holds its own scratch
+ * from the first call on it until the process ends. Every call on one GPU
+ * shares that GPU's scratch: calls that overlap on one GPU (on different
+ * streams or host threads without ordering) race; calls ordered on one stream
+ * are safe, each one writing the parts of the scratch it reads. */
+int mppca_denoise_device_noalloc(const float* vol, float* out, int X, int Y, int Z, int T,
+                                 int px, int py, int pz, int stride, int form, int32_t* rank,
+                                 float* margin, int32_t* info, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

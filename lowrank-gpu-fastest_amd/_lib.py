@@ -12,7 +12,7 @@ import pathlib
 
 PKG = pathlib.Path(__file__).resolve().parent
 LIB_PATH = PKG / "libmppca.so"
-ABI_VERSION = 102
+ABI_VERSION = 103
 FORM_CODES = {"plain": 0, "fp32": 1}
 
 _i32, _i64, _vp, _f = ctypes.c_int, ctypes.c_int64, ctypes.c_void_p, ctypes.c_float
@@ -39,6 +39,8 @@ SIGNATURES = {
     "mppca_denoise_device": (_i32, [_vp, _vp] + [_i32] * 9 + [_vp, _i64, _vp, _vp, _vp, _vp]),
     "mppca_denoise_host": (_i32, [_vp, _vp] + [_i32] * 9 + [_vp, _i64] + [_vp] * 4
                            + [_i32, _i32] + [_vp] * 6),
+    "mppca_noalloc_scratch_bytes": (_i64, []),
+    "mppca_denoise_device_noalloc": (_i32, [_vp, _vp] + [_i32] * 9 + [_vp] * 4),
 }
 
 

@@ -126,3 +126,17 @@ def test_check_raises(pkg, lib):
     lib.mppca_num_patches(5, 5, 5, 11, 11, 11, 3)
     with pytest.raises(pkg._lib.MppcaError):
         pkg._lib.check(-1, "probe")
+
+
+def test_noalloc_limits_on_host(lib):
+    """The no-allocation entry point states its scratch size and rejects a
+    geometry whose workspace exceeds it before any GPU work (dummy pointers)."""
+    assert lib.mppca_noalloc_scratch_bytes() == 64 << 20
+    big = lib.mppca_workspace_bytes(96, 96, 48, 48, 11, 11, 11, 3, PLAIN)
+    assert big > lib.mppca_noalloc_scratch_bytes()
+    rc = lib.mppca_denoise_device_noalloc(16, 32, 96, 96, 48, 48, 11, 11, 11, 3, PLAIN,
+                                          None, None, None, None)
+    assert rc == -3  # MPPCA_ENOSPACE
+    assert b"static scratch" in lib.mppca_last_error()
+    assert lib.mppca_denoise_device_noalloc(16, 16, 8, 8, 8, 6, 3, 3, 3, 2, PLAIN,
+                                            None, None, None, None) == -1  # vol == out
