@@ -16,7 +16,8 @@ import sys
 
 PKG = pathlib.Path(__file__).resolve().parent
 ROOT = PKG.parent
-SRCS = sorted((PKG / "csrc").glob("*.hip"))
+# HIP sources, plus host-only C++ (compiled by hipcc as plain C++)
+SRCS = sorted((PKG / "csrc").glob("*.hip")) + sorted((PKG / "csrc").glob("*.cpp"))
 OUT = PKG / "libmppca.so"
 ARCH = "gfx950"
 
@@ -46,8 +47,11 @@ def hipcc() -> str:
 
 def build(verbose: bool = False, extra_flags=()) -> pathlib.Path:
     tmp = OUT.with_suffix(".so.tmp")
+    rocm = pathlib.Path(hipcc()).resolve().parent.parent
+    # the host pipeline's DMA downloads call the HSA runtime (linked by SONAME,
+    # libhsa-runtime64.so.1, so the process keeps the one torch's HIP runtime loaded)
     cmd = [hipcc(), *HIPCC_FLAGS, *extra_flags, f"-I{ROOT / 'include'}",
-           *[str(s) for s in SRCS], "-o", str(tmp)]
+           *[str(s) for s in SRCS], f"-L{rocm / 'lib'}", "-lhsa-runtime64", "-o", str(tmp)]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)

@@ -247,4 +247,8 @@ int launch_recon_factors(const float* xc, const float* fac, const int* rank, con
                          hipStream_t st);
 int recon_factor_tile_x();
 
+// Host copy of n bytes into pinned staging (host_copy.cpp): non-temporal
+// stores where the CPU allows, memcpy otherwise.
+void stage_copy(char* dst, const char* src, size_t n);
+
 }  // namespace mppca

@@ -9,16 +9,33 @@
 // increase with the x index, and patch indices are x-major, so they form one
 // contiguous index range) get their Gram matrices and eigensolves. The output
 // planes that no unfinished patch touches (x below the first unfinished
-// corner) are reconstructed and downloaded on the output stream while later
-// slabs upload and compute. Every stage runs the device-resident path's
-// kernels; a slab's sliding-window Gram launch has few columns and so splits
-// their z walks into runs (the prefix restarts at a run's first window), which
-// leaves the result equal to mppca_denoise_device's up to FP64 rounding of the
-// Gram, with the same ranks. Other routes run the whole volume at once.
+// corner) are then reconstructed, and a downloader thread copies them to the
+// host on a DMA engine (SDMA, through the HSA runtime) as soon as their
+// reconstruction has finished, while later slabs upload and compute.
+//
+// Downloads go through the DMA engines rather than hipMemcpyAsync because the
+// HIP runtime may carry out a device-to-host copy as a copy kernel on the
+// compute units (measured: a 256-workgroup kernel per download, resident for
+// the whole PCIe transfer), which then competes with the slab kernels for the
+// CUs. Uploads already run on the DMA engines through hipMemcpyAsync. When the
+// HSA runtime cannot place a buffer (host memory it does not know, such as
+// pageable memory), downloads fall back to hipMemcpyAsync on the output stream.
+//
+// Every stage runs the device-resident path's kernels; a slab's sliding-window
+// Gram launch has few columns and so splits their z walks into runs (the prefix
+// restarts at a run's first window), which leaves the result equal to
+// mppca_denoise_device's up to FP64 rounding of the Gram, with the same ranks.
+// Other routes run the whole volume at once.
+#include <hsa/hsa.h>
+#include <hsa/hsa_ext_amd.h>
 #include <string.h>
 
+#include <algorithm>
 #include <atomic>
+#include <functional>
 #include <memory>
+#include <mutex>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -28,15 +45,15 @@ namespace mppca {
 namespace {
 
 // Byte ranges copied by `nthreads` workers, range after range; range i is
-// complete when done[i] reaches nthreads. A worker may first wait for an
-// event (the download of an output range) before copying its share.
+// complete when done[i] reaches nthreads. A worker may first wait (for the
+// download of an output range) before copying its share.
 class RangeCopier {
  public:
   struct Range {
     char* dst;
     const char* src;
     size_t bytes;
-    hipEvent_t after;  // null: no wait
+    std::function<void()> after;  // empty: no wait
   };
   RangeCopier(std::vector<Range> ranges, int nthreads)
       : ranges_(std::move(ranges)), done_(new std::atomic<int>[ranges_.size()]), n_(nthreads) {
@@ -45,11 +62,11 @@ class RangeCopier {
       workers_.emplace_back([this, w] {
         for (size_t i = 0; i < ranges_.size(); ++i) {
           const Range& r = ranges_[i];
-          if (r.after) (void)hipEventSynchronize(r.after);
+          if (r.after) r.after();
           const size_t per = (r.bytes / n_ + 63) & ~(size_t)63;
           const size_t a = (size_t)w * per < r.bytes ? (size_t)w * per : r.bytes;
           const size_t b = a + per < r.bytes && w + 1 < n_ ? a + per : r.bytes;
-          if (b > a) memcpy(r.dst + a, r.src + a, b - a);
+          if (b > a) stage_copy(r.dst + a, r.src + a, b - a);
           done_[i].fetch_add(1, std::memory_order_release);
         }
       });
@@ -70,10 +87,10 @@ class RangeCopier {
 
 struct Events {
   std::vector<hipEvent_t> ev;
-  int create(size_t n) {
+  int create(size_t n, unsigned flags = hipEventDisableTiming) {
     ev.assign(n, nullptr);
     for (auto& e : ev) {
-      const hipError_t r = hipEventCreateWithFlags(&e, hipEventDisableTiming);
+      const hipError_t r = hipEventCreateWithFlags(&e, flags);
       if (r != hipSuccess) return set_err((int)r, "event: %s", hipGetErrorString(r));
     }
     return MPPCA_OK;
@@ -87,6 +104,151 @@ struct Events {
 int hip_ok(hipError_t r, const char* what) {
   return r == hipSuccess ? MPPCA_OK : set_err((int)r, "%s: %s", what, hipGetErrorString(r));
 }
+
+// ---- DMA downloads through the HSA runtime ----------------------------------------
+
+hsa_status_t first_cpu_agent(hsa_agent_t a, void* data) {
+  hsa_device_type_t t;
+  if (hsa_agent_get_info(a, HSA_AGENT_INFO_DEVICE, &t) == HSA_STATUS_SUCCESS &&
+      t == HSA_DEVICE_TYPE_CPU) {
+    *(hsa_agent_t*)data = a;
+    return HSA_STATUS_INFO_BREAK;
+  }
+  return HSA_STATUS_SUCCESS;
+}
+
+bool pointer_known(const void* p, hsa_agent_t* owner) {
+  hsa_amd_pointer_info_t info;
+  memset(&info, 0, sizeof info);
+  info.size = sizeof info;
+  if (hsa_amd_pointer_info(p, &info, nullptr, nullptr, nullptr) != HSA_STATUS_SUCCESS) return false;
+  if (info.type != HSA_EXT_POINTER_TYPE_HSA && info.type != HSA_EXT_POINTER_TYPE_LOCKED &&
+      info.type != HSA_EXT_POINTER_TYPE_HSA_VMEM)
+    return false;
+  if (owner) *owner = info.agentOwner;
+  return true;
+}
+
+// The agents of a device -> host copy from dev (device memory) into host
+// (pinned host memory), or false when the HSA runtime does not know either.
+bool dma_agents(const void* dev, const void* host, hsa_agent_t* gpu, hsa_agent_t* cpu) {
+  static std::once_flag once;
+  static bool init_ok = false;
+  static hsa_agent_t cpu_agent{0};
+  std::call_once(once, [] {
+    // the HIP runtime has initialised HSA already; this takes one more reference
+    init_ok = hsa_init() == HSA_STATUS_SUCCESS;
+    if (init_ok) (void)hsa_iterate_agents(first_cpu_agent, &cpu_agent);
+  });
+  if (!init_ok || cpu_agent.handle == 0) return false;
+  hsa_agent_t owner{0};
+  hsa_device_type_t t;
+  if (!pointer_known(dev, &owner) || owner.handle == 0 ||
+      hsa_agent_get_info(owner, HSA_AGENT_INFO_DEVICE, &t) != HSA_STATUS_SUCCESS ||
+      t != HSA_DEVICE_TYPE_GPU)
+    return false;
+  if (!pointer_known(host, nullptr)) return false;
+  *gpu = owner;
+  *cpu = cpu_agent;
+  return true;
+}
+
+// Downloads posted by the enqueuing thread, in order: job j waits for its
+// reconstruction event on the host, then starts one DMA copy whose completion
+// decrements signal j.
+class DmaDownloader {
+ public:
+  struct Job {
+    char* dst;
+    const char* src;
+    size_t bytes;
+    hipEvent_t after;
+  };
+  DmaDownloader(hsa_agent_t gpu, hsa_agent_t cpu, int n)
+      : gpu_(gpu), cpu_(cpu), jobs_(n), sig_(n, hsa_signal_t{0}) {}
+  int start() {
+    for (auto& s : sig_)
+      if (hsa_signal_create(1, 0, nullptr, &s) != HSA_STATUS_SUCCESS) {
+        s.handle = 0;
+        return set_err(MPPCA_EINVAL, "download: cannot create an HSA signal");
+      }
+    thread_ = std::thread([this] { run(); });
+    return MPPCA_OK;
+  }
+  void post(int j, const Job& job) {
+    jobs_[j] = job;
+    posted_.store(j + 1, std::memory_order_release);
+  }
+  // Waits for download j (posted); false if it failed or never started.
+  bool wait(int j) {
+    while (issued_.load(std::memory_order_acquire) <= j) {
+      if (stopped_.load(std::memory_order_acquire)) return false;
+      std::this_thread::yield();
+    }
+    hsa_signal_value_t v;
+    while ((v = hsa_signal_wait_scacquire(sig_[j], HSA_SIGNAL_CONDITION_LT, 1, UINT64_MAX,
+                                          HSA_WAIT_STATE_ACTIVE)) > 0) {
+    }
+    return v == 0;
+  }
+  // Stops after the posted jobs, waits for every started copy; status of all.
+  int finish() {
+    closed_.store(true, std::memory_order_release);
+    if (thread_.joinable()) thread_.join();
+    const int n = issued_.load(std::memory_order_acquire);
+    bool ok = true;
+    for (int j = 0; j < n; ++j) ok = wait(j) && ok;
+    for (auto& s : sig_)
+      if (s.handle) (void)hsa_signal_destroy(s);
+    sig_.clear();
+    if (!err_.empty()) return set_err(err_code_, "%s", err_.c_str());
+    if (!ok) return set_err(MPPCA_EINVAL, "download: a DMA copy reported an error");
+    return MPPCA_OK;
+  }
+  ~DmaDownloader() {
+    if (!sig_.empty()) (void)finish();
+  }
+
+ private:
+  void run() {
+    for (int j = 0; j < (int)jobs_.size(); ++j) {
+      while (posted_.load(std::memory_order_acquire) <= j) {
+        if (closed_.load(std::memory_order_acquire) &&
+            posted_.load(std::memory_order_acquire) <= j) {
+          stopped_.store(true, std::memory_order_release);
+          return;
+        }
+        std::this_thread::yield();
+      }
+      const Job& job = jobs_[j];
+      const hipError_t e = hipEventSynchronize(job.after);
+      if (e != hipSuccess) {
+        err_code_ = (int)e;
+        err_ = std::string("download: waiting for the reconstruction: ") + hipGetErrorString(e);
+        break;
+      }
+      if (job.bytes == 0) {
+        hsa_signal_store_screlease(sig_[j], 0);
+      } else if (hsa_amd_memory_async_copy(job.dst, cpu_, job.src, gpu_, job.bytes, 0, nullptr,
+                                           sig_[j]) != HSA_STATUS_SUCCESS) {
+        err_code_ = MPPCA_EINVAL;
+        err_ = "download: hsa_amd_memory_async_copy refused the copy";
+        break;
+      }
+      issued_.store(j + 1, std::memory_order_release);
+    }
+    stopped_.store(true, std::memory_order_release);
+  }
+
+  hsa_agent_t gpu_, cpu_;
+  std::vector<Job> jobs_;
+  std::vector<hsa_signal_t> sig_;
+  std::atomic<int> posted_{0}, issued_{0};
+  std::atomic<bool> closed_{false}, stopped_{false};
+  std::thread thread_;
+  std::string err_;
+  int err_code_ = 0;
+};
 
 }  // namespace
 }  // namespace mppca
@@ -118,113 +280,205 @@ extern "C" int mppca_denoise_host(const float* vol, float* out, int X, int Y, in
   const size_t plane = (size_t)Y * Z * T;  // floats per x-plane
   const bool pipelined = form == MPPCA_FORM_PLAIN && w.fac != nullptr;
   const int K = pipelined ? (nslabs < X ? nslabs : X) : 1;
+  // Slab boundaries. The last slab is narrower (about a third of the others):
+  // everything after the last upload -- its Gram and eigensolves, the
+  // reconstruction of the planes its patches reach and their download -- is
+  // the pipeline's tail.
   std::vector<int> xa(K + 1);
-  for (int i = 0; i <= K; ++i) xa[i] = (int)((long long)i * X / K);
+  {
+    const int wl = K >= 3 ? std::max(1, (int)((X + (3 * K - 2) / 2) / (3 * K - 2))) : 0;
+    const int K0 = wl > 0 ? K - 1 : K, X0 = X - wl;
+    for (int i = 0; i <= K0; ++i) xa[i] = (int)((long long)i * X0 / K0);
+    xa[K] = X;
+  }
 
-  // host staging of the input, slab by slab, ahead of the uploads
+  // host staging of the input, ahead of the uploads, in pieces of a few
+  // planes: a piece is uploaded as soon as it is staged, so the first upload
+  // starts after one piece rather than one slab
+  struct Piece {
+    int x0, x1, slab;
+  };
+  std::vector<Piece> pieces;
+  const size_t piece_bytes = (size_t)4 << 20;  // about 4 MiB per upload
+  const int piece_planes =
+      (int)(plane * sizeof(float) >= piece_bytes ? 1 : piece_bytes / (plane * sizeof(float)));
+  for (int i = 0; i < K; ++i)
+    for (int x = xa[i]; x < xa[i + 1]; x += piece_planes)
+      pieces.push_back({x, x + piece_planes < xa[i + 1] ? x + piece_planes : xa[i + 1], i});
   const char* src = (const char*)vol;
   std::unique_ptr<RangeCopier> in_copy;
   if (h_stage_in) {
     std::vector<RangeCopier::Range> r;
-    for (int i = 0; i < K; ++i)
-      r.push_back({(char*)(h_stage_in + xa[i] * plane), (const char*)(vol + xa[i] * plane),
-                   (size_t)(xa[i + 1] - xa[i]) * plane * sizeof(float), nullptr});
+    for (const Piece& q : pieces)
+      r.push_back({(char*)(h_stage_in + q.x0 * plane), (const char*)(vol + q.x0 * plane),
+                   (size_t)(q.x1 - q.x0) * plane * sizeof(float), {}});
     in_copy.reset(new RangeCopier(std::move(r), nthreads));
     src = (const char*)h_stage_in;
   }
-  Events ev_in, ev_rec, ev_out;
-  if ((rc = ev_in.create(K)) || (rc = ev_rec.create(K)) || (rc = ev_out.create(K))) return rc;
+  // Downloads: one per slab, the last slab's reconstruction split into up to
+  // kTailPieces downloads so that the tail's copies start before its whole
+  // reconstruction has run. Reconstruction events release to system scope:
+  // the DMA engines read d_out from memory, outside the GPU's caches.
+  constexpr int kTailPieces = 4;
+  const int NJ = K + kTailPieces;
+  Events ev_in, ev_eig, ev_rec, ev_out;
+  if ((rc = ev_in.create(K)) || (rc = ev_eig.create(K)) ||
+      (rc = ev_rec.create(NJ, hipEventDisableTiming | hipEventReleaseToSystem)) ||
+      (rc = ev_out.create(NJ)))
+    return rc;
   float* dst_out = h_stage_out ? h_stage_out : out;
+  hsa_agent_t gpu_agent{0}, cpu_agent{0};
+  std::unique_ptr<DmaDownloader> dma;
+  if (dma_agents(d_out, dst_out, &gpu_agent, &cpu_agent)) {
+    dma.reset(new DmaDownloader(gpu_agent, cpu_agent, NJ));
+    if ((rc = dma->start())) return rc;
+  }
+  // Two compute stages on two streams: centring, Gram and eigensolves of
+  // every slab in order on the compute stream, the reconstructions on a
+  // second stream, so that slab i's reconstruction runs beside slab i + 1's
+  // Gram and eigensolves (each slab's launches are smaller than the GPU, and
+  // the stages then fill each other's idle CUs). With DMA downloads the output
+  // stream carries no copies and is the reconstruction stream.
+  hipStream_t rs = (dma && so != sc) ? so : sc;
   int* rk = rank ? rank : w.rank;
   float* mg = margin ? margin : w.margin;
   int* inf = info ? info : w.info;
-  std::vector<int> out_lo(K, 0), out_hi(K, 0);  // output planes downloaded after slab i
+  std::vector<int> out_lo(NJ, 0), out_hi(NJ, 0);  // output planes of download j
+  int njobs = 0;
+
+  // Uploads run from their own thread, piece by piece as the staging finishes,
+  // so that they never wait behind kernel launches; slab i's uploads are
+  // enqueued (and ev_in[i] recorded) once up_slabs > i.
+  std::atomic<int> up_slabs{0};
+  std::atomic<int> up_err{0};
+  std::string up_msg;
+  std::thread uploader([&] {
+    size_t pc = 0;
+    for (int i = 0; i < K; ++i) {
+      for (; pc < pieces.size() && pieces[pc].slab == i; ++pc) {
+        const Piece& q = pieces[pc];
+        if (in_copy) in_copy->wait(pc);
+        const size_t o = (size_t)q.x0 * plane, n = (size_t)(q.x1 - q.x0) * plane;
+        const hipError_t e = hipMemcpyAsync(d_vol + o, src + o * sizeof(float), n * sizeof(float),
+                                            hipMemcpyHostToDevice, si);
+        if (e != hipSuccess) {
+          up_msg = std::string("upload: ") + hipGetErrorString(e);
+          up_err.store((int)e, std::memory_order_release);
+          return;
+        }
+      }
+      const hipError_t e = hipEventRecord(ev_in.ev[i], si);
+      if (e != hipSuccess) {
+        up_msg = std::string("event: ") + hipGetErrorString(e);
+        up_err.store((int)e, std::memory_order_release);
+        return;
+      }
+      up_slabs.store(i + 1, std::memory_order_release);
+    }
+  });
 
   int ix_done = 0, x_done = 0;
   const long long per_ix = (long long)g.ncy * g.ncz;
   const long long ff = pipelined ? factor_floats(T) : 0;
   const int tw = recon_factor_tile_x();
-  // Reconstruction and download of the planes slab j finished, [x_done, x_hi).
-  // The download is a copy kernel on the CUs that slows a memory-bound kernel
-  // beside it many times over (the centring), so slab j's reconstruction is
-  // enqueued after slab j + 1's centring: the download then runs beside the
-  // Gram and eigensolver of slab j + 1.
-  auto finish_slab = [&](int j, int x_hi) -> int {
+  // Reconstruction of planes [x_done, x_hi) on stream st, in npiece pieces,
+  // each downloaded once it has run.
+  auto finish_planes = [&](int x_hi, hipStream_t st, int npiece) -> int {
     int r = MPPCA_OK;
-    if (x_hi > x_done) {
-      if ((r = launch_recon_factors(w.xc, w.fac, rk, w.weight, w.mean, d_out, g, x_done, x_hi,
-                                    sc)))
-        return r;
+    const int x0 = x_done, span = x_hi - x0;
+    for (int k = 0; k < npiece; ++k) {
+      int b = k + 1 == npiece ? x_hi : x0 + (int)((long long)span * (k + 1) / npiece);
+      if (b != X) b -= b % tw;
+      if (k + 1 < npiece && b <= x_done) continue;
+      const int j = njobs++;
       out_lo[j] = x_done;
-      out_hi[j] = x_hi;
-      x_done = x_hi;
-    }
-    if ((r = hip_ok(hipEventRecord(ev_rec.ev[j], sc), "event")) ||
-        (r = hip_ok(hipStreamWaitEvent(so, ev_rec.ev[j], 0), "wait")))
-      return r;
-    if (out_hi[j] > out_lo[j]) {
+      out_hi[j] = b > x_done ? b : x_done;
+      if (b > x_done) {
+        if (pipelined && (r = launch_recon_factors(w.xc, w.fac, rk, w.weight, w.mean, d_out, g,
+                                                   x_done, b, st)))
+          return r;
+        x_done = b;
+      }
+      if ((r = hip_ok(hipEventRecord(ev_rec.ev[j], st), "event"))) return r;
       const size_t o = (size_t)out_lo[j] * plane, m = (size_t)(out_hi[j] - out_lo[j]) * plane;
-      if ((r = hip_ok(hipMemcpyAsync(dst_out + o, d_out + o, m * sizeof(float),
-                                     hipMemcpyDeviceToHost, so), "download")))
+      if (dma) {
+        dma->post(j, {(char*)(dst_out + o), (const char*)(d_out + o), m * sizeof(float),
+                      ev_rec.ev[j]});
+        continue;
+      }
+      if ((r = hip_ok(hipStreamWaitEvent(so, ev_rec.ev[j], 0), "wait"))) return r;
+      if (m > 0 && (r = hip_ok(hipMemcpyAsync(dst_out + o, d_out + o, m * sizeof(float),
+                                              hipMemcpyDeviceToHost, so), "download")))
         return r;
+      if ((r = hip_ok(hipEventRecord(ev_out.ev[j], so), "event"))) return r;
     }
-    return hip_ok(hipEventRecord(ev_out.ev[j], so), "event");
+    return r;
   };
-  int pend_hi = 0;  // planes the last slab finished (reconstructed after the next centring)
   for (int i = 0; i < K; ++i) {
-    const size_t off = (size_t)xa[i] * plane, n = (size_t)(xa[i + 1] - xa[i]) * plane;
-    if (in_copy) in_copy->wait(i);
-    if ((rc = hip_ok(hipMemcpyAsync(d_vol + off, src + off * sizeof(float), n * sizeof(float),
-                                    hipMemcpyHostToDevice, si), "upload")))
-      break;
-    if ((rc = hip_ok(hipEventRecord(ev_in.ev[i], si), "event")) ||
-        (rc = hip_ok(hipStreamWaitEvent(sc, ev_in.ev[i], 0), "wait")))
-      break;
+    while (up_slabs.load(std::memory_order_acquire) <= i && !up_err.load(std::memory_order_acquire))
+      std::this_thread::yield();
+    if (up_err.load(std::memory_order_acquire)) break;  // reported after the join
+    hipStream_t S = sc;
+    if ((rc = hip_ok(hipStreamWaitEvent(S, ev_in.ev[i], 0), "wait"))) break;
     if (!pipelined) {  // whole volume: one slab, the device path, one download
-      if ((rc = run_device_path(d_vol, d_out, g, form, w, rank, margin, info, sc))) break;
-      out_lo[0] = 0;
-      out_hi[0] = X;
-      x_done = X;
-      if ((rc = finish_slab(0, X))) break;
+      if ((rc = run_device_path(d_vol, d_out, g, form, w, rank, margin, info, S))) break;
+      if ((rc = finish_planes(X, S, 1))) break;
       continue;
     }
+    const size_t off = (size_t)xa[i] * plane;
     const long long vox0 = (long long)xa[i] * Y * Z, nv = (long long)(xa[i + 1] - xa[i]) * Y * Z;
-    if ((rc = launch_centre(d_vol + off, w.xc + off, w.mean + vox0, nv, T, sc))) break;
-    if (i > 0 && (rc = finish_slab(i - 1, pend_hi))) break;
+    if (nv > 0 && (rc = launch_centre(d_vol + off, w.xc + off, w.mean + vox0, nv, T, S))) break;
     int ix_new = ix_done;
     while (ix_new < g.ncx && corner_of(ix_new, g.nsx, X, px, stride) + px <= xa[i + 1]) ++ix_new;
     if (ix_new > ix_done) {
       const long long b0 = ix_done * per_ix, nb = (ix_new - ix_done) * per_ix;
-      if ((rc = launch_gram64(w.xc, (double*)w.gram + b0 * T * T, g, b0, nb, sc))) break;
+      if ((rc = launch_gram64(w.xc, (double*)w.gram + b0 * T * T, g, b0, nb, S))) break;
       if ((rc = launch_eig64_mp((double*)w.gram + b0 * T * T, nb, T, g.N, nullptr,
                                 w.fac + b0 * ff, rk + b0, w.weight + b0, mg + b0, nullptr,
-                                inf + b0, sc)))
+                                inf + b0, S)))
         break;
       ix_done = ix_new;
     }
+    // the reconstruction stream takes over: it reads the centred planes and the
+    // factors of every slab up to this one, all written on the compute stream
+    if ((rc = hip_ok(hipEventRecord(ev_eig.ev[i], S), "event")) ||
+        (rc = hip_ok(hipStreamWaitEvent(rs, ev_eig.ev[i], 0), "wait")))
+      break;
     // planes below the first unfinished corner (and uploaded) are final
     int x_hi = ix_done == g.ncx ? X : corner_of(ix_done, g.nsx, X, px, stride);
     if (x_hi > xa[i + 1]) x_hi = xa[i + 1];
     if (x_hi != X) x_hi -= x_hi % tw;
-    pend_hi = x_hi > x_done ? x_hi : x_done;
-    if (i + 1 == K && (rc = finish_slab(i, pend_hi))) break;
+    if ((rc = finish_planes(x_hi > x_done ? x_hi : x_done, rs, i + 1 == K ? kTailPieces : 1)))
+      break;
   }
+  uploader.join();
+  if (!rc && up_err.load()) rc = set_err(up_err.load(), "%s", up_msg.c_str());
   if (!rc && pipelined && x_done != X)
     rc = set_err(MPPCA_EINVAL, "denoise_host: planes [%d, %d) never finished", x_done, X);
   if (!rc && h_stage_out) {  // pinned staging -> caller's memory, range by range
     std::vector<RangeCopier::Range> r;
-    for (int i = 0; i < K; ++i)
-      if (out_hi[i] > out_lo[i])
-        r.push_back({(char*)(out + (size_t)out_lo[i] * plane),
-                     (const char*)(h_stage_out + (size_t)out_lo[i] * plane),
-                     (size_t)(out_hi[i] - out_lo[i]) * plane * sizeof(float), ev_out.ev[i]});
+    DmaDownloader* dl = dma.get();
+    for (int j = 0; j < njobs; ++j)
+      if (out_hi[j] > out_lo[j]) {
+        std::function<void()> after;
+        if (dl)
+          after = [dl, j] { (void)dl->wait(j); };
+        else
+          after = [e = ev_out.ev[j]] { (void)hipEventSynchronize(e); };
+        r.push_back({(char*)(out + (size_t)out_lo[j] * plane),
+                     (const char*)(h_stage_out + (size_t)out_lo[j] * plane),
+                     (size_t)(out_hi[j] - out_lo[j]) * plane * sizeof(float), std::move(after)});
+      }
     RangeCopier out_copy(std::move(r), nthreads);
   }
-  // drain every stream before the staging and the events go away
+  // finish the downloads and drain every stream before the staging and the
+  // events go away
+  const int rd = dma ? dma->finish() : MPPCA_OK;
   const hipError_t e1 = hipStreamSynchronize(so), e2 = hipStreamSynchronize(si),
                    e3 = hipStreamSynchronize(sc);
   in_copy.reset();
   if (rc) return rc;
+  if (rd) return rd;
   if ((rc = hip_ok(e1, "output stream")) || (rc = hip_ok(e2, "input stream")) ||
       (rc = hip_ok(e3, "compute stream")))
     return rc;

@@ -82,12 +82,13 @@ class Denoiser:
         self.margin = torch.empty(self.n_patches, dtype=torch.float32, device=dev)
         self.sweeps = torch.empty(self.n_patches, dtype=torch.int32, device=dev)
         self._pin_in = None
-        # host -> host pipeline: x-slabs and host copy threads (a sweep of 4-12
-        # slabs and 8-16 threads on the default volume: 6 slabs and 8 threads
+        # host -> host pipeline: x-slabs and host copy threads (a sweep of 4-16
+        # slabs and 6-16 threads on the default volume: 7 slabs and 6 threads
         # best; more slabs are slower because every launch of a slab costs at
-        # least one pass of its longest workgroup, however few workgroups it has)
-        self.nslabs = 6
-        self.nthreads = max(1, min(8, os.cpu_count() or 1))
+        # least one pass of its longest workgroup, however few workgroups it has,
+        # and more threads take cores from the threads that drive the GPU)
+        self.nslabs = 7
+        self.nthreads = max(1, min(6, os.cpu_count() or 1))
 
     def _geom_args(self):
         x, y, z, t = self.shape
@@ -239,7 +240,9 @@ class Denoiser:
         # returned as is (as step8_tuned.py:144-146 returns its pinned tensor)
         host = torch.empty(self.shape, dtype=torch.float32, pin_memory=True)
         sc, si, so = self._streams
-        sc.wait_stream(torch.cuda.current_stream(self.device))  # earlier work on the buffers
+        cur = torch.cuda.current_stream(self.device)
+        sc.wait_stream(cur)  # earlier work on the buffers
+        so.wait_stream(cur)  # (the output stream computes too when downloads use DMA)
         rc = self.lib.mppca_denoise_host(
             a.ctypes.data, host.data_ptr(), *self._geom_args(), self.form_code,
             self._ws_ptr, self._ws_bytes, self.vol_d.data_ptr(), self.out_d.data_ptr(),
