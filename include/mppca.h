@@ -53,6 +53,30 @@ extern "C" {
 #define MPPCA_FORM_PLAIN 0
 #define MPPCA_FORM_FP32 1
 
+/* Tile schedule: the kernel choices of the whole path for one geometry.
+ *   gram      MPPCA_GRAM_*: the FP64 Gram's kernel (plain form): the int8
+ *             sliding window, the FP64 sliding window, or per-patch sums.
+ *             A kernel that cannot take the geometry gives MPPCA_EUNSUPPORTED.
+ *   eig       MPPCA_EIG_*: the FP64 eigensolver's layout: the 8 x 8 lane grid
+ *             of three kernels (T <= 48) or one kernel with the matrix in LDS.
+ *   recon_sx  x-slabs of 16 voxels per wavefront in the fused reconstruction
+ *             (1, 2 or 4).
+ * 0 in any field keeps the built-in rule. Results do not depend on the
+ * choice beyond rounding: every Gram kernel is within its stated bound of
+ * the exact Gram, and the eigensolver layouts and tiles compute the same
+ * arithmetic. N and T key the entries of the autotune table. */
+#define MPPCA_GRAM_AUTO 0
+#define MPPCA_GRAM_LIMB 1
+#define MPPCA_GRAM_SLIDE 2
+#define MPPCA_GRAM_PATCH 3
+#define MPPCA_EIG_AUTO 0
+#define MPPCA_EIG_GRID 1
+#define MPPCA_EIG_LDS 2
+typedef struct {
+  int32_t N, T;
+  int32_t gram, eig, recon_sx;
+} mppca_tuning;
+
 /* Version of this ABI (major * 100 + minor). */
 int mppca_abi_version(void);
 
@@ -250,6 +274,74 @@ int mppca_denoise_host(const float* vol, float* out, int X, int Y, int Z, int T,
                        float* h_stage_out, int nslabs, int nthreads,
                        void* s_compute, void* s_in, void* s_out,
                        int32_t* rank, float* margin, int32_t* info);
+
+/* ---------------------------------------------------------------------------
+ * Batches of volumes of one geometry
+ *
+ * A batch is nvol volumes of one geometry stored back to back in one device
+ * buffer (volume v at v * X*Y*Z*T floats), denoised by the same kernel
+ * launches: every stage runs once over the whole batch, its grid spanning
+ * all nvol volumes (the eigensolver sees nvol * P matrices), so that
+ * small volumes fill the GPU together and the launch overheads are shared.
+ * Per-patch arrays (rank, margin, info) hold nvol * P entries, volume v's
+ * from v * P. Volumes of different geometries form different batches.
+ * Each volume's result is the one mppca_denoise_device gives it alone, bit
+ * for bit on the factor route (plain form, T <= 64).
+ * ------------------------------------------------------------------------- */
+
+#define MPPCA_MAX_BATCH 256
+
+/* Installs the autotune table: per (N, T), the tile schedule every entry
+ * point then uses for that geometry (where the caller passes none). The
+ * table is read-only once installed: the first call sets it, later calls
+ * return MPPCA_EINVAL. An entry with a field 0 keeps that field's rule.
+ * Has no counterpart in the reference, whose launch configurations are
+ * constants picked by a sweep (step8_tuned.py:22-23; micro/tune_recon.py). */
+int mppca_tune_table_set(const mppca_tuning* entries, int n);
+/* Entries of the installed table (0 if none). */
+int mppca_tune_table_size(void);
+
+/* Workspace bytes of a batch of nvol volumes (< 0: error). */
+int64_t mppca_batch_workspace_bytes(int nvol, int X, int Y, int Z, int T, int px, int py,
+                                    int pz, int stride, int form);
+
+/* mppca_denoise_device over a batch of nvol volumes (vol and out hold
+ * nvol * X*Y*Z*T floats each), with the tile schedule `tuning` (NULL: the
+ * autotune table's entry for this geometry, else the built-in rules). Replaces step8_tuned.denoise's device part
+ * (step8_tuned.py:119-145), applied to several volumes in the same launches. */
+int mppca_denoise_device_batch(const float* vol, float* out, int nvol, int X, int Y, int Z, int T,
+                               int px, int py, int pz, int stride, int form,
+                               const mppca_tuning* tuning, void* workspace,
+                               int64_t workspace_bytes, int32_t* rank, float* margin,
+                               int32_t* info, void* stream);
+
+/* The launches of mppca_denoise_device_batch for these exact buffers,
+ * captured once into an executable HIP graph (*graph): replaying it
+ * (mppca_graph_launch) runs the whole path with one launch call. The graph
+ * holds no memset nodes (zero fills are kernels). Destroy it with
+ * mppca_graph_destroy. */
+int mppca_graph_create(void** graph, const float* vol, float* out, int nvol, int X, int Y, int Z,
+                       int T, int px, int py, int pz, int stride, int form,
+                       const mppca_tuning* tuning, void* workspace, int64_t workspace_bytes,
+                       int32_t* rank, float* margin, int32_t* info);
+int mppca_graph_launch(void* graph, void* stream);
+int mppca_graph_destroy(void* graph);
+
+/* ---------------------------------------------------------------------------
+ * Transfers
+ * ------------------------------------------------------------------------- */
+
+/* Blocking device -> host copy on a DMA engine (not a copy kernel on the
+ * CUs): waits for after_event (a hipEvent_t, or NULL) on the host, then
+ * copies `bytes` from src_dev into dst_host (pinned host memory; other host
+ * memory takes the runtime's own copy). Replaces the pinned D2H of
+ * step8_tuned.py:144-146. */
+int mppca_copy_d2h(void* dst_host, const void* src_dev, int64_t bytes, void* after_event);
+
+/* Blocking host copy of `bytes` by nthreads threads (1..256), with
+ * non-temporal stores where the CPU allows: the staging copy of a pageable
+ * volume into pinned memory before an upload. */
+int mppca_host_copy(void* dst, const void* src, int64_t bytes, int nthreads);
 
 /* ---------------------------------------------------------------------------
  * Whole path without any allocation

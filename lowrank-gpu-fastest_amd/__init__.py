@@ -20,6 +20,7 @@ from .phantom import (  # noqa: F401
 )
 from .denoise import FORMS, Denoiser, SolverStatusError, denoise, get_denoiser  # noqa: F401
 from .serve import AdmissionError, Server, device_bytes, poisson_run  # noqa: F401
+from .batch import BatchDenoiser, denoise_many  # noqa: F401
 
 __all__ = [
     "denoise",
@@ -41,4 +42,6 @@ __all__ = [
     "poisson_run",
     "device_bytes",
     "AdmissionError",
+    "BatchDenoiser",
+    "denoise_many",
 ]

@@ -12,7 +12,7 @@ import pathlib
 
 PKG = pathlib.Path(__file__).resolve().parent
 LIB_PATH = PKG / "libmppca.so"
-ABI_VERSION = 103
+ABI_VERSION = 104
 FORM_CODES = {"plain": 0, "fp32": 1}
 
 _i32, _i64, _vp, _f = ctypes.c_int, ctypes.c_int64, ctypes.c_void_p, ctypes.c_float
@@ -39,9 +39,29 @@ SIGNATURES = {
     "mppca_denoise_device": (_i32, [_vp, _vp] + [_i32] * 9 + [_vp, _i64, _vp, _vp, _vp, _vp]),
     "mppca_denoise_host": (_i32, [_vp, _vp] + [_i32] * 9 + [_vp, _i64] + [_vp] * 4
                            + [_i32, _i32] + [_vp] * 6),
+    "mppca_batch_workspace_bytes": (_i64, [_i32] * 10),
+    "mppca_denoise_device_batch": (_i32, [_vp, _vp] + [_i32] * 10 + [_vp, _vp, _i64] + [_vp] * 4),
+    "mppca_graph_create": (_i32, [_vp, _vp, _vp] + [_i32] * 10 + [_vp, _vp, _i64] + [_vp] * 3),
+    "mppca_tune_table_set": (_i32, [_vp, _i32]),
+    "mppca_tune_table_size": (_i32, []),
+    "mppca_graph_launch": (_i32, [_vp, _vp]),
+    "mppca_graph_destroy": (_i32, [_vp]),
+    "mppca_copy_d2h": (_i32, [_vp, _vp, _i64, _vp]),
+    "mppca_host_copy": (_i32, [_vp, _vp, _i64, _i32]),
     "mppca_noalloc_scratch_bytes": (_i64, []),
     "mppca_denoise_device_noalloc": (_i32, [_vp, _vp] + [_i32] * 9 + [_vp] * 4),
 }
+
+
+class Tuning(ctypes.Structure):
+    """``mppca_tuning`` (include/mppca.h): the tile schedule of one geometry."""
+
+    _fields_ = [("N", ctypes.c_int32), ("T", ctypes.c_int32), ("gram", ctypes.c_int32),
+                ("eig", ctypes.c_int32), ("recon_sx", ctypes.c_int32)]
+
+
+GRAM_KERNELS = {"auto": 0, "limb": 1, "slide": 2, "patch": 3}
+EIG_LAYOUTS = {"auto": 0, "grid": 1, "lds": 2}
 
 
 class MppcaError(RuntimeError):

@@ -3,6 +3,9 @@
 #include <stdarg.h>
 #include <stdio.h>
 
+#include <atomic>
+#include <vector>
+
 #include "common.h"
 
 namespace mppca {
@@ -66,11 +69,27 @@ int make_geom(Geom* g, int X, int Y, int Z, int T, int px, int py, int pz, int s
   g->N = (int)N;
   g->P = (long long)g->ncx * g->ncy * g->ncz;
   g->nvox = (long long)X * Y * Z;
+  g->nvol = 1;
+  g->tune = tune_lookup(g->N, T);
   if (g->nvox * T >= (1LL << 40)) return set_err(MPPCA_EUNSUPPORTED, "volume too large");
   return MPPCA_OK;
 }
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// ---- the autotune table: installed once, read-only afterwards ----------------------
+struct TuneTable {
+  std::vector<mppca_tuning> e;
+};
+static std::atomic<const TuneTable*> g_tune{nullptr};
+
+Tune tune_lookup(int N, int T) {
+  const TuneTable* t = g_tune.load(std::memory_order_acquire);
+  if (t)
+    for (const mppca_tuning& x : t->e)
+      if (x.N == N && x.T == T) return Tune{x.gram, x.eig, x.recon_sx};
+  return Tune{0, 0, 0};
+}
 
 Workspace carve_workspace(char* base, const Geom& g, int form) {
   Workspace w;
@@ -80,25 +99,27 @@ Workspace carve_workspace(char* base, const Geom& g, int form) {
     off += align256(bytes);
     return (void*)p;
   };
-  const size_t ptt = (size_t)g.P * g.T * g.T;
-  w.xc = (float*)take((size_t)g.nvox * g.T * sizeof(float));
-  w.mean = (float*)take((size_t)g.nvox * sizeof(float));
+  // every buffer holds the nvol volumes of a batch back to back
+  const size_t P = (size_t)g.P * g.nvol, nvox = (size_t)g.nvox * g.nvol;
+  const size_t ptt = P * g.T * g.T;
+  w.xc = (float*)take(nvox * g.T * sizeof(float));
+  w.mean = (float*)take(nvox * sizeof(float));
   w.proj = nullptr;
   w.fac = nullptr;
   if (form == MPPCA_FORM_PLAIN) {
     w.gram = take(ptt * sizeof(double));
     if (g.T <= kFactorMaxT)
-      w.fac = (float*)take((size_t)g.P * factor_floats(g.T) * sizeof(float));
+      w.fac = (float*)take(P * factor_floats(g.T) * sizeof(float));
     else
       w.proj = (float*)take(ptt * sizeof(float));
   } else {
     w.gram = take(ptt * sizeof(float));
     w.proj = (float*)w.gram;
   }
-  w.weight = (float*)take((size_t)g.P * sizeof(float));
-  w.rank = (int*)take((size_t)g.P * sizeof(int));
-  w.margin = (float*)take((size_t)g.P * sizeof(float));
-  w.info = (int*)take((size_t)g.P * sizeof(int));
+  w.weight = (float*)take(P * sizeof(float));
+  w.rank = (int*)take(P * sizeof(int));
+  w.margin = (float*)take(P * sizeof(float));
+  w.info = (int*)take(P * sizeof(int));
   w.bytes = off;
   return w;
 }
@@ -285,6 +306,134 @@ int mppca_denoise_device(const float* vol, float* out, int X, int Y, int Z, int 
   return run_device_path(vol, out, g, form, w, rank, margin, info, (hipStream_t)stream);
 }
 
+int mppca_tune_table_set(const mppca_tuning* entries, int n) {
+  if (n < 0 || (n > 0 && !entries)) return set_err(MPPCA_EINVAL, "tune table: bad entries");
+  for (int k = 0; k < n; ++k) {
+    const mppca_tuning& x = entries[k];
+    if (x.gram < 0 || x.gram > MPPCA_GRAM_PATCH || x.eig < 0 || x.eig > MPPCA_EIG_LDS ||
+        (x.recon_sx != 0 && x.recon_sx != 1 && x.recon_sx != 2 && x.recon_sx != 4))
+      return set_err(MPPCA_EINVAL, "tune table: entry %d (N %d, T %d) out of range", k, x.N, x.T);
+  }
+  TuneTable* t = new TuneTable{std::vector<mppca_tuning>(entries, entries + n)};
+  const TuneTable* expect = nullptr;
+  if (!g_tune.compare_exchange_strong(expect, t)) {
+    delete t;
+    return set_err(MPPCA_EINVAL, "tune table: already installed (it is read-only)");
+  }
+  return MPPCA_OK;  // kept until the process ends
+}
+
+int mppca_tune_table_size(void) {
+  const TuneTable* t = g_tune.load(std::memory_order_acquire);
+  return t ? (int)t->e.size() : 0;
+}
+
+static int batch_geom(Geom* g, int nvol, int X, int Y, int Z, int T, int px, int py, int pz,
+                      int stride, int form, const mppca_tuning* tuning) {
+  int rc = make_geom(g, X, Y, Z, T, px, py, pz, stride);
+  if (rc) return rc;
+  if (tuning) {
+    const mppca_tuning& x = *tuning;
+    if (x.gram < 0 || x.gram > MPPCA_GRAM_PATCH || x.eig < 0 || x.eig > MPPCA_EIG_LDS ||
+        (x.recon_sx != 0 && x.recon_sx != 1 && x.recon_sx != 2 && x.recon_sx != 4))
+      return set_err(MPPCA_EINVAL, "tuning (%d, %d, %d) out of range", x.gram, x.eig,
+                     x.recon_sx);
+    g->tune = Tune{x.gram, x.eig, x.recon_sx};
+  }
+  if (!valid_form(form)) return set_err(MPPCA_EINVAL, "unknown form %d", form);
+  if (nvol < 1 || nvol > MPPCA_MAX_BATCH)
+    return set_err(MPPCA_EINVAL, "batch of %d volumes outside [1, %d]", nvol, MPPCA_MAX_BATCH);
+  if (g->P * nvol >= (1LL << 31) || g->nvox * T * nvol >= (1LL << 40))
+    return set_err(MPPCA_EUNSUPPORTED, "batch of %d volumes too large", nvol);
+  g->nvol = nvol;
+  return MPPCA_OK;
+}
+
+int64_t mppca_batch_workspace_bytes(int nvol, int X, int Y, int Z, int T, int px, int py, int pz,
+                                    int stride, int form) {
+  Geom g;
+  const int rc = batch_geom(&g, nvol, X, Y, Z, T, px, py, pz, stride, form, nullptr);
+  if (rc) return rc;
+  return (int64_t)carve_workspace(nullptr, g, form).bytes;
+}
+
+int mppca_denoise_device_batch(const float* vol, float* out, int nvol, int X, int Y, int Z, int T,
+                               int px, int py, int pz, int stride, int form,
+                               const mppca_tuning* tuning, void* workspace,
+                               int64_t workspace_bytes, int32_t* rank, float* margin,
+                               int32_t* info, void* stream) {
+  Geom g;
+  int rc = batch_geom(&g, nvol, X, Y, Z, T, px, py, pz, stride, form, tuning);
+  if (rc) return rc;
+  if (!vol || !out || !workspace) return set_err(MPPCA_EINVAL, "denoise (batch): null pointer");
+  if (vol == out) return set_err(MPPCA_EINVAL, "denoise (batch): vol and out must be distinct");
+  if ((uintptr_t)workspace % 256 != 0)
+    return set_err(MPPCA_EINVAL, "denoise (batch): workspace must be 256-byte aligned");
+  Workspace w = carve_workspace((char*)workspace, g, form);
+  if ((int64_t)w.bytes > workspace_bytes)
+    return set_err(MPPCA_ENOSPACE, "denoise (batch): workspace %lld < %zu bytes",
+                   (long long)workspace_bytes, w.bytes);
+  return run_device_path(vol, out, g, form, w, rank, margin, info, (hipStream_t)stream);
+}
+
+int mppca_graph_create(void** graph, const float* vol, float* out, int nvol, int X, int Y, int Z,
+                       int T, int px, int py, int pz, int stride, int form,
+                       const mppca_tuning* tuning, void* workspace, int64_t workspace_bytes,
+                       int32_t* rank, float* margin, int32_t* info) {
+  if (!graph) return set_err(MPPCA_EINVAL, "graph: null handle pointer");
+  *graph = nullptr;
+  Geom g;
+  int rc = batch_geom(&g, nvol, X, Y, Z, T, px, py, pz, stride, form, tuning);
+  if (rc) return rc;
+  if (!vol || !out || !workspace) return set_err(MPPCA_EINVAL, "graph: null pointer");
+  if (vol == out) return set_err(MPPCA_EINVAL, "graph: vol and out must be distinct");
+  if ((uintptr_t)workspace % 256 != 0)
+    return set_err(MPPCA_EINVAL, "graph: workspace must be 256-byte aligned");
+  Workspace w = carve_workspace((char*)workspace, g, form);
+  if ((int64_t)w.bytes > workspace_bytes)
+    return set_err(MPPCA_ENOSPACE, "graph: workspace %lld < %zu bytes",
+                   (long long)workspace_bytes, w.bytes);
+  // capture on a private stream: no other work can join the graph
+  hipStream_t cs = nullptr;
+  hipError_t e = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking);
+  if (e != hipSuccess) return set_err((int)e, "graph: stream: %s", hipGetErrorString(e));
+  hipGraph_t gr = nullptr;
+  hipGraphExec_t ex = nullptr;
+  e = hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed);
+  if (e == hipSuccess) {
+    rc = run_device_path(vol, out, g, form, w, rank, margin, info, cs);
+    const hipError_t e2 = hipStreamEndCapture(cs, &gr);
+    if (!rc && e2 != hipSuccess)
+      rc = set_err((int)e2, "graph: end of capture: %s", hipGetErrorString(e2));
+    if (!rc) {
+      e = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+      if (e != hipSuccess) rc = set_err((int)e, "graph: instantiate: %s", hipGetErrorString(e));
+    }
+  } else {
+    rc = set_err((int)e, "graph: begin capture: %s", hipGetErrorString(e));
+  }
+  if (gr) (void)hipGraphDestroy(gr);
+  (void)hipStreamDestroy(cs);
+  if (rc) {
+    if (ex) (void)hipGraphExecDestroy(ex);
+    return rc;
+  }
+  *graph = (void*)ex;
+  return MPPCA_OK;
+}
+
+int mppca_graph_launch(void* graph, void* stream) {
+  if (!graph) return set_err(MPPCA_EINVAL, "graph: null graph");
+  const hipError_t e = hipGraphLaunch((hipGraphExec_t)graph, (hipStream_t)stream);
+  return e == hipSuccess ? MPPCA_OK : set_err((int)e, "graph launch: %s", hipGetErrorString(e));
+}
+
+int mppca_graph_destroy(void* graph) {
+  if (!graph) return MPPCA_OK;
+  const hipError_t e = hipGraphExecDestroy((hipGraphExec_t)graph);
+  return e == hipSuccess ? MPPCA_OK : set_err((int)e, "graph destroy: %s", hipGetErrorString(e));
+}
+
 }  // extern "C"
 
 namespace mppca {
@@ -296,21 +445,24 @@ int run_device_path(const float* vol, float* out, const Geom& g, int form, const
   int* rk = rank ? rank : w.rank;
   float* mg = margin ? margin : w.margin;
   int* inf = info ? info : w.info;
-  if ((rc = launch_centre(vol, w.xc, w.mean, g.nvox, T, st))) return rc;
+  // the voxel-wise and matrix-wise stages see the batch's volumes as one
+  // array; the others map their grids onto the volumes (Geom::nvol)
+  const long long P = g.P * g.nvol;
+  if ((rc = launch_centre(vol, w.xc, w.mean, g.nvox * g.nvol, T, st))) return rc;
   if (form == MPPCA_FORM_PLAIN) {
     if ((rc = launch_gram64(w.xc, (double*)w.gram, g, 0, g.P, st))) return rc;
-    if ((rc = launch_eig64_mp((double*)w.gram, g.P, T, g.N, w.proj, w.fac, rk, w.weight, mg,
-                              nullptr, inf, st)))
+    if ((rc = launch_eig64_mp((double*)w.gram, P, T, g.N, w.proj, w.fac, rk, w.weight, mg,
+                              nullptr, inf, st, g.tune.eig)))
       return rc;
     if (w.fac) return launch_recon_factors(w.xc, w.fac, rk, w.weight, w.mean, out, g, 0, g.X, st);
   } else {
     if ((rc = launch_gram(w.xc, (float*)w.gram, g, 0, g.P, st))) return rc;
-    if ((rc = launch_eigh_mp((const float*)w.gram, g.P, T, g.N, 20, w.proj, rk, w.weight, mg,
+    if ((rc = launch_eigh_mp((const float*)w.gram, P, T, g.N, 20, w.proj, rk, w.weight, mg,
                              nullptr, inf, st)))
       return rc;
   }
   // projector route: atomic scatter into the zeroed output, then normalise
-  if ((rc = zero_fill(out, g.nvox * g.T, st))) return rc;
+  if ((rc = zero_fill(out, g.nvox * g.T * g.nvol, st))) return rc;
   if ((rc = launch_recon(w.xc, w.proj, w.weight, out, g, 0, g.P, st))) return rc;
   return launch_normalize(out, w.mean, w.weight, out, g, st);
 }

@@ -15,7 +15,7 @@
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef double doublex4 __attribute__((ext_vector_type(4)));
 
-#define MPPCA_ABI_VERSION 103
+#define MPPCA_ABI_VERSION 104
 #define MAX_T 128
 #define MAX_N 4096
 
@@ -26,6 +26,16 @@ int set_err(int code, const char* fmt, ...);
 int launch_status(const char* what);
 int prepare_lds(const void* fn, size_t bytes);
 
+// ---- tile schedule -------------------------------------------------------------
+// Kernel choices of the whole path for one geometry (0 = the built-in rule):
+// the Gram kernel (MPPCA_GRAM_*), the eigensolver layout (MPPCA_EIG_*) and the
+// reconstruction's x-slabs per wavefront (1, 2 or 4). Set per call, or looked
+// up by (N, T) in the read-only table an autotuner installs once (abi.hip).
+struct Tune {
+  int gram, eig, recon_sx;
+};
+Tune tune_lookup(int N, int T);
+
 // ---- geometry ---------------------------------------------------------------
 struct Geom {
   int X, Y, Z, T;
@@ -35,6 +45,12 @@ struct Geom {
   int N;              // voxels per patch
   long long P;        // patches
   long long nvox;     // X*Y*Z
+  // Volumes of one batch (1 for a single volume): nvol volumes of this same
+  // geometry stored back to back, volume v's data at v * (its size) in every
+  // buffer (voxels, means, Gram matrices, factors, per-patch arrays). Kernels
+  // take whole volumes' work for each: their grids span all nvol volumes.
+  int nvol;
+  Tune tune;  // kernel choices (all 0: the built-in rules)
 };
 
 int make_geom(Geom* g, int X, int Y, int Z, int T, int px, int py, int pz, int stride);
@@ -237,15 +253,16 @@ int launch_mp(const float* lam, long long nb, int T, int N, int* rank, float* we
 // proj (T x T projectors) and/or fac (rank-p factors, T <= kFactorMaxT) may be
 // null; each non-null one is written.
 int launch_eig64_mp(double* gram, long long nb, int T, int N, float* proj, float* fac, int* rank,
-                    float* weight, float* margin, float* lam_desc, int* info, hipStream_t st);
+                    float* weight, float* margin, float* lam_desc, int* info, hipStream_t st,
+                    int variant = 0);
 int launch_recon(const float* xc, const float* proj, const float* weight, float* acc,
                  const Geom& g, long long b0, long long nb, hipStream_t st);
-// Output planes x in [x0, x1) (x0 a multiple of recon_factor_tile_x(), x1 too
+// Output planes x in [x0, x1) (x0 a multiple of recon_factor_tile_x(g), x1 too
 // or X); reads the factors of every patch that meets them.
 int launch_recon_factors(const float* xc, const float* fac, const int* rank, const float* weight,
                          const float* mean, float* out, const Geom& g, int x0, int x1,
                          hipStream_t st);
-int recon_factor_tile_x();
+int recon_factor_tile_x(const Geom& g);
 
 // Host copy of n bytes into pinned staging (host_copy.cpp): non-temporal
 // stores where the CPU allows, memcpy otherwise.

@@ -1398,15 +1398,23 @@ static int launch_rows(double* gram, long long nb, int T, int N, float* proj, fl
 }
 
 int launch_eig64_mp(double* gram, long long nb, int T, int N, float* proj, float* fac, int* rank,
-                    float* weight, float* margin, float* lam_desc, int* info, hipStream_t st) {
+                    float* weight, float* margin, float* lam_desc, int* info, hipStream_t st,
+                    int variant) {
   if (fac && T > kFactorMaxT)
     return set_err(MPPCA_EUNSUPPORTED, "eig64: factors need T <= %d (T = %d)", kFactorMaxT, T);
-  if (T <= 16)
-    return launch_rows<16>(gram, nb, T, N, proj, fac, rank, weight, margin, lam_desc, info, st);
-  if (T <= 32)
-    return launch_rows<32>(gram, nb, T, N, proj, fac, rank, weight, margin, lam_desc, info, st);
-  if (T <= 48)
-    return launch_rows<48>(gram, nb, T, N, proj, fac, rank, weight, margin, lam_desc, info, st);
+  if (variant != MPPCA_EIG_AUTO && variant != MPPCA_EIG_GRID && variant != MPPCA_EIG_LDS)
+    return set_err(MPPCA_EINVAL, "eig64: unknown eigensolver layout %d", variant);
+  if (variant == MPPCA_EIG_GRID && T > 48)
+    return set_err(MPPCA_EUNSUPPORTED, "eig64: the lane-grid eigensolver takes T <= 48 (T = %d)",
+                   T);
+  if (variant != MPPCA_EIG_LDS) {
+    if (T <= 16)
+      return launch_rows<16>(gram, nb, T, N, proj, fac, rank, weight, margin, lam_desc, info, st);
+    if (T <= 32)
+      return launch_rows<32>(gram, nb, T, N, proj, fac, rank, weight, margin, lam_desc, info, st);
+    if (T <= 48)
+      return launch_rows<48>(gram, nb, T, N, proj, fac, rank, weight, margin, lam_desc, info, st);
+  }
   const size_t lds = tri_lds_bytes(T);
   int rc = prepare_lds((const void*)k_eig64_mp, lds);
   if (rc) return rc;

@@ -58,11 +58,12 @@ __global__ __launch_bounds__(256) void k_gram(const float* __restrict__ xc, S* _
   // workgroups are dealt round-robin to the 8 XCDs; XCD k takes a contiguous
   // run of patches, so neighbouring patches (which share most voxels) share an L2
   const long long blk = (long long)(blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
-  const long long bl = blk * 4 + wave;
-  if (bl >= nb) return;
+  const long long bl = blk * 4 + wave;  // over the nb patches of each of the batch's volumes
+  if (bl >= nb * g.nvol) return;
   const int T = g.T, N = g.N;
   const int i = lane & 15, kq = lane >> 4;
-  const float* xb = xc + patch_base(g, b0 + bl) * T;
+  const long long vb = bl / nb;
+  const float* xb = xc + vb * g.nvox * T + patch_base(g, b0 + (bl - vb * nb)) * T;
   constexpr int NACC = NTB * (NTB + 1) / 2;
   V4 acc[NACC];
 #pragma unroll
@@ -123,7 +124,7 @@ __global__ __launch_bounds__(256) void k_gram(const float* __restrict__ xc, S* _
 template <int NTB, typename S>
 static int launch_gram_t(const float* xc, S* gram, const Geom& g, long long b0, long long nb,
                          hipStream_t st) {
-  const long long blocks = (nb + 3) / 4;
+  const long long blocks = (nb * g.nvol + 3) / 4;
   const long long per_xcd = (blocks + 7) / 8;
   if (per_xcd > (1LL << 28)) return set_err(MPPCA_EUNSUPPORTED, "gram: too many patches");
   hipLaunchKernelGGL((k_gram<NTB, S>), dim3((unsigned)(8 * per_xcd)), dim3(256),
@@ -205,14 +206,18 @@ __global__ __launch_bounds__(64 * NTB * (NTB + 1) / 2 * slide_cols(NTB)) __attri
   const int blk = unit / nseg, seg = unit - blk * nseg;
   const int h = threadIdx.x / NCOL, tid = threadIdx.x - h * NCOL;  // column slot, thread in it
   const int col_raw = blk * slide_cols(NTB) + h;
-  if (blk * slide_cols(NTB) >= ncol) return;  // whole workgroup past the end
+  const int ncall = ncol * g.nvol;  // the columns of every volume of the batch
+  if (blk * slide_cols(NTB) >= ncall) return;  // whole workgroup past the end
   // this run's windows [w0, w1) and planes [z0, z1)
   const int w0 = (int)((long long)seg * g.ncz / nseg), w1 = (int)((long long)(seg + 1) * g.ncz / nseg);
   const int z0 = corner_of(w0, g.nsz, g.Z, g.pz, g.stride);
   const int z1 = corner_of(w1 - 1, g.nsz, g.Z, g.pz, g.stride) + g.pz;
   // a slot past the last column repeats it without storing, to keep the barriers
-  const bool writer = col_raw < ncol;
-  const int col = writer ? col_raw : ncol - 1;
+  const bool writer = col_raw < ncall;
+  const int colb = writer ? col_raw : ncall - 1;
+  const int vb = colb / ncol, col = colb - vb * ncol;  // volume of the batch, column in it
+  xc += (long long)vb * g.nvox * g.T;
+  gram += (long long)vb * g.P * g.T * g.T;
   float* s_pl = s_lds + h * (R * TP + F);                 // this column's plane tile
   unsigned* s_rowb = (unsigned*)(s_pl + R * TP);          // byte offset of footprint row n
   const int ix = ix0 + col / g.ncy, iy = col % g.ncy;
@@ -395,12 +400,16 @@ __global__ __launch_bounds__(64 * NTB * (NTB + 1) / 2 * limb_cols(NTB)) void k_g
   const int blk = unit / nseg, seg = unit - blk * nseg;
   const int h = threadIdx.x / NCOL, tid = threadIdx.x - h * NCOL;
   const int col_raw = blk * limb_cols(NTB) + h;
-  if (blk * limb_cols(NTB) >= ncol) return;
+  const int ncall = ncol * g.nvol;  // the columns of every volume of the batch
+  if (blk * limb_cols(NTB) >= ncall) return;
   const int w0 = (int)((long long)seg * g.ncz / nseg), w1 = (int)((long long)(seg + 1) * g.ncz / nseg);
   const int z0 = corner_of(w0, g.nsz, g.Z, g.pz, g.stride);
   const int z1 = corner_of(w1 - 1, g.nsz, g.Z, g.pz, g.stride) + g.pz;
-  const bool writer = col_raw < ncol;
-  const int col = writer ? col_raw : ncol - 1;
+  const bool writer = col_raw < ncall;
+  const int colb = writer ? col_raw : ncall - 1;
+  const int vb = colb / ncol, col = colb - vb * ncol;  // volume of the batch, column in it
+  xc += (long long)vb * g.nvox * g.T;
+  gram += (long long)vb * g.P * g.T * g.T;
   unsigned char* s_t = s_lb + h * limb_col_bytes(NTB, F);  // 2 x kLimbs tiles
   unsigned* s_rowb = (unsigned*)(s_t + 2 * kLimbs * TB);   // byte offset of footprint row n
   unsigned* s_wmax = s_rowb + R;                           // 2 x per-wavefront plane maxima
@@ -675,7 +684,7 @@ static int launch_gram_slide_t(const float* xc, double* gram, const Geom& g, int
   if (rc != MPPCA_OK) return rc;
   const int ncol = nix * g.ncy;
   int nblk = 0, per_xcd = 0;
-  const int nseg = slide_runs(g, ncol, slide_cols(NTB),
+  const int nseg = slide_runs(g, ncol * g.nvol, slide_cols(NTB),
                               resident_slots((const void*)k_gram_slide<NTB>, NTHR, lds, cache),
                               &nblk, &per_xcd);
   if (nseg == 0) return set_err(MPPCA_EUNSUPPORTED, "gram (sliding): too many columns");
@@ -694,7 +703,7 @@ static int launch_gram_limb_t(const float* xc, double* gram, const Geom& g, int 
   if (rc != MPPCA_OK) return rc;
   const int ncol = nix * g.ncy;
   int nblk = 0, per_xcd = 0;
-  const int nseg = slide_runs(g, ncol, limb_cols(NTB),
+  const int nseg = slide_runs(g, ncol * g.nvol, limb_cols(NTB),
                               resident_slots((const void*)k_gram_limb<NTB, RH>, NTHR, lds, cache),
                               &nblk, &per_xcd);
   if (nseg == 0) return set_err(MPPCA_EUNSUPPORTED, "gram (int8 sliding): too many columns");
@@ -734,7 +743,15 @@ int launch_gram64(const float* xc, double* gram, const Geom& g, long long b0, lo
   // whole x-corner columns: a sliding-window kernel
   const long long per_ix = (long long)g.ncy * g.ncz;
   const bool cols = nb > 0 && b0 % per_ix == 0 && nb % per_ix == 0;
-  const int lntb = cols ? limb_ntb(g) : 0;
+  const int want = g.tune.gram;
+  if (want != MPPCA_GRAM_AUTO && want != MPPCA_GRAM_LIMB && want != MPPCA_GRAM_SLIDE &&
+      want != MPPCA_GRAM_PATCH)
+    return set_err(MPPCA_EINVAL, "gram64: unknown Gram kernel %d", want);
+  const int lntb = cols && (want == MPPCA_GRAM_AUTO || want == MPPCA_GRAM_LIMB) ? limb_ntb(g) : 0;
+  if (want == MPPCA_GRAM_LIMB && lntb == 0)
+    return set_err(MPPCA_EUNSUPPORTED, "gram64: the int8 sliding kernel does not take this launch");
+  if (want == MPPCA_GRAM_SLIDE && !(cols && slide_fits(g)))
+    return set_err(MPPCA_EUNSUPPORTED, "gram64: the FP64 sliding kernel does not take this launch");
   if (lntb > 0) {
     const int ix0 = (int)(b0 / per_ix), nix = (int)(nb / per_ix);
     const bool two = limb_rows(g.px * g.py) > 64;  // tile rows of 128 voxels
@@ -749,7 +766,7 @@ int launch_gram64(const float* xc, double* gram, const Geom& g, long long b0, lo
                    : launch_gram_limb_t<4, 1>(xc, gram, g, ix0, nix, st);
     }
   }
-  if (cols && slide_fits(g)) {
+  if (cols && slide_fits(g) && want != MPPCA_GRAM_PATCH) {
     const int ix0 = (int)(b0 / per_ix), nix = (int)(nb / per_ix);
     switch (ntb_of(g.T)) {
       case 1: return launch_gram_slide_t<1>(xc, gram, g, ix0, nix, st);

@@ -380,7 +380,7 @@ extern "C" int mppca_denoise_host(const float* vol, float* out, int X, int Y, in
   int ix_done = 0, x_done = 0;
   const long long per_ix = (long long)g.ncy * g.ncz;
   const long long ff = pipelined ? factor_floats(T) : 0;
-  const int tw = recon_factor_tile_x();
+  const int tw = recon_factor_tile_x(g);
   // Reconstruction of planes [x_done, x_hi) on stream st, in npiece pieces,
   // each downloaded once it has run.
   auto finish_planes = [&](int x_hi, hipStream_t st, int npiece) -> int {
@@ -435,7 +435,7 @@ extern "C" int mppca_denoise_host(const float* vol, float* out, int X, int Y, in
       if ((rc = launch_gram64(w.xc, (double*)w.gram + b0 * T * T, g, b0, nb, S))) break;
       if ((rc = launch_eig64_mp((double*)w.gram + b0 * T * T, nb, T, g.N, nullptr,
                                 w.fac + b0 * ff, rk + b0, w.weight + b0, mg + b0, nullptr,
-                                inf + b0, S)))
+                                inf + b0, S, g.tune.eig)))
         break;
       ix_done = ix_new;
     }
@@ -482,5 +482,45 @@ extern "C" int mppca_denoise_host(const float* vol, float* out, int X, int Y, in
   if ((rc = hip_ok(e1, "output stream")) || (rc = hip_ok(e2, "input stream")) ||
       (rc = hip_ok(e3, "compute stream")))
     return rc;
+  return MPPCA_OK;
+}
+
+extern "C" int mppca_copy_d2h(void* dst_host, const void* src_dev, int64_t bytes,
+                              void* after_event) {
+  if (!dst_host || !src_dev || bytes < 0) return set_err(MPPCA_EINVAL, "copy_d2h: bad arguments");
+  if (after_event) {
+    const hipError_t e = hipEventSynchronize((hipEvent_t)after_event);
+    if (e != hipSuccess) return set_err((int)e, "copy_d2h: event: %s", hipGetErrorString(e));
+  }
+  if (bytes == 0) return MPPCA_OK;
+  hsa_agent_t gpu{0}, cpu{0};
+  if (!dma_agents(src_dev, dst_host, &gpu, &cpu)) {  // memory HSA does not know: the runtime copy
+    const hipError_t e = hipMemcpy(dst_host, src_dev, (size_t)bytes, hipMemcpyDeviceToHost);
+    return e == hipSuccess ? MPPCA_OK : set_err((int)e, "copy_d2h: %s", hipGetErrorString(e));
+  }
+  hsa_signal_t sig;
+  if (hsa_signal_create(1, 0, nullptr, &sig) != HSA_STATUS_SUCCESS)
+    return set_err(MPPCA_EINVAL, "copy_d2h: cannot create an HSA signal");
+  int rc = MPPCA_OK;
+  if (hsa_amd_memory_async_copy(dst_host, cpu, src_dev, gpu, (size_t)bytes, 0, nullptr, sig) !=
+      HSA_STATUS_SUCCESS) {
+    rc = set_err(MPPCA_EINVAL, "copy_d2h: hsa_amd_memory_async_copy refused the copy");
+  } else {
+    hsa_signal_value_t v;
+    while ((v = hsa_signal_wait_scacquire(sig, HSA_SIGNAL_CONDITION_LT, 1, UINT64_MAX,
+                                          HSA_WAIT_STATE_ACTIVE)) > 0) {
+    }
+    if (v != 0) rc = set_err(MPPCA_EINVAL, "copy_d2h: the DMA copy reported an error");
+  }
+  (void)hsa_signal_destroy(sig);
+  return rc;
+}
+
+extern "C" int mppca_host_copy(void* dst, const void* src, int64_t bytes, int nthreads) {
+  if (!dst || !src || bytes < 0 || nthreads < 1 || nthreads > 256)
+    return set_err(MPPCA_EINVAL, "host_copy: bad arguments");
+  std::vector<RangeCopier::Range> r;
+  r.push_back({(char*)dst, (const char*)src, (size_t)bytes, {}});
+  RangeCopier c(std::move(r), nthreads);
   return MPPCA_OK;
 }

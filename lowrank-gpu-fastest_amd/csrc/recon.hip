@@ -24,9 +24,12 @@ __global__ __launch_bounds__(256) void k_recon(const float* __restrict__ xc,
   extern __shared__ float smem[];
   float* pf = smem;                      // KQ * NTB * 64 floats
   int* s_off = (int*)(pf + TP * TP);     // N ints
-  const long long bl = blockIdx.x;
-  if (bl >= nb) return;
+  const long long bl = blockIdx.x;  // over the nb patches of each of the batch's volumes
+  if (bl >= nb * g.nvol) return;
   const int T = g.T;
+  const long long vb = bl / nb;
+  xc += vb * g.nvox * T;
+  acc += vb * g.nvox * T;
   const float* Pb = proj + bl * (long long)T * T;
   for (int idx = threadIdx.x; idx < KQ * NTB * 64; idx += blockDim.x) {
     const int l = idx & 63, st = idx >> 6;
@@ -37,7 +40,7 @@ __global__ __launch_bounds__(256) void k_recon(const float* __restrict__ xc,
   fill_offsets(s_off, g);
   __syncthreads();
   const float w = weight[bl];
-  const long long base = patch_base(g, b0 + bl);
+  const long long base = patch_base(g, b0 + (bl - vb * nb));
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int i = lane & 15, kq = lane >> 4;
   const int nrb = (g.N + 15) / 16;
@@ -85,7 +88,8 @@ static int launch_recon_t(const float* xc, const float* proj, const float* weigh
   // NTB = 7 once N > 3824
   const int rc = prepare_lds((const void*)k_recon<NTB>, lds);
   if (rc) return rc;
-  hipLaunchKernelGGL((k_recon<NTB>), dim3((unsigned)nb), dim3(256), lds, st, xc, proj, weight, acc,
+  if (nb * g.nvol >= (1LL << 31)) return set_err(MPPCA_EUNSUPPORTED, "recon: too many patches");
+  hipLaunchKernelGGL((k_recon<NTB>), dim3((unsigned)(nb * g.nvol)), dim3(256), lds, st, xc, proj, weight, acc,
                      g, b0, nb);
   return launch_status("recon kernel");
 }
@@ -156,8 +160,19 @@ void k_recon_lr(const float* __restrict__ xc,
   // Workgroups are dealt round-robin to the 8 XCDs; XCD k takes the contiguous
   // run [k*per_xcd, (k+1)*per_xcd) of block tiles, so neighbouring tiles, which
   // share most of their patches, share one L2.
-  const int bt = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
-  if (bt >= nbx * nby * nbz) return;
+  const int btb = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
+  const int ntile = nbx * nby * nbz;  // tiles of one volume of the batch
+  if (btb >= ntile * g.nvol) return;
+  const int vb = btb / ntile, bt = btb - vb * ntile;
+  {
+    const long long vf = (long long)vb * g.nvox * g.T;
+    xc += vf;
+    out += vf;
+    mean += (long long)vb * g.nvox;
+    fac += (long long)vb * g.P * factor_floats(g.T);
+    rank += (long long)vb * g.P;
+    weight += (long long)vb * g.P;
+  }
   const int bz = bt % nbz, byx = bt / nbz, by = byx % nby, bx = bx0 + byx / nby;
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int x0 = bx * SX, y0 = by * 4, z0 = (bz * 4 + wave) * 4;
@@ -420,7 +435,7 @@ static int launch_recon_lr_t(const float* xc, const float* fac, const int* rank,
   const int bx0 = x0 / SX, bx1 = (x1 + SX - 1) / SX;
   const int nbx = bx1 - bx0, nby = (g.Y + 3) / 4, nbz = (g.Z + 15) / 16;
   if (nbx <= 0) return MPPCA_OK;
-  const long long nbt = (long long)nbx * nby * nbz;
+  const long long nbt = (long long)nbx * nby * nbz * g.nvol;
   if (nbt > (1LL << 30)) return set_err(MPPCA_EUNSUPPORTED, "recon: volume too large");
   const int per_xcd = (int)((nbt + 7) / 8);
   if (g.P >= (1LL << 31)) return set_err(MPPCA_EUNSUPPORTED, "recon: 2^31 patches or more");
@@ -431,22 +446,34 @@ static int launch_recon_lr_t(const float* xc, const float* fac, const int* rank,
   return launch_status("recon (factors) kernel");
 }
 
-int recon_factor_tile_x() { return 2; }
+int recon_factor_tile_x(const Geom& g) { return g.tune.recon_sx == 0 ? 2 : g.tune.recon_sx; }
+
+template <int SX>
+static int launch_recon_lr_sx(const float* xc, const float* fac, const int* rank,
+                              const float* weight, const float* mean, float* out, const Geom& g,
+                              int x0, int x1, hipStream_t st) {
+  switch (ntb_of(g.T)) {
+    case 1: return launch_recon_lr_t<1, SX>(xc, fac, rank, weight, mean, out, g, x0, x1, st);
+    case 2: return launch_recon_lr_t<2, SX>(xc, fac, rank, weight, mean, out, g, x0, x1, st);
+    case 3: return launch_recon_lr_t<3, SX>(xc, fac, rank, weight, mean, out, g, x0, x1, st);
+    case 4: return launch_recon_lr_t<4, SX>(xc, fac, rank, weight, mean, out, g, x0, x1, st);
+    default:
+      return set_err(MPPCA_EUNSUPPORTED, "recon (factors): T = %d above 64", g.T);
+  }
+}
 
 int launch_recon_factors(const float* xc, const float* fac, const int* rank, const float* weight,
                          const float* mean, float* out, const Geom& g, int x0, int x1,
                          hipStream_t st) {
-  if (x0 < 0 || x1 > g.X || x0 % recon_factor_tile_x() != 0 ||
-      (x1 != g.X && x1 % recon_factor_tile_x() != 0))
+  // x-slabs per wavefront: 2 unless the tile schedule says otherwise
+  const int sx = g.tune.recon_sx == 0 ? 2 : g.tune.recon_sx;
+  if (sx != 1 && sx != 2 && sx != 4)
+    return set_err(MPPCA_EINVAL, "recon (factors): %d x-slabs per wavefront (1, 2 or 4)", sx);
+  if (x0 < 0 || x1 > g.X || x0 % sx != 0 || (x1 != g.X && x1 % sx != 0))
     return set_err(MPPCA_EINVAL, "recon (factors): x range [%d, %d) not tile aligned", x0, x1);
-  switch (ntb_of(g.T)) {
-    case 1: return launch_recon_lr_t<1, 2>(xc, fac, rank, weight, mean, out, g, x0, x1, st);
-    case 2: return launch_recon_lr_t<2, 2>(xc, fac, rank, weight, mean, out, g, x0, x1, st);
-    case 3: return launch_recon_lr_t<3, 2>(xc, fac, rank, weight, mean, out, g, x0, x1, st);
-    case 4: return launch_recon_lr_t<4, 2>(xc, fac, rank, weight, mean, out, g, x0, x1, st);
-    default:
-      return set_err(MPPCA_EUNSUPPORTED, "recon (factors): T = %d above 64", g.T);
-  }
+  if (sx == 1) return launch_recon_lr_sx<1>(xc, fac, rank, weight, mean, out, g, x0, x1, st);
+  if (sx == 4) return launch_recon_lr_sx<4>(xc, fac, rank, weight, mean, out, g, x0, x1, st);
+  return launch_recon_lr_sx<2>(xc, fac, rank, weight, mean, out, g, x0, x1, st);
 }
 
 }  // namespace mppca

@@ -129,7 +129,14 @@ __global__ __launch_bounds__(256) void k_normalize(const float* acc, const float
                                                    const float* __restrict__ weight, float* out,
                                                    Geom g) {
   __shared__ float s_w[64], s_m[64];
-  const long long v0 = (long long)blockIdx.x * 64;
+  // blocks of volume vb of the batch follow those of volume vb - 1
+  const long long bpv = (g.nvox + 63) / 64;
+  const long long vb = blockIdx.x / bpv;
+  acc += vb * g.nvox * g.T;
+  out += vb * g.nvox * g.T;
+  mean += vb * g.nvox;
+  weight += vb * g.P;
+  const long long v0 = (long long)(blockIdx.x - vb * bpv) * 64;
   if (threadIdx.x < 64) {
     const long long v = v0 + threadIdx.x;
     float W = 0.f, mv = 0.f;
@@ -176,7 +183,9 @@ __global__ __launch_bounds__(256) void k_normalize(const float* acc, const float
 
 int launch_normalize(const float* acc, const float* mean, const float* weight, float* out,
                      const Geom& g, hipStream_t st) {
-  const unsigned blocks = (unsigned)((g.nvox + 63) / 64);
+  const long long nblk = (g.nvox + 63) / 64 * g.nvol;
+  if (nblk >= (1LL << 31)) return set_err(MPPCA_EUNSUPPORTED, "normalize: volume too large");
+  const unsigned blocks = (unsigned)nblk;
   hipLaunchKernelGGL(k_normalize, dim3(blocks), dim3(256), 0, st, acc, mean, weight, out, g);
   return launch_status("normalize kernel");
 }
