@@ -23,9 +23,22 @@
 //    guards near-equal eigenvalues.
 // 5. Back-transform x = H_0 H_1 ... H_{T-3} z and P_b = sum_j x_j x_j^T,
 //    stored as float32 for the FP32 reconstruction.
+#include <type_traits>
+#include <utility>
+
 #include "common.h"
 
 namespace mppca {
+
+// f(integral_constant<int, I>) for I = 0 .. N-1, unrolled at compile time.
+template <class F, int... I>
+__device__ __forceinline__ void static_for_impl(F& f, std::integer_sequence<int, I...>) {
+  (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+  static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
 
 // Eigenvectors computed concurrently (one lane each): 8, or 4 above T = 96 so
 // that the whole working set of a T = 128 matrix stays within 160 KB of LDS.
@@ -678,8 +691,11 @@ __global__ __launch_bounds__(64) void k_tri_grid(double* __restrict__ gram, int 
   // hold the matching rows and columns through ds_bpermute; A v is summed over
   // the 8 lanes of a grid row with DPP. Register blocks above mb are finished
   // (their v and w entries vanish) and are skipped at compile time.
-#pragma unroll
-  for (int mb = 0; mb < R; ++mb) {
+  // The register-block loop is unrolled at compile time through a generic
+  // lambda (a loop the compiler leaves rolled would index the register arrays
+  // at run time, which puts them in scratch memory).
+  auto block_step = [&](auto mbc) __attribute__((always_inline)) {
+    constexpr int mb = decltype(mbc)::value;
     for (int r0 = 0; r0 < 8; ++r0) {
       const int k = 8 * mb + r0;
       if (k + 2 >= T) break;
@@ -779,7 +795,8 @@ __global__ __launch_bounds__(64) void k_tri_grid(double* __restrict__ gram, int 
             a[m][n] = fma(-vr[m], wc[n], fma(-wr[m], vc[n], a[m][n]));
       }
     }
-  }
+  };
+  static_for<R>(block_step);
   // last 2x2 block: rows T-2 and T-1
   double dd2 = 0.0, dd1 = 0.0, ee2 = 0.0;
 #pragma unroll
@@ -1404,8 +1421,8 @@ int launch_eig64_mp(double* gram, long long nb, int T, int N, float* proj, float
     return set_err(MPPCA_EUNSUPPORTED, "eig64: factors need T <= %d (T = %d)", kFactorMaxT, T);
   if (variant != MPPCA_EIG_AUTO && variant != MPPCA_EIG_GRID && variant != MPPCA_EIG_LDS)
     return set_err(MPPCA_EINVAL, "eig64: unknown eigensolver layout %d", variant);
-  if (variant == MPPCA_EIG_GRID && T > 48)
-    return set_err(MPPCA_EUNSUPPORTED, "eig64: the lane-grid eigensolver takes T <= 48 (T = %d)",
+  if (variant == MPPCA_EIG_GRID && T > 64)
+    return set_err(MPPCA_EUNSUPPORTED, "eig64: the lane-grid eigensolver takes T <= 64 (T = %d)",
                    T);
   if (variant != MPPCA_EIG_LDS) {
     if (T <= 16)
@@ -1414,6 +1431,8 @@ int launch_eig64_mp(double* gram, long long nb, int T, int N, float* proj, float
       return launch_rows<32>(gram, nb, T, N, proj, fac, rank, weight, margin, lam_desc, info, st);
     if (T <= 48)
       return launch_rows<48>(gram, nb, T, N, proj, fac, rank, weight, margin, lam_desc, info, st);
+    if (T <= 64)
+      return launch_rows<64>(gram, nb, T, N, proj, fac, rank, weight, margin, lam_desc, info, st);
   }
   const size_t lds = tri_lds_bytes(T);
   int rc = prepare_lds((const void*)k_eig64_mp, lds);
