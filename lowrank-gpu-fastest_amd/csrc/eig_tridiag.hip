@@ -815,6 +815,168 @@ __global__ __launch_bounds__(64) void k_tri_grid(double* __restrict__ gram, int 
   if (gr == (T - 1) % 8 && gc == (T - 2) % 8) Gb[(T - 1) * T + T - 2] = ee2;
 }
 
+// Sum over the 16 lanes of a DPP row (every lane of the row gets it).
+__device__ __forceinline__ double row16_sum(double x) {
+  x = group8_sum(x);
+  return x + dpp64<kDppRowMirror>(x);
+}
+
+// ---------------------------------------------------------------------------
+// Tridiagonalisation for 64 < T <= 128 (TP = T rounded up to 16): the
+// reduction of k_tri_grid on a 16 x 16 lane grid of four wavefronts, so that
+// the matrix stays in registers (lane (gr, gc) holds A[gr + 16m][gc + 16n],
+// m, n < TP / 16: at most 64 doubles). Grid row gr is DPP row gr % 4 of
+// wavefront gr / 4, so a sum along a matrix row is a 16-lane DPP reduction;
+// what crosses wavefronts -- the Householder vector v, p = tau A v and the
+// partial sums of p.v -- goes through LDS, with two barriers per step. Writes
+// the hand-off of k_tri_grid (reflectors, tau, d and e in the Gram buffer).
+// ---------------------------------------------------------------------------
+template <int TP>
+__global__ __launch_bounds__(256) void k_tri_grid4(double* __restrict__ gram, int T,
+                                                   long long nb) {
+  constexpr int R = TP / 16;  // register rows / columns per lane of the 16 x 16 grid
+  __shared__ double s_v[TP];  // the step's Householder vector, by index
+  __shared__ double s_p[TP];  // p = tau A v of the step, by row
+  __shared__ double s_part[4];
+  __shared__ double s_tk;
+  const long long b = blockIdx.x;
+  if (b >= nb) return;
+  const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+  const int gr = t >> 4, gc = t & 15;  // lane-grid row and column
+  double* Gb = gram + b * (long long)T * T;
+  double a[R][R];
+  bool finite = true;
+#pragma unroll
+  for (int m = 0; m < R; ++m)
+#pragma unroll
+    for (int n = 0; n < R; ++n) {
+      const int i = gr + 16 * m, j = gc + 16 * n;
+      a[m][n] = (i < T && j < T) ? Gb[i * T + j] : 0.0;
+      finite = finite && isfinite(a[m][n]);
+    }
+  if (__syncthreads_or(!finite)) {  // the whole matrix is in registers: flag it
+    if (t == 0) Gb[(T - 2) * T] = NAN;
+    return;
+  }
+  // Step k = 16 mb + r0: row k is register row mb of grid row r0. Register
+  // blocks above mb are finished and skipped at compile time (the block loop
+  // is unrolled through a generic lambda).
+  auto block_step = [&](auto mbc) __attribute__((always_inline)) {
+    constexpr int mb = decltype(mbc)::value;
+    for (int r0 = 0; r0 < 16; ++r0) {
+      const int k = 16 * mb + r0;
+      if (k + 2 >= T) break;
+      if (wave == (r0 >> 2)) {  // the wavefront that holds row k
+        const int base = 16 * (r0 & 3);  // its DPP row
+        const double dk = lane_read64(a[mb][mb], base + r0);  // A[k][k]
+        constexpr int nb1 = mb + 1 < R ? mb + 1 : mb;  // column k+1 = 16(mb+1) when r0 = 15
+        const double asrc = r0 == 15 ? a[mb][nb1] : a[mb][mb];
+        const double alpha = lane_read64(asrc, base + ((r0 + 1) & 15));  // A[k][k+1]
+        double part = 0.0;
+#pragma unroll
+        for (int n = mb; n < R; ++n) {
+          const double xv = a[mb][n];
+          if (n >= mb + 2)
+            part = fma(xv, xv, part);
+          else
+            part = gc + 16 * n > k + 1 ? fma(xv, xv, part) : part;
+        }
+        const double ss = lane_read64(row16_sum(part), base);
+        // dlarfg, as k_tri_grid
+        double tk = 0.0, beta = alpha, sc = 0.0;
+        if (ss > 0.0) {
+          const double s2 = fma(alpha, alpha, ss);
+          double mu, rmu;
+          if (s2 >= 0x1p-900) {
+            rmu = rsq64(s2);
+            mu = s2 * rmu;
+          } else {
+            mu = sqrt(s2);
+            rmu = 1.0 / mu;
+          }
+          const double aa = fabs(alpha);
+          beta = alpha >= 0.0 ? -mu : mu;
+          tk = fma(aa, rmu, 1.0);
+          const double sa = rcp64(aa + mu);
+          sc = alpha >= 0.0 ? sa : -sa;
+        }
+        if (gr == r0) {  // v at columns gc + 16n, and reflector row k of the hand-off
+#pragma unroll
+          for (int n = 0; n < R; ++n) {
+            const int j = gc + 16 * n;
+            double v = 0.0;
+            if (n >= mb + 2)
+              v = a[mb][n] * sc;
+            else if (n >= mb)
+              v = j == k + 1 ? 1.0 : (j > k + 1 ? a[mb][n] * sc : 0.0);
+            s_v[j] = v;
+            if (j < T) Gb[k * T + j] = j == 0 ? tk : v;
+          }
+          if (gc == 0) {
+            s_tk = tk;
+            Gb[(T - 2) * T + k] = dk;
+            Gb[(T - 1) * T + k] = beta;
+          }
+        }
+      }
+      __syncthreads();
+      const double tk = s_tk;
+      double vc[R], vr[R];
+#pragma unroll
+      for (int n = 0; n < R; ++n) {
+        vc[n] = n >= mb ? s_v[gc + 16 * n] : 0.0;
+        vr[n] = n >= mb ? s_v[gr + 16 * n] : 0.0;
+      }
+      double pl[R];
+      if (tk != 0.0) {  // the same on every lane
+        double part2 = 0.0;
+#pragma unroll
+        for (int m = mb; m < R; ++m) {
+          double acc = 0.0;
+#pragma unroll
+          for (int n = mb; n < R; ++n) acc = fma(a[m][n], vc[n], acc);
+          acc = row16_sum(acc);
+          pl[m] = (m >= mb + 1 || gr + 16 * m > k) ? tk * acc : 0.0;
+          part2 = fma(pl[m], vr[m], part2);
+          if (gc == 0) s_p[gr + 16 * m] = pl[m];
+        }
+        // part2 is the same on the 16 lanes of a grid row: each row once (the
+        // lane reads stay outside the one-lane branch, so that every lane has
+        // computed the value read)
+        const double wsum = (lane_read64(part2, 0) + lane_read64(part2, 16)) +
+                            (lane_read64(part2, 32) + lane_read64(part2, 48));
+        if (lane == 0) s_part[wave] = wsum;
+      }
+      // also when tk = 0: no wavefront writes s_v again before every one has read it
+      __syncthreads();
+      if (tk != 0.0) {
+        const double K = 0.5 * tk * ((s_part[0] + s_part[1]) + (s_part[2] + s_part[3]));
+        double wr[R], wc[R];
+#pragma unroll
+        for (int m = mb; m < R; ++m) wr[m] = pl[m] - K * vr[m];
+#pragma unroll
+        for (int n = mb; n < R; ++n) wc[n] = s_p[gc + 16 * n] - K * vc[n];
+#pragma unroll
+        for (int m = mb; m < R; ++m)
+#pragma unroll
+          for (int n = mb; n < R; ++n)
+            a[m][n] = fma(-vr[m], wc[n], fma(-wr[m], vc[n], a[m][n]));
+      }
+    }
+  };
+  static_for<R>(block_step);
+  // last 2x2 block: rows T-2 and T-1
+#pragma unroll
+  for (int m = 0; m < R; ++m)
+#pragma unroll
+    for (int n = 0; n < R; ++n) {
+      const int i = gr + 16 * m, j = gc + 16 * n;
+      if (i == T - 2 && j == T - 2) Gb[(T - 2) * T + T - 2] = a[m][n];
+      if (i == T - 1 && j == T - 1) Gb[(T - 2) * T + T - 1] = a[m][n];
+      if (i == T - 1 && j == T - 2) Gb[(T - 1) * T + T - 2] = a[m][n];
+    }
+}
+
 // Where k_eig_vals leaves the eigenvalues of a patch for k_eig_vecs: the last
 // 2T 32-bit words of the patch's projector block, or of its factor block (Uw of
 // the last slot tile, written only for ranks above TP - 16, after the vector
@@ -841,6 +1003,7 @@ __global__ __launch_bounds__(64) void k_eig_vals(const double* __restrict__ gram
                                                    long long nb, float* __restrict__ proj,
                                                    float* __restrict__ fac) {
   constexpr int NS = vals_slots(TP);
+  constexpr int RC = (TP + 63) / 64;  // matrix rows per lane in the preparation
   __shared__ double s_ds[NS][TP], s_e2s[NS][TP];  // for the substituting recount
   __shared__ double s_pt[64];                     // n-section points
   __shared__ int s_cnt[64];                       // eigenvalues below each point
@@ -873,31 +1036,49 @@ __global__ __launch_bounds__(64) void k_eig_vals(const double* __restrict__ gram
   for (long long bs = bfirst; bs <= blast; ++bs) {
     const int sl = (int)(bs - bfirst);
     const double* Gs = gram + bs * (long long)T * T;
-    double dl = 0.0, el = 0.0;
+    // rows lane + 64 c of the matrix (two per lane above T = 64)
+    double dl[RC], el[RC];
     bool fin = true;
-    if (lane < T) {
-      dl = Gs[(T - 2) * T + lane];
-      el = lane + 1 < T ? Gs[(T - 1) * T + lane] : 0.0;
-      fin = isfinite(dl) && isfinite(el);
-      s_ds[sl][lane] = dl;
+#pragma unroll
+    for (int c = 0; c < RC; ++c) {
+      const int i = lane + 64 * c;
+      dl[c] = 0.0;
+      el[c] = 0.0;
+      if (i < T) {
+        dl[c] = Gs[(T - 2) * T + i];
+        el[c] = i + 1 < T ? Gs[(T - 1) * T + i] : 0.0;
+        fin = fin && isfinite(dl[c]) && isfinite(el[c]);
+        s_ds[sl][i] = dl[c];
+      }
     }
     const bool fin_all = __any(!fin) == 0;
     __syncthreads();
-    if (lane + 1 < T) {
-      const double thr = 8.8817841970012523e-16 * (fabs(dl) + fabs(s_ds[sl][lane + 1]));
-      if (fabs(el) <= thr) el = 0.0;
+#pragma unroll
+    for (int c = 0; c < RC; ++c) {
+      const int i = lane + 64 * c;
+      if (i + 1 < T) {
+        const double thr = 8.8817841970012523e-16 * (fabs(dl[c]) + fabs(s_ds[sl][i + 1]));
+        if (fabs(el[c]) <= thr) el[c] = 0.0;
+      }
     }
-    s_e2s[sl][lane < TP ? lane : 0] = 0.0;  // scratch for the left neighbour's |e|
-    __syncthreads();
-    if (lane < T) s_e2s[sl][lane] = fabs(el);
+    // |e| (zero past T), for the left neighbour's Gershgorin radius
+#pragma unroll
+    for (int c = 0; c < RC; ++c) {
+      const int i = lane + 64 * c;
+      if (i < TP) s_e2s[sl][i] = i < T ? fabs(el[c]) : 0.0;
+    }
     __syncthreads();
     double lo_g = INFINITY, hi_g = -INFINITY, emax2 = 0.0;
-    if (lane < T) {
-      const double elo = lane > 0 ? s_e2s[sl][lane - 1] : 0.0;
-      const double ehi = fabs(el);
-      lo_g = dl - elo - ehi;
-      hi_g = dl + elo + ehi;
-      emax2 = el * el;
+#pragma unroll
+    for (int c = 0; c < RC; ++c) {
+      const int i = lane + 64 * c;
+      if (i < T) {
+        const double elo = i > 0 ? s_e2s[sl][i - 1] : 0.0;
+        const double ehi = fabs(el[c]);
+        lo_g = fmin(lo_g, dl[c] - elo - ehi);
+        hi_g = fmax(hi_g, dl[c] + elo + ehi);
+        emax2 = fmax(emax2, el[c] * el[c]);
+      }
     }
     lo_g = wave_min(lo_g);
     hi_g = wave_max(hi_g);
@@ -909,11 +1090,15 @@ __global__ __launch_bounds__(64) void k_eig_vals(const double* __restrict__ gram
     const double bsc = bnorm > 0.0 ? bnorm : 1.0;
     const double inv = 1.0 / bsc;
     __syncthreads();
-    if (lane < TP) {
-      // scaled, padded with 4 from row T on (a padded row multiplies every
-      // minor by 4 - x > 0 and couples to nothing, so it never changes a sign)
-      s_ds[sl][lane] = lane < T ? dl * inv : 4.0;
-      s_e2s[sl][lane] = lane + 1 < T ? (el * el) * (inv * inv) : 0.0;
+#pragma unroll
+    for (int c = 0; c < RC; ++c) {
+      const int i = lane + 64 * c;
+      if (i < TP) {
+        // scaled, padded with 4 from row T on (a padded row multiplies every
+        // minor by 4 - x > 0 and couples to nothing, so it never changes a sign)
+        s_ds[sl][i] = i < T ? dl[c] * inv : 4.0;
+        s_e2s[sl][i] = i + 1 < T ? (el[c] * el[c]) * (inv * inv) : 0.0;
+      }
     }
     if (lane == 0) {
       s_lo[sl] = (lo_g - slack) * inv;
@@ -1096,7 +1281,7 @@ __global__ __launch_bounds__(64) void k_eig_vals(const double* __restrict__ gram
 // group, back-transform on the lane grid, factors or projector rows. Its own
 // kernel so that it runs at the occupancy of its few registers.
 template <int TP>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_eig_vecs(const double* __restrict__ gram, int T, int N,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(TP > 64 ? 2 : 4))) void k_eig_vecs(const double* __restrict__ gram, int T, int N,
                                                  long long nb, float* __restrict__ proj,
                                                  float* __restrict__ fac,
                                                  int* __restrict__ rank,
@@ -1106,6 +1291,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                                                  int* __restrict__ info) {
   constexpr int LD = TP + 1;
   constexpr int R = TP / 8;  // register columns per lane of the 8 x 8 lane grid
+  constexpr int RC = (TP + 63) / 64;  // matrix rows per lane where lanes are rows
   __shared__ double s_d[TP], s_e[TP], s_e2[TP], s_tau[TP];
   __shared__ double s_lam[TP];
   __shared__ double s_dm[kRowGroup * LD];
@@ -1121,22 +1307,26 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
   const long long b = blockIdx.x;
   if (b >= nb) return;
   const int lane = threadIdx.x;
-  const bool row_ok = lane < T;
   const int gr = lane >> 3, gc = lane & 7;  // lane-grid row and column
   const double* Gb = gram + b * (long long)T * T;
   float* Pb = proj ? proj + b * (long long)T * T : nullptr;
   float* Fb = fac ? fac + b * (2LL * TP * TP) : nullptr;  // TP = 16*ceil(T/16)
   bool finite = true;
-  unsigned lam_lo = 0u, lam_hi = 0u;
-  if (lane < T) {
-    // the eigenvalues left by k_eig_vals (largest first), loaded with the rest
-    const unsigned* sc = lam_scratch(Pb, Fb, T, TP);
-    lam_lo = sc[2 * lane];
-    lam_hi = sc[2 * lane + 1];
-    s_d[lane] = Gb[(T - 2) * T + lane];
-    s_e[lane] = lane + 1 < T ? Gb[(T - 1) * T + lane] : 0.0;
-    if (lane + 2 < T) s_tau[lane] = Gb[lane * T];
-    finite = isfinite(s_d[lane]) && isfinite(s_e[lane]);
+  unsigned lam_lo[RC], lam_hi[RC];
+#pragma unroll
+  for (int c = 0; c < RC; ++c) {
+    const int i = lane + 64 * c;
+    lam_lo[c] = lam_hi[c] = 0u;
+    if (i < T) {
+      // the eigenvalues left by k_eig_vals (largest first), loaded with the rest
+      const unsigned* sc = lam_scratch(Pb, Fb, T, TP);
+      lam_lo[c] = sc[2 * i];
+      lam_hi[c] = sc[2 * i + 1];
+      s_d[i] = Gb[(T - 2) * T + i];
+      s_e[i] = i + 1 < T ? Gb[(T - 1) * T + i] : 0.0;
+      if (i + 2 < T) s_tau[i] = Gb[i * T];
+      finite = finite && isfinite(s_d[i]) && isfinite(s_e[i]);
+    }
   }
   if (__any(!finite)) {  // non-finite input: no rank, no projector, flagged
     if (Pb)
@@ -1152,10 +1342,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
       for (int c = lane; c < T; c += 64) lam_desc[b * T + c] = NAN;
     return;
   }
-  if (lane < T) {
-    const double lam = __longlong_as_double((long long)lam_lo | ((long long)lam_hi << 32));
-    s_lam[lane] = lam;
-    s_lamf[lane] = (float)lam;
+#pragma unroll
+  for (int c = 0; c < RC; ++c) {
+    const int i = lane + 64 * c;
+    if (i < T) {
+      const double lam =
+          __longlong_as_double((long long)lam_lo[c] | ((long long)lam_hi[c] << 32));
+      s_lam[i] = lam;
+      s_lamf[i] = (float)lam;
+    }
   }
   __syncthreads();
   if (lam_desc)
@@ -1174,19 +1369,38 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
     }
   }
   // the same split and scale as k_eig_vals (the data are the same)
-  if (lane + 1 < T) {
-    const double thr = 8.8817841970012523e-16 * (fabs(s_d[lane]) + fabs(s_d[lane + 1]));
-    if (fabs(s_e[lane]) <= thr) s_e[lane] = 0.0;
+  double es[RC];
+#pragma unroll
+  for (int c = 0; c < RC; ++c) {
+    const int i = lane + 64 * c;
+    es[c] = 0.0;
+    if (i < T) {
+      es[c] = s_e[i];
+      if (i + 1 < T) {
+        const double thr = 8.8817841970012523e-16 * (fabs(s_d[i]) + fabs(s_d[i + 1]));
+        if (fabs(es[c]) <= thr) es[c] = 0.0;
+      }
+    }
+  }
+  __syncthreads();  // every split test has read the unsplit e
+#pragma unroll
+  for (int c = 0; c < RC; ++c) {
+    const int i = lane + 64 * c;
+    if (i < T) s_e[i] = es[c];
   }
   __syncthreads();
   double lo_g = INFINITY, hi_g = -INFINITY, emax2 = 0.0;
-  if (lane < T) {
-    const double el = lane > 0 ? fabs(s_e[lane - 1]) : 0.0;
-    const double er = fabs(s_e[lane]);
-    lo_g = s_d[lane] - el - er;
-    hi_g = s_d[lane] + el + er;
-    emax2 = s_e[lane] * s_e[lane];
-    s_e2[lane] = emax2;
+#pragma unroll
+  for (int c = 0; c < RC; ++c) {
+    const int i = lane + 64 * c;
+    if (i < T) {
+      const double el = i > 0 ? fabs(s_e[i - 1]) : 0.0;
+      const double er = fabs(s_e[i]);
+      lo_g = fmin(lo_g, s_d[i] - el - er);
+      hi_g = fmax(hi_g, s_d[i] + el + er);
+      emax2 = fmax(emax2, s_e[i] * s_e[i]);
+      s_e2[i] = s_e[i] * s_e[i];
+    }
   }
   lo_g = wave_min(lo_g);
   hi_g = wave_max(hi_g);
@@ -1198,9 +1412,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
 
   // ---- 4./5. top-p eigenvectors, back-transform, projector rows -----------------
   const double pvfloor = fmax(u * bnorm, 2.2250738585072014e-308);
-  float* dst = Pb ? Pb + lane * T : nullptr;  // row `lane` of P_b
-  if (p == 0 && row_ok && dst)
-    for (int c = 0; c < T; ++c) dst[c] = 0.f;
+  if (p == 0 && Pb)
+    for (int idx = lane; idx < T * T; idx += 64) Pb[idx] = 0.f;
   for (int g0 = 0; g0 < p; g0 += kRowGroup) {
     const int gn = p - g0 < kRowGroup ? p - g0 : kRowGroup;
     // Twisted factorisation of T - lambda I for vector g = gr of the group on
@@ -1311,14 +1524,32 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
     // modified Gram-Schmidt inside the group (guards near-equal eigenvalues)
     for (int a2 = 1; a2 < gn; ++a2) {
       for (int c2 = 0; c2 < a2; ++c2) {
-        const double part = lane < T ? s_z[a2 * LD + lane] * s_z[c2 * LD + lane] : 0.0;
+        double part = 0.0;
+#pragma unroll
+        for (int c = 0; c < RC; ++c) {
+          const int i = lane + 64 * c;
+          if (i < T) part = fma(s_z[a2 * LD + i], s_z[c2 * LD + i], part);
+        }
         const double dot = wave_sum(part);
-        if (lane < T) s_z[a2 * LD + lane] -= dot * s_z[c2 * LD + lane];
+#pragma unroll
+        for (int c = 0; c < RC; ++c) {
+          const int i = lane + 64 * c;
+          if (i < T) s_z[a2 * LD + i] -= dot * s_z[c2 * LD + i];
+        }
         __syncthreads();
       }
-      const double part = lane < T ? s_z[a2 * LD + lane] * s_z[a2 * LD + lane] : 0.0;
+      double part = 0.0;
+#pragma unroll
+      for (int c = 0; c < RC; ++c) {
+        const int i = lane + 64 * c;
+        if (i < T) part = fma(s_z[a2 * LD + i], s_z[a2 * LD + i], part);
+      }
       const double sc2 = 1.0 / sqrt(wave_sum(part));
-      if (lane < T) s_z[a2 * LD + lane] *= sc2;
+#pragma unroll
+      for (int c = 0; c < RC; ++c) {
+        const int i = lane + 64 * c;
+        if (i < T) s_z[a2 * LD + i] *= sc2;
+      }
       __syncthreads();
     }
     // back-transform x = H_0 (H_1 (... H_{T-3} z)) on the lane grid: lane
@@ -1332,26 +1563,37 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
     for (int n = 0; n < R; ++n)
       zr[n] = (gr < gn && gc + 8 * n < T) ? s_z[gr * LD + gc + 8 * n] : 0.0;
     __syncthreads();
-    // reflector rows klo .. kc of a chunk, column `lane` (0 at column 0, where
-    // tau sits, and past T): the next chunk's rows are loaded into registers
-    // while the current chunk is applied. Unconditional, clamped loads.
-    const int cl = lane > 0 && lane < T ? lane : T - 1;
-    double hn[kHChunk];
+    // reflector rows klo .. kc of a chunk, columns lane + 64 c (0 at column 0,
+    // where tau sits, and past T): the next chunk's rows are loaded into
+    // registers while the current chunk is applied. Unconditional, clamped loads.
+    int cl[RC];
+#pragma unroll
+    for (int c = 0; c < RC; ++c) {
+      const int i = lane + 64 * c;
+      cl[c] = i > 0 && i < T ? i : T - 1;
+    }
+    double hn[RC][kHChunk];
     auto fetch_chunk = [&](int kc) {
 #pragma unroll
-      for (int r = 0; r < kHChunk; ++r) {
-        const int row = kc - kHChunk + 1 + r;
-        hn[r] = Gb[(row > 0 ? row : 0) * T + cl];
-      }
+      for (int c = 0; c < RC; ++c)
+#pragma unroll
+        for (int r = 0; r < kHChunk; ++r) {
+          const int row = kc - kHChunk + 1 + r;
+          hn[c][r] = Gb[(row > 0 ? row : 0) * T + cl[c]];
+        }
     };
     fetch_chunk(T - 3);
     for (int kc = T - 3; kc >= 0; kc -= kHChunk) {
       const int klo = kc - kHChunk + 1 > 0 ? kc - kHChunk + 1 : 0;
       const int r0 = klo - (kc - kHChunk + 1);  // first register row of the chunk
-      if (lane < TP)
 #pragma unroll
-        for (int r = 0; r < kHChunk; ++r)
-          if (r >= r0) s_h[(r - r0) * TP + lane] = lane > 0 && lane < T ? hn[r] : 0.0;
+      for (int c = 0; c < RC; ++c) {
+        const int i = lane + 64 * c;
+        if (i < TP)
+#pragma unroll
+          for (int r = 0; r < kHChunk; ++r)
+            if (r >= r0) s_h[(r - r0) * TP + i] = i > 0 && i < T ? hn[c][r] : 0.0;
+      }
       __syncthreads();
       fetch_chunk(kc - kHChunk > 0 ? kc - kHChunk : 0);  // rows of the next chunk
       for (int k = kc; k >= klo; --k) {
@@ -1378,19 +1620,24 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
     __syncthreads();
     if (Fb) write_factors(Fb, s_z, LD, g0, gn, p, T, TP, 1.0 / (2.0 + (double)p), lane);
     // P[l][c] (+)= sum_g x_g[l] x_g[c]; rank > 8 accumulates over passes in
-    // global memory (each lane owns its row, so no other lane touches it)
-    if (row_ok && dst) {
-      double xl[kRowGroup];
+    // global memory (each lane owns its rows, so no other lane touches them)
+    if (Pb)
 #pragma unroll
-      for (int g = 0; g < kRowGroup; ++g) xl[g] = g < gn ? s_z[g * LD + lane] : 0.0;
-      for (int c = 0; c < T; ++c) {
-        double acc = g0 == 0 ? 0.0 : (double)dst[c];
+      for (int rr = 0; rr < RC; ++rr) {
+        const int l = lane + 64 * rr;
+        if (l >= T) continue;
+        float* dst = Pb + l * T;  // row l of P_b
+        double xl[kRowGroup];
 #pragma unroll
-        for (int g = 0; g < kRowGroup; ++g)
-          if (g < gn) acc = fma(xl[g], s_z[g * LD + c], acc);
-        dst[c] = (float)acc;
+        for (int g = 0; g < kRowGroup; ++g) xl[g] = g < gn ? s_z[g * LD + l] : 0.0;
+        for (int c = 0; c < T; ++c) {
+          double acc = g0 == 0 ? 0.0 : (double)dst[c];
+#pragma unroll
+          for (int g = 0; g < kRowGroup; ++g)
+            if (g < gn) acc = fma(xl[g], s_z[g * LD + c], acc);
+          dst[c] = (float)acc;
+        }
       }
-    }
     __syncthreads();
   }
   if (Fb) zero_factor_slots(Fb, p, TP, lane);
@@ -1400,7 +1647,12 @@ template <int TP>
 static int launch_rows(double* gram, long long nb, int T, int N, float* proj, float* fac,
                        int* rank, float* weight, float* margin, float* lam_desc, int* info,
                        hipStream_t st) {
-  hipLaunchKernelGGL((k_tri_grid<TP>), dim3((unsigned)nb), dim3(64), 0, st, gram, T, nb);
+  if (TP <= 64)  // one wavefront per matrix; above, four
+    hipLaunchKernelGGL((k_tri_grid<TP <= 64 ? TP : 16>), dim3((unsigned)nb), dim3(64), 0, st,
+                       gram, T, nb);
+  else
+    hipLaunchKernelGGL((k_tri_grid4<TP <= 64 ? 80 : TP>), dim3((unsigned)nb), dim3(256), 0, st,
+                       gram, T, nb);
   int rc = launch_status("tridiagonalisation kernel");
   if (rc) return rc;
   // k_eig_vals hands the eigenvalues over in the projector or factor block
@@ -1421,9 +1673,7 @@ int launch_eig64_mp(double* gram, long long nb, int T, int N, float* proj, float
     return set_err(MPPCA_EUNSUPPORTED, "eig64: factors need T <= %d (T = %d)", kFactorMaxT, T);
   if (variant != MPPCA_EIG_AUTO && variant != MPPCA_EIG_GRID && variant != MPPCA_EIG_LDS)
     return set_err(MPPCA_EINVAL, "eig64: unknown eigensolver layout %d", variant);
-  if (variant == MPPCA_EIG_GRID && T > 64)
-    return set_err(MPPCA_EUNSUPPORTED, "eig64: the lane-grid eigensolver takes T <= 64 (T = %d)",
-                   T);
+
   if (variant != MPPCA_EIG_LDS) {
     if (T <= 16)
       return launch_rows<16>(gram, nb, T, N, proj, fac, rank, weight, margin, lam_desc, info, st);
@@ -1433,6 +1683,13 @@ int launch_eig64_mp(double* gram, long long nb, int T, int N, float* proj, float
       return launch_rows<48>(gram, nb, T, N, proj, fac, rank, weight, margin, lam_desc, info, st);
     if (T <= 64)
       return launch_rows<64>(gram, nb, T, N, proj, fac, rank, weight, margin, lam_desc, info, st);
+    if (T <= 80)
+      return launch_rows<80>(gram, nb, T, N, proj, fac, rank, weight, margin, lam_desc, info, st);
+    if (T <= 96)
+      return launch_rows<96>(gram, nb, T, N, proj, fac, rank, weight, margin, lam_desc, info, st);
+    if (T <= 112)
+      return launch_rows<112>(gram, nb, T, N, proj, fac, rank, weight, margin, lam_desc, info, st);
+    return launch_rows<128>(gram, nb, T, N, proj, fac, rank, weight, margin, lam_desc, info, st);
   }
   const size_t lds = tri_lds_bytes(T);
   int rc = prepare_lds((const void*)k_eig64_mp, lds);
