@@ -250,6 +250,34 @@ def validate(vol, clean, res, info, patch, stride, workers):
     return out, oracle_s
 
 
+def serve_run(pkg, volumes, rate_hz, n_requests, burst_n=None):
+    """Continuous-batching server (BatchServer) over ``volumes`` (a list of
+    (vol, patch, stride)): burst throughput (every request submitted at once)
+    and latency under Poisson arrivals at ``rate_hz`` (arrival to the result in
+    host memory)."""
+    burst_n = burst_n or max(24, 4 * len(volumes))
+    with pkg.BatchServer() as srv:
+        for f in [srv.submit(v, p, s, keep=False) for v, p, s in volumes for _ in range(2)]:
+            f.result()  # lanes allocated, graphs captured
+        t0 = time.perf_counter()
+        fs = [srv.submit(*volumes[k % len(volumes)], keep=False) for k in range(burst_n)]
+        for f in fs:
+            f.result()
+        burst_vps = burst_n / (time.perf_counter() - t0)
+        b0, v0 = srv.batches, srv.batched_volumes
+        r = pkg.poisson_run(srv, volumes, rate_hz=rate_hz, n_requests=n_requests, seed=0)
+        nb, nv = srv.batches - b0, srv.batched_volumes - v0
+    return {"server": "BatchServer (continuous batching; upload, compute, download stages "
+                      "overlapped across requests)",
+            "arrivals": "poisson", "rate_hz": rate_hz, "requests": n_requests,
+            "p50_ms": round(r["latency_ms"]["p50"], 3), "p99_ms": round(r["latency_ms"]["p99"], 3),
+            "max_ms": round(r["latency_ms"]["max"], 3),
+            "volumes_per_s": round(r["volumes_per_s"], 2),
+            "burst_volumes_per_s": round(burst_vps, 2), "burst_requests": burst_n,
+            "mean_batch": round(nv / max(nb, 1), 2),
+            "latency": "arrival to host result, host->host"}
+
+
 # ---- HBM traffic: rocprofv3 --pmc child runs ---------------------------------
 def pmc_child(args):
     """Body of a profiled child run: two denoises, then an elementwise kernel
@@ -387,7 +415,10 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=100.0,
                     help="Poisson arrival rate (volumes/s) of the fixed-load serving run")
     ap.add_argument("--serve-requests", type=int, default=200)
-    ap.add_argument("--serve-workers", type=int, default=2)
+    ap.add_argument("--no-mixed-serving", action="store_true")
+    ap.add_argument("--mixed-rate", type=float, default=40.0,
+                    help="Poisson arrival rate of the six-shape serving run")
+    ap.add_argument("--mixed-requests", type=int, default=80)
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.pmc_child:
@@ -492,32 +523,21 @@ def main():
     if traffic and dom in traffic:
         roofline["traffic"] = round(traffic[dom]["fetch_bytes"] + traffic[dom]["write_bytes"])
 
-    # ---- serving: Poisson arrivals of this volume at a fixed rate -----------
-    srv_res = None
+    # ---- serving: continuous batching, Poisson arrivals at a fixed rate -----
+    srv_res = mix_res = None
     if args.serve_requests > 0:
-        with pkg.Server(workers=args.serve_workers) as srv:
-            warm = [srv.submit(vol, patch, stride, keep=False)
-                    for _ in range(2 * args.serve_workers)]
-            for f in warm:
-                f.result()
-            nb = 4 * args.serve_workers * 5
-            t0 = time.perf_counter()
-            burst = [srv.submit(vol, patch, stride, keep=False) for _ in range(nb)]
-            for f in burst:
-                f.result()
-            burst_vps = nb / (time.perf_counter() - t0)
-            r = pkg.poisson_run(srv, [(vol, patch, stride)], rate_hz=args.serve_rate,
-                                n_requests=args.serve_requests, seed=0)
-        srv_res = {"arrivals": "poisson", "rate_hz": args.serve_rate,
-                   "requests": args.serve_requests, "workers": args.serve_workers,
-                   "p50_ms": round(r["latency_ms"]["p50"], 3),
-                   "p99_ms": round(r["latency_ms"]["p99"], 3),
-                   "max_ms": round(r["latency_ms"]["max"], 3),
-                   "volumes_per_s": round(r["volumes_per_s"], 2),
-                   "burst_volumes_per_s": round(burst_vps, 2),
-                   "latency": "arrival to host result, host->host"}
+        srv_res = serve_run(pkg, [(vol, patch, stride)], args.serve_rate, args.serve_requests)
         log(f"serving at {args.serve_rate:.0f}/s: p50 {srv_res['p50_ms']:.2f} p99 "
-            f"{srv_res['p99_ms']:.2f} ms; burst {burst_vps:.1f} volumes/s")
+            f"{srv_res['p99_ms']:.2f} ms; burst {srv_res['burst_volumes_per_s']:.1f} volumes/s")
+        if not args.no_mixed_serving:
+            log("generating the six mixed-shape volumes (seeds 10-15)")
+            mixed = [pkg.mixed_volume(k, 10 + k) for k in range(len(pkg.MIXED_GEOMETRIES))]
+            mix_res = serve_run(pkg, mixed, args.mixed_rate, args.mixed_requests)
+            mix_res["shapes"] = [list(map(int, v.shape)) for v, _, _ in mixed]
+            log(f"mixed-shape serving at {args.mixed_rate:.0f}/s: p50 {mix_res['p50_ms']:.2f} "
+                f"p99 {mix_res['p99_ms']:.2f} ms; burst {mix_res['burst_volumes_per_s']:.1f} "
+                "volumes/s")
+            del mixed
 
     cpu = host_cpu()
     workers = max(1, min(16, cpu["cores"]))
@@ -546,6 +566,7 @@ def main():
         "device_resident_ms": round(dev_ms, 4),
         "device_resident_volumes_per_s": round(1e3 / dev_ms, 2),
         "serving_at_fixed_load": srv_res,
+        "serving_mixed_shapes": mix_res,
         "kernels": kernels,
         "roofline": roofline,
         "traffic_note": traffic_note if traffic is None else

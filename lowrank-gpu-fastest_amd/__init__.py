@@ -21,6 +21,8 @@ from .phantom import (  # noqa: F401
 from .denoise import FORMS, Denoiser, SolverStatusError, denoise, get_denoiser  # noqa: F401
 from .serve import AdmissionError, Server, device_bytes, poisson_run  # noqa: F401
 from .batch import BatchDenoiser, denoise_many  # noqa: F401
+from .batch_serve import BatchServer, lane_bytes  # noqa: F401
+from . import autotune  # noqa: F401
 
 __all__ = [
     "denoise",
@@ -44,4 +46,6 @@ __all__ = [
     "AdmissionError",
     "BatchDenoiser",
     "denoise_many",
+    "BatchServer",
+    "lane_bytes",
 ]
