@@ -49,9 +49,23 @@ extern "C" {
  *          FP32 MFMA reconstruction from the rank-p factors (T <= 64; above
  *          that from the projector). Matches step 0 to ~1e-7.
  *   FP32 : the reference's all-FP32 arithmetic (FP32 MFMA Gram, FP32 Jacobi
- *          eigensolver), as step8_tuned.py computes it. */
+ *          eigensolver), as step8_tuned.py computes it.
+ *   MIXED: the precision form. Gram on FP16 MFMA with each float split into
+ *          hi + lo halves (three products, FP32 accumulation, the centred
+ *          volume scaled by a power of two per volume into FP16's range);
+ *          FP32 Householder reduction; the MP decision in FP64 on the
+ *          eigenvalues (FP64 suffix sums, FP64 inequality); patches whose
+ *          relative MP margin |lhs - rhs| / (|lhs| + |rhs|) falls below
+ *          MPPCA_MIXED_RESOLVE_MARGIN are solved again from the exact FP64
+ *          Gram with the FP64 eigensolver, up to MPPCA_MIXED_RESOLVE_FRAC of
+ *          the patches (more are flagged -3 in the status output). Rank-p
+ *          factors and FP32 MFMA reconstruction as PLAIN. The margin output
+ *          holds the relative margin. Its reference is the FP64 oracle. */
 #define MPPCA_FORM_PLAIN 0
 #define MPPCA_FORM_FP32 1
+#define MPPCA_FORM_MIXED 2
+#define MPPCA_MIXED_RESOLVE_MARGIN 1e-4
+#define MPPCA_MIXED_RESOLVE_FRAC 8 /* at most P / 8 patches re-solved per batch */
 
 /* Tile schedule: the kernel choices of the whole path for one geometry.
  *   gram      MPPCA_GRAM_*: the FP64 Gram's kernel (plain form): the int8

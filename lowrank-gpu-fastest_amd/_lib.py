@@ -13,7 +13,7 @@ import pathlib
 PKG = pathlib.Path(__file__).resolve().parent
 LIB_PATH = PKG / "libmppca.so"
 ABI_VERSION = 104
-FORM_CODES = {"plain": 0, "fp32": 1}
+FORM_CODES = {"plain": 0, "fp32": 1, "mixed": 2}
 
 _i32, _i64, _vp, _f = ctypes.c_int, ctypes.c_int64, ctypes.c_void_p, ctypes.c_float
 _cp = ctypes.c_char_p

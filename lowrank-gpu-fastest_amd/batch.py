@@ -26,7 +26,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from .denoise import FORMS, FACTOR_MAX_T, SolverStatusError
+from .denoise import FACTOR_FORMS, FORMS, FACTOR_MAX_T, SolverStatusError
 from .geometry import check_geometry, num_patches
 
 
@@ -69,7 +69,7 @@ class BatchDenoiser:
         self.rank = torch.empty(P, dtype=torch.int32, device=dev)
         self.margin = torch.empty(P, dtype=torch.float32, device=dev)
         self.sweeps = torch.empty(P, dtype=torch.int32, device=dev)
-        self.route = ("factors" if form == "plain" and self.shape[3] <= FACTOR_MAX_T
+        self.route = ("factors" if form in FACTOR_FORMS and self.shape[3] <= FACTOR_MAX_T
                       else "projector")
         self._graphs = {}
 
@@ -167,7 +167,8 @@ class BatchDenoiser:
             return
         st = st.cpu().numpy()
         msg = (f"{int((st == -2).sum())} patch(es) hold NaN or Inf, "
-               f"{int((st == -1).sum())} did not converge, in a batch of {n} volumes")
+               f"{int((st == -1).sum())} did not converge, {int((st == -3).sum())} past the "
+               f"re-solve capacity, in a batch of {n} volumes")
         if on_bad == "raise":
             raise SolverStatusError(msg, st)
         import warnings

@@ -351,7 +351,8 @@ class BatchServer:
                     st = status[slot * P:(slot + 1) * P]
                     if self.on_bad != "ignore" and st.min() < 0:
                         msg = (f"{int((st == -2).sum())} patch(es) hold NaN or Inf, "
-                               f"{int((st == -1).sum())} did not converge")
+                               f"{int((st == -1).sum())} did not converge, "
+                               f"{int((st == -3).sum())} past the re-solve capacity")
                         if self.on_bad == "raise":
                             raise SolverStatusError(msg, st.copy())
                     req.fut.t_done = time.perf_counter()

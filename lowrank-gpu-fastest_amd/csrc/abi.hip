@@ -106,7 +106,7 @@ Workspace carve_workspace(char* base, const Geom& g, int form) {
   w.mean = (float*)take(nvox * sizeof(float));
   w.proj = nullptr;
   w.fac = nullptr;
-  if (form == MPPCA_FORM_PLAIN) {
+  if (form != MPPCA_FORM_FP32) {
     w.gram = take(ptt * sizeof(double));
     if (g.T <= kFactorMaxT)
       w.fac = (float*)take(P * factor_floats(g.T) * sizeof(float));
@@ -120,11 +120,32 @@ Workspace carve_workspace(char* base, const Geom& g, int form) {
   w.rank = (int*)take(P * sizeof(int));
   w.margin = (float*)take(P * sizeof(float));
   w.info = (int*)take(P * sizeof(int));
+  w.amax = nullptr;
+  w.rlist = w.rcount = w.rrank = w.rinfo = nullptr;
+  w.rcap = 0;
+  w.rgram = nullptr;
+  w.rblk = w.rweight = w.rmargin = nullptr;
+  if (form == MPPCA_FORM_MIXED) {
+    const size_t cap = (P + MPPCA_MIXED_RESOLVE_FRAC - 1) / MPPCA_MIXED_RESOLVE_FRAC;
+    const size_t blk = g.T <= kFactorMaxT ? (size_t)factor_floats(g.T) : (size_t)g.T * g.T;
+    w.rcap = (int)cap;
+    w.amax = (unsigned*)take((size_t)g.nvol * kAmaxSlots * sizeof(unsigned));
+    w.rlist = (int*)take(cap * sizeof(int));
+    w.rcount = (int*)take(sizeof(int));
+    w.rgram = (double*)take(cap * g.T * g.T * sizeof(double));
+    w.rblk = (float*)take(cap * blk * sizeof(float));
+    w.rrank = (int*)take(cap * sizeof(int));
+    w.rweight = (float*)take(cap * sizeof(float));
+    w.rmargin = (float*)take(cap * sizeof(float));
+    w.rinfo = (int*)take(cap * sizeof(int));
+  }
   w.bytes = off;
   return w;
 }
 
-bool valid_form(int form) { return form == MPPCA_FORM_PLAIN || form == MPPCA_FORM_FP32; }
+bool valid_form(int form) {
+  return form == MPPCA_FORM_PLAIN || form == MPPCA_FORM_FP32 || form == MPPCA_FORM_MIXED;
+}
 
 }  // namespace mppca
 
@@ -448,8 +469,11 @@ int run_device_path(const float* vol, float* out, const Geom& g, int form, const
   // the voxel-wise and matrix-wise stages see the batch's volumes as one
   // array; the others map their grids onto the volumes (Geom::nvol)
   const long long P = g.P * g.nvol;
-  if ((rc = launch_centre(vol, w.xc, w.mean, g.nvox * g.nvol, T, st))) return rc;
-  if (form == MPPCA_FORM_PLAIN) {
+  if ((rc = launch_centre(vol, w.xc, w.mean, g.nvox * g.nvol, T, st, w.amax, g.nvox))) return rc;
+  if (form == MPPCA_FORM_MIXED) {
+    if ((rc = run_mixed_solve(g, w, rk, mg, inf, st))) return rc;
+    if (w.fac) return launch_recon_factors(w.xc, w.fac, rk, w.weight, w.mean, out, g, 0, g.X, st);
+  } else if (form == MPPCA_FORM_PLAIN) {
     if ((rc = launch_gram64(w.xc, (double*)w.gram, g, 0, g.P, st))) return rc;
     if ((rc = launch_eig64_mp((double*)w.gram, P, T, g.N, w.proj, w.fac, rk, w.weight, mg,
                               nullptr, inf, st, g.tune.eig)))

@@ -226,6 +226,19 @@ struct Workspace {
   int* rank;
   float* margin;
   int* info;
+  // precision form only: per-volume max |xc|, and the re-solve list of up to
+  // rcap patches with their exact FP64 Grams and outputs (factor or projector
+  // blocks, rank, weight, margin, status)
+  unsigned* amax;
+  int* rlist;
+  int* rcount;
+  int rcap;
+  double* rgram;
+  float* rblk;
+  int* rrank;
+  float* rweight;
+  float* rmargin;
+  int* rinfo;
   size_t bytes;
 };
 Workspace carve_workspace(char* base, const Geom& g, int form);  // base null: sizes only
@@ -236,7 +249,11 @@ int run_device_path(const float* vol, float* out, const Geom& g, int form, const
 
 // ---- launchers (defined per stage file) ----------------------------------------
 int zero_fill(float* p, long long n, hipStream_t st);
-int launch_centre(const float* vol, float* xc, float* mean, long long nvox, int T, hipStream_t st);
+// amax (optional): per-volume max |xc| of finite entries, volumes of nvox1
+// voxels, spread over kAmaxSlots slots per volume (the max of the slots)
+constexpr int kAmaxSlots = 64;
+int launch_centre(const float* vol, float* xc, float* mean, long long nvox, int T, hipStream_t st,
+                  unsigned* amax = nullptr, long long nvox1 = 0);
 int launch_normalize(const float* acc, const float* mean, const float* weight, float* out,
                      const Geom& g, hipStream_t st);
 int launch_gram(const float* xc, float* gram, const Geom& g, long long b0, long long nb,
@@ -255,6 +272,14 @@ int launch_mp(const float* lam, long long nb, int T, int N, int* rank, float* we
 int launch_eig64_mp(double* gram, long long nb, int T, int N, float* proj, float* fac, int* rank,
                     float* weight, float* margin, float* lam_desc, int* info, hipStream_t st,
                     int variant = 0);
+// The precision form (mixed.hip, and its solves in eig_tridiag.hip).
+int launch_eig_mixed(double* gram, long long nb, int T, int N, float* proj, float* fac, int* rank,
+                     float* weight, float* margin, int* info, hipStream_t st,
+                     bool fp32_reduction);
+int launch_gram_h3(const float* xc, const unsigned* amax, double* gram, const Geom& g,
+                   hipStream_t st);
+int run_mixed_solve(const Geom& g, const Workspace& w, int* rank, float* margin, int* info,
+                    hipStream_t st);
 int launch_recon(const float* xc, const float* proj, const float* weight, float* acc,
                  const Geom& g, long long b0, long long nb, hipStream_t st);
 // Output planes x in [x0, x1) (x0 a multiple of recon_factor_tile_x(g), x1 too

@@ -205,6 +205,46 @@ __device__ __forceinline__ double rsq64(double x) {
   return y;
 }
 
+// Single-precision forms of the lane helpers above (the FP32 Householder
+// reduction of the mixed form): one 32-bit lane move per value; the hardware
+// reciprocal and reciprocal square root estimates plus one Newton step.
+template <int CTRL>
+__device__ __forceinline__ float dpp32(float x) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), CTRL, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float lane_read64(float x, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
+}
+__device__ __forceinline__ float bperm64(float x, int src) {
+  return __int_as_float(__builtin_amdgcn_ds_bpermute(src << 2, __float_as_int(x)));
+}
+__device__ __forceinline__ float group8_sum(float x) {
+  x += dpp32<kDppQuadXor1>(x);
+  x += dpp32<kDppQuadXor2>(x);
+  x += dpp32<kDppRowHalfMirror>(x);
+  return x;
+}
+__device__ __forceinline__ float groups_sum(float x) {
+  x += dpp32<kDppRowRor8>(x);
+  return (lane_read64(x, 0) + lane_read64(x, 16)) + (lane_read64(x, 32) + lane_read64(x, 48));
+}
+__device__ __forceinline__ float rcp64(float q) {
+  const float r = __builtin_amdgcn_rcpf(q);
+  return fmaf(fmaf(-q, r, 1.f), r, r);
+}
+__device__ __forceinline__ float rsq64(float x) {
+  const float y = __builtin_amdgcn_rsqf(x);
+  return fmaf(0.5f * y, fmaf(-(x * y), y, 1.f), y);
+}
+// Smallest sum of squares that takes the estimate path of the reduction
+// (below it: IEEE sqrt and a division).
+template <typename S>
+__device__ __forceinline__ S tiny_sq();
+template <>
+__device__ __forceinline__ double tiny_sq<double>() { return 0x1p-900; }
+template <>
+__device__ __forceinline__ float tiny_sq<float>() { return 0x1p-100f; }
+
 // Number of eigenvalues of the tridiagonal (d, e) smaller than x.
 __device__ __forceinline__ int sturm_count(const double* d, const double* e2, int T, double x,
                                            double pivmin) {
@@ -648,6 +688,47 @@ __device__ void mp_rule_wave(const float* __restrict__ lam, float* __restrict__ 
   *margin_out = (float)wave_min(mg);
 }
 
+// MP rule in FP64, the precision form's decision and the FP64 oracle's
+// arithmetic (step0_numpy_loop.py:18-24 on float64 data): e_i = max(lam_i, 0) / n,
+// suffix sums from the smallest value in order, both sides of the inequality
+// in double. margin: min over i < M-1 of |lhs_i - rhs_i| / (|lhs_i| + |rhs_i|),
+// 0 where both sides vanish (index M-1 is left out: its lhs is exactly 0).
+// e and rc are LDS arrays of M doubles.
+__device__ void mp_rule_wave64(const double* __restrict__ lam, double* __restrict__ e,
+                               double* __restrict__ rc, int T, int N, int lane, int* p_out,
+                               float* margin_out) {
+  const int M = N < T - 1 ? N : T - 1;
+  const int n = N > T - 1 ? N : T - 1;
+  for (int i = lane; i < M; i += 64) e[i] = fmax(lam[i], 0.0) / (double)n;
+  __syncthreads();
+  if (lane == 0) {
+    double run = e[M - 1];
+    rc[M - 1] = run;
+#pragma unroll 8
+    for (int i = M - 2; i >= 0; --i) {
+      run += e[i];
+      rc[i] = run;
+    }
+  }
+  __syncthreads();
+  const double sq = sqrt((double)M * (double)n);
+  const double elast = e[M - 1];
+  int cnt = 0;
+  double mg = INFINITY;
+  for (int i = lane; i < M; i += 64) {
+    const double lhs = (e[i] - elast) * (double)(M - i) * (double)(n - i);
+    const double rhs = 4.0 * rc[i] * sq;
+    cnt += lhs > rhs ? 1 : 0;
+    if (i < M - 1) {
+      const double den = fabs(lhs) + fabs(rhs);
+      const double r = den > 0.0 ? fabs(lhs - rhs) / den : 0.0;
+      mg = r < mg ? r : mg;
+    }
+  }
+  *p_out = (int)wave_sum((double)cnt);
+  *margin_out = (float)wave_min(mg);
+}
+
 // ---------------------------------------------------------------------------
 // Register-resident eigensolver for T <= 48 (TP = T rounded up to 16, 32 or 48),
 // three kernels on the 8 x 8 lane grid of a wavefront: k_tri_grid reduces each
@@ -658,10 +739,36 @@ __device__ void mp_rule_wave(const float* __restrict__ lam, float* __restrict__ 
 //   row k <= T-3   reflector v_k at columns k+1 .. T-1 (v_{k+1} = 1), tau_k
 //                  at column 0 (v_k vanishes there)
 //   row T-2        the diagonal d_0 .. d_{T-1}
-//   row T-1        the off-diagonal e_0 .. e_{T-2}
+//   row T-1        the off-diagonal e_0 .. e_{T-2}; with REV, at column T-1
+//                  the frame order of the reduction (1: reversed)
 // A non-finite input matrix leaves d_0 = NaN and nothing else.
+// REV (the precision form's solves): the reduction starts at the end of the
+// frame axis with the smaller Gram diagonal, i.e. the matrix is reduced with
+// its frames reversed when the first half of the diagonal outweighs the
+// second. Householder reduction of a graded matrix is more accurate from its
+// small end (the FP32 reduction of decaying time courses moved the output by
+// 7e-7 against the FP64 oracle in frame order, 3e-7 reversed); k_eig_vecs
+// undoes the reversal on the eigenvectors.
 // ---------------------------------------------------------------------------
-template <int TP>
+// Reduction from the low-energy end: whether the first half of the diagonal
+// of the T x T matrix G outweighs the second (same result on every lane).
+template <typename In>
+__device__ __forceinline__ bool reverse_frames(const In* G, int T, int lane) {
+  float lo = 0.f, hi = 0.f;
+  for (int i = lane; i < T; i += 64) {
+    const float d = (float)G[i * T + i];
+    if (2 * i < T - 1) lo += d;
+    if (2 * i > T - 1) hi += d;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    lo += __shfl_xor(lo, o);
+    hi += __shfl_xor(hi, o);
+  }
+  return lo > hi;
+}
+
+template <int TP, typename S = double, typename In = double, bool REV = false>
 __global__ __launch_bounds__(64) void k_tri_grid(double* __restrict__ gram, int T, long long nb) {
   constexpr int R = TP / 8;  // register rows / columns per lane of the 8 x 8 lane grid
   const long long b = blockIdx.x;
@@ -669,15 +776,20 @@ __global__ __launch_bounds__(64) void k_tri_grid(double* __restrict__ gram, int 
   const int lane = threadIdx.x;
   const int gr = lane >> 3, gc = lane & 7;  // lane-grid row and column
   double* Gb = gram + b * (long long)T * T;
+  // In = float: the matrix arrives as float32 at the start of its double slot;
+  // every lane has read it (the finiteness vote below) before any write
+  const In* Gin = (const In*)Gb;
+  const bool rev = REV && reverse_frames(Gin, T, lane);
   // lane (gr, gc) holds A[gr + 8m][gc + 8n], m, n < R (zero past T)
-  double a[R][R];
+  S a[R][R];
   bool finite = true;
 #pragma unroll
   for (int m = 0; m < R; ++m)
 #pragma unroll
     for (int n = 0; n < R; ++n) {
       const int i = gr + 8 * m, j = gc + 8 * n;
-      a[m][n] = (i < T && j < T) ? Gb[i * T + j] : 0.0;
+      const int si = rev ? T - 1 - i : i, sj = rev ? T - 1 - j : j;
+      a[m][n] = (i < T && j < T) ? (S)Gin[si * T + sj] : S(0);
       finite = finite && isfinite(a[m][n]);
     }
   if (__any(!finite)) {  // the whole matrix is in registers: flag it
@@ -699,64 +811,64 @@ __global__ __launch_bounds__(64) void k_tri_grid(double* __restrict__ gram, int 
     for (int r0 = 0; r0 < 8; ++r0) {
       const int k = 8 * mb + r0;
       if (k + 2 >= T) break;
-      const double dk = lane_read64(a[mb][mb], 9 * r0);  // A[k][k]
+      const S dk = lane_read64(a[mb][mb], 9 * r0);  // A[k][k]
       const int nb1 = mb + 1 < R ? mb + 1 : mb;  // column k+1 = 8(mb+1) when r0 = 7
-      const double asrc = r0 == 7 ? a[mb][nb1] : a[mb][mb];
-      const double alpha = lane_read64(asrc, 8 * r0 + ((r0 + 1) & 7));  // A[k][k+1]
+      const S asrc = r0 == 7 ? a[mb][nb1] : a[mb][mb];
+      const S alpha = lane_read64(asrc, 8 * r0 + ((r0 + 1) & 7));  // A[k][k+1]
       // Index tests against k = 8 mb + r0 (r0 < 8) are static from block
       // mb + 2 on (every column there is > k + 1) and, for rows > k, from
       // block mb + 1 on: only the first one or two register blocks select.
-      double part = 0.0;
+      S part = S(0);
 #pragma unroll
       for (int n = mb; n < R; ++n) {
-        const double xv = a[mb][n];
+        const S xv = a[mb][n];
         if (n >= mb + 2)
           part = fma(xv, xv, part);
         else
           part = gc + 8 * n > k + 1 ? fma(xv, xv, part) : part;
       }
-      const double ss = lane_read64(group8_sum(part), 8 * r0);
+      const S ss = lane_read64(group8_sum(part), 8 * r0);
       // dlarfg: beta = -sign(alpha) mu, mu = |(alpha, x)|; tau = (beta - alpha) / beta
       // = 1 + |alpha| / mu; v scaled by 1 / (alpha - beta) = sign(alpha) / (|alpha| + mu).
       // The operands are wave-uniform but live in VALU registers: the
       // reciprocal square root and reciprocal estimates with Newton steps
       // cost a third of IEEE sqrt and two divisions (tiny sums take those).
-      double tk = 0.0, beta = alpha, sc = 0.0;
-      if (ss > 0.0) {
-        const double s2 = fma(alpha, alpha, ss);
-        double mu, rmu;
-        if (s2 >= 0x1p-900) {
+      S tk = S(0), beta = alpha, sc = S(0);
+      if (ss > S(0)) {
+        const S s2 = fma(alpha, alpha, ss);
+        S mu, rmu;
+        if (s2 >= tiny_sq<S>()) {
           rmu = rsq64(s2);
           mu = s2 * rmu;
         } else {
           mu = sqrt(s2);
-          rmu = 1.0 / mu;
+          rmu = S(1) / mu;
         }
-        const double aa = fabs(alpha);
-        beta = alpha >= 0.0 ? -mu : mu;
-        tk = fma(aa, rmu, 1.0);
-        const double sa = rcp64(aa + mu);
-        sc = alpha >= 0.0 ? sa : -sa;
+        const S aa = fabs(alpha);
+        beta = alpha >= S(0) ? -mu : mu;
+        tk = fma(aa, rmu, S(1));
+        const S sa = rcp64(aa + mu);
+        sc = alpha >= S(0) ? sa : -sa;
       }
       // v at this lane's columns gc + 8n and rows gr + 8m (row k of A lives in
       // grid row r0: column j in grid column j % 8, register column j / 8)
-      double vc[R], vr[R];
+      S vc[R], vr[R];
 #pragma unroll
       for (int n = 0; n < R; ++n) {
-        vc[n] = 0.0;
-        vr[n] = 0.0;
+        vc[n] = S(0);
+        vr[n] = S(0);
       }
 #pragma unroll
       for (int n = mb; n < R; ++n) {
-        const double xc = bperm64(a[mb][n], 8 * r0 + gc);
-        const double xr = bperm64(a[mb][n], 8 * r0 + gr);
+        const S xc = bperm64(a[mb][n], 8 * r0 + gc);
+        const S xr = bperm64(a[mb][n], 8 * r0 + gr);
         if (n >= mb + 2) {
           vc[n] = xc * sc;
           vr[n] = xr * sc;
         } else {
           const int j = gc + 8 * n, i = gr + 8 * n;
-          vc[n] = j == k + 1 ? 1.0 : (j > k + 1 ? xc * sc : 0.0);
-          vr[n] = i == k + 1 ? 1.0 : (i > k + 1 ? xr * sc : 0.0);
+          vc[n] = j == k + 1 ? S(1) : (j > k + 1 ? xc * sc : S(0));
+          vr[n] = i == k + 1 ? S(1) : (i > k + 1 ? xr * sc : S(0));
         }
       }
       if (gr == r0) {  // reflector k (zero for indices <= k; column 0 carries tau_k)
@@ -770,20 +882,20 @@ __global__ __launch_bounds__(64) void k_tri_grid(double* __restrict__ gram, int 
         Gb[(T - 2) * T + k] = dk;
         Gb[(T - 1) * T + k] = beta;
       }
-      if (tk != 0.0) {
-        double pl[R], part2 = 0.0;
+      if (tk != S(0)) {
+        S pl[R], part2 = S(0);
 #pragma unroll
         for (int m = mb; m < R; ++m) {
-          double acc = 0.0;
+          S acc = S(0);
 #pragma unroll
           for (int n = mb; n < R; ++n) acc = fma(a[m][n], vc[n], acc);
           acc = group8_sum(acc);
-          pl[m] = (m >= mb + 1 || gr + 8 * m > k) ? tk * acc : 0.0;
+          pl[m] = (m >= mb + 1 || gr + 8 * m > k) ? tk * acc : S(0);
           part2 = fma(pl[m], vr[m], part2);
         }
         // part2 is the same on the 8 lanes of a grid row: each row once
-        const double K = 0.5 * tk * groups_sum(part2);
-        double wr[R], wc[R];
+        const S K = S(0.5) * tk * groups_sum(part2);
+        S wr[R], wc[R];
 #pragma unroll
         for (int m = mb; m < R; ++m) wr[m] = pl[m] - K * vr[m];
 #pragma unroll
@@ -798,7 +910,7 @@ __global__ __launch_bounds__(64) void k_tri_grid(double* __restrict__ gram, int 
   };
   static_for<R>(block_step);
   // last 2x2 block: rows T-2 and T-1
-  double dd2 = 0.0, dd1 = 0.0, ee2 = 0.0;
+  S dd2 = S(0), dd1 = S(0), ee2 = S(0);
 #pragma unroll
   for (int m = 0; m < R; ++m)
 #pragma unroll
@@ -813,12 +925,17 @@ __global__ __launch_bounds__(64) void k_tri_grid(double* __restrict__ gram, int 
   if (gr == (T - 2) % 8 && gc == (T - 2) % 8) Gb[(T - 2) * T + T - 2] = dd2;
   if (gr == (T - 1) % 8 && gc == (T - 1) % 8) Gb[(T - 2) * T + T - 1] = dd1;
   if (gr == (T - 1) % 8 && gc == (T - 2) % 8) Gb[(T - 1) * T + T - 2] = ee2;
+  if (REV && lane == 0) Gb[(T - 1) * T + T - 1] = rev ? 1.0 : 0.0;
 }
 
 // Sum over the 16 lanes of a DPP row (every lane of the row gets it).
 __device__ __forceinline__ double row16_sum(double x) {
   x = group8_sum(x);
   return x + dpp64<kDppRowMirror>(x);
+}
+__device__ __forceinline__ float row16_sum(float x) {
+  x = group8_sum(x);
+  return x + dpp32<kDppRowMirror>(x);
 }
 
 // ---------------------------------------------------------------------------
@@ -831,27 +948,31 @@ __device__ __forceinline__ double row16_sum(double x) {
 // partial sums of p.v -- goes through LDS, with two barriers per step. Writes
 // the hand-off of k_tri_grid (reflectors, tau, d and e in the Gram buffer).
 // ---------------------------------------------------------------------------
-template <int TP>
+template <int TP, typename S = double, typename In = double, bool REV = false>
 __global__ __launch_bounds__(256) void k_tri_grid4(double* __restrict__ gram, int T,
                                                    long long nb) {
   constexpr int R = TP / 16;  // register rows / columns per lane of the 16 x 16 grid
-  __shared__ double s_v[TP];  // the step's Householder vector, by index
-  __shared__ double s_p[TP];  // p = tau A v of the step, by row
-  __shared__ double s_part[4];
-  __shared__ double s_tk;
+  __shared__ S s_v[TP];  // the step's Householder vector, by index
+  __shared__ S s_p[TP];  // p = tau A v of the step, by row
+  __shared__ S s_part[4];
+  __shared__ S s_tk;
   const long long b = blockIdx.x;
   if (b >= nb) return;
   const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
   const int gr = t >> 4, gc = t & 15;  // lane-grid row and column
   double* Gb = gram + b * (long long)T * T;
-  double a[R][R];
+  const In* Gin = (const In*)Gb;  // read by every thread before the barrier below
+  // every wavefront reaches the same decision from the same diagonal
+  const bool rev = REV && reverse_frames(Gin, T, lane);
+  S a[R][R];
   bool finite = true;
 #pragma unroll
   for (int m = 0; m < R; ++m)
 #pragma unroll
     for (int n = 0; n < R; ++n) {
       const int i = gr + 16 * m, j = gc + 16 * n;
-      a[m][n] = (i < T && j < T) ? Gb[i * T + j] : 0.0;
+      const int si = rev ? T - 1 - i : i, sj = rev ? T - 1 - j : j;
+      a[m][n] = (i < T && j < T) ? (S)Gin[si * T + sj] : S(0);
       finite = finite && isfinite(a[m][n]);
     }
   if (__syncthreads_or(!finite)) {  // the whole matrix is in registers: flag it
@@ -868,47 +989,47 @@ __global__ __launch_bounds__(256) void k_tri_grid4(double* __restrict__ gram, in
       if (k + 2 >= T) break;
       if (wave == (r0 >> 2)) {  // the wavefront that holds row k
         const int base = 16 * (r0 & 3);  // its DPP row
-        const double dk = lane_read64(a[mb][mb], base + r0);  // A[k][k]
+        const S dk = lane_read64(a[mb][mb], base + r0);  // A[k][k]
         constexpr int nb1 = mb + 1 < R ? mb + 1 : mb;  // column k+1 = 16(mb+1) when r0 = 15
-        const double asrc = r0 == 15 ? a[mb][nb1] : a[mb][mb];
-        const double alpha = lane_read64(asrc, base + ((r0 + 1) & 15));  // A[k][k+1]
-        double part = 0.0;
+        const S asrc = r0 == 15 ? a[mb][nb1] : a[mb][mb];
+        const S alpha = lane_read64(asrc, base + ((r0 + 1) & 15));  // A[k][k+1]
+        S part = S(0);
 #pragma unroll
         for (int n = mb; n < R; ++n) {
-          const double xv = a[mb][n];
+          const S xv = a[mb][n];
           if (n >= mb + 2)
             part = fma(xv, xv, part);
           else
             part = gc + 16 * n > k + 1 ? fma(xv, xv, part) : part;
         }
-        const double ss = lane_read64(row16_sum(part), base);
+        const S ss = lane_read64(row16_sum(part), base);
         // dlarfg, as k_tri_grid
-        double tk = 0.0, beta = alpha, sc = 0.0;
-        if (ss > 0.0) {
-          const double s2 = fma(alpha, alpha, ss);
-          double mu, rmu;
-          if (s2 >= 0x1p-900) {
+        S tk = S(0), beta = alpha, sc = S(0);
+        if (ss > S(0)) {
+          const S s2 = fma(alpha, alpha, ss);
+          S mu, rmu;
+          if (s2 >= tiny_sq<S>()) {
             rmu = rsq64(s2);
             mu = s2 * rmu;
           } else {
             mu = sqrt(s2);
-            rmu = 1.0 / mu;
+            rmu = S(1) / mu;
           }
-          const double aa = fabs(alpha);
-          beta = alpha >= 0.0 ? -mu : mu;
-          tk = fma(aa, rmu, 1.0);
-          const double sa = rcp64(aa + mu);
-          sc = alpha >= 0.0 ? sa : -sa;
+          const S aa = fabs(alpha);
+          beta = alpha >= S(0) ? -mu : mu;
+          tk = fma(aa, rmu, S(1));
+          const S sa = rcp64(aa + mu);
+          sc = alpha >= S(0) ? sa : -sa;
         }
         if (gr == r0) {  // v at columns gc + 16n, and reflector row k of the hand-off
 #pragma unroll
           for (int n = 0; n < R; ++n) {
             const int j = gc + 16 * n;
-            double v = 0.0;
+            S v = S(0);
             if (n >= mb + 2)
               v = a[mb][n] * sc;
             else if (n >= mb)
-              v = j == k + 1 ? 1.0 : (j > k + 1 ? a[mb][n] * sc : 0.0);
+              v = j == k + 1 ? S(1) : (j > k + 1 ? a[mb][n] * sc : S(0));
             s_v[j] = v;
             if (j < T) Gb[k * T + j] = j == 0 ? tk : v;
           }
@@ -920,38 +1041,38 @@ __global__ __launch_bounds__(256) void k_tri_grid4(double* __restrict__ gram, in
         }
       }
       __syncthreads();
-      const double tk = s_tk;
-      double vc[R], vr[R];
+      const S tk = s_tk;
+      S vc[R], vr[R];
 #pragma unroll
       for (int n = 0; n < R; ++n) {
-        vc[n] = n >= mb ? s_v[gc + 16 * n] : 0.0;
-        vr[n] = n >= mb ? s_v[gr + 16 * n] : 0.0;
+        vc[n] = n >= mb ? s_v[gc + 16 * n] : S(0);
+        vr[n] = n >= mb ? s_v[gr + 16 * n] : S(0);
       }
-      double pl[R];
-      if (tk != 0.0) {  // the same on every lane
-        double part2 = 0.0;
+      S pl[R];
+      if (tk != S(0)) {  // the same on every lane
+        S part2 = S(0);
 #pragma unroll
         for (int m = mb; m < R; ++m) {
-          double acc = 0.0;
+          S acc = S(0);
 #pragma unroll
           for (int n = mb; n < R; ++n) acc = fma(a[m][n], vc[n], acc);
           acc = row16_sum(acc);
-          pl[m] = (m >= mb + 1 || gr + 16 * m > k) ? tk * acc : 0.0;
+          pl[m] = (m >= mb + 1 || gr + 16 * m > k) ? tk * acc : S(0);
           part2 = fma(pl[m], vr[m], part2);
           if (gc == 0) s_p[gr + 16 * m] = pl[m];
         }
         // part2 is the same on the 16 lanes of a grid row: each row once (the
         // lane reads stay outside the one-lane branch, so that every lane has
         // computed the value read)
-        const double wsum = (lane_read64(part2, 0) + lane_read64(part2, 16)) +
+        const S wsum = (lane_read64(part2, 0) + lane_read64(part2, 16)) +
                             (lane_read64(part2, 32) + lane_read64(part2, 48));
         if (lane == 0) s_part[wave] = wsum;
       }
       // also when tk = 0: no wavefront writes s_v again before every one has read it
       __syncthreads();
-      if (tk != 0.0) {
-        const double K = 0.5 * tk * ((s_part[0] + s_part[1]) + (s_part[2] + s_part[3]));
-        double wr[R], wc[R];
+      if (tk != S(0)) {
+        const S K = S(0.5) * tk * ((s_part[0] + s_part[1]) + (s_part[2] + s_part[3]));
+        S wr[R], wc[R];
 #pragma unroll
         for (int m = mb; m < R; ++m) wr[m] = pl[m] - K * vr[m];
 #pragma unroll
@@ -975,6 +1096,7 @@ __global__ __launch_bounds__(256) void k_tri_grid4(double* __restrict__ gram, in
       if (i == T - 1 && j == T - 1) Gb[(T - 2) * T + T - 1] = a[m][n];
       if (i == T - 1 && j == T - 2) Gb[(T - 1) * T + T - 2] = a[m][n];
     }
+  if (REV && t == 0) Gb[(T - 1) * T + T - 1] = rev ? 1.0 : 0.0;
 }
 
 // Where k_eig_vals leaves the eigenvalues of a patch for k_eig_vecs: the last
@@ -1280,7 +1402,7 @@ __global__ __launch_bounds__(64) void k_eig_vals(const double* __restrict__ gram
 // and rank left by k_eig_vals: twisted factorisation, Gram-Schmidt inside the
 // group, back-transform on the lane grid, factors or projector rows. Its own
 // kernel so that it runs at the occupancy of its few registers.
-template <int TP>
+template <int TP, bool RULE64 = false>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(TP > 64 ? 2 : 4))) void k_eig_vecs(const double* __restrict__ gram, int T, int N,
                                                  long long nb, float* __restrict__ proj,
                                                  float* __restrict__ fac,
@@ -1360,7 +1482,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(TP > 64 ? 2 
   int p;
   {
     float mg;
-    mp_rule_wave(s_lamf, s_ef, s_rcf, s_dsfx, T, N, kErrFp64Route, lane, &p, &mg);
+    if (RULE64)
+      mp_rule_wave64(s_lam, s_dm, s_dm + TP, T, N, lane, &p, &mg);
+    else
+      mp_rule_wave(s_lamf, s_ef, s_rcf, s_dsfx, T, N, kErrFp64Route, lane, &p, &mg);
     if (lane == 0) {
       rank[b] = p;
       weight[b] = (float)(1.0 / (2.0 + (double)p));
@@ -1618,6 +1743,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(TP > 64 ? 2 
         if (gc + 8 * n < T) s_z[gr * LD + gc + 8 * n] = zr[n];
     }
     __syncthreads();
+    if (RULE64 && Gb[(T - 1) * T + T - 1] != 0.0) {  // reduced with the frames reversed
+      const int half = T / 2;
+      for (int q = lane; q < gn * half; q += 64) {
+        const int g = q / half, i = q - g * half;
+        const double x = s_z[g * LD + i];
+        s_z[g * LD + i] = s_z[g * LD + T - 1 - i];
+        s_z[g * LD + T - 1 - i] = x;
+      }
+      __syncthreads();
+    }
     if (Fb) write_factors(Fb, s_z, LD, g0, gn, p, T, TP, 1.0 / (2.0 + (double)p), lane);
     // P[l][c] (+)= sum_g x_g[l] x_g[c]; rank > 8 accumulates over passes in
     // global memory (each lane owns its rows, so no other lane touches them)
@@ -1643,16 +1778,32 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(TP > 64 ? 2 
   if (Fb) zero_factor_slots(Fb, p, TP, lane);
 }
 
+// Modes of the three-kernel solve: PLAIN (FP64 reduction, step 0's MP rule),
+// MIXED (FP32 reduction of a float32 Gram, FP64 rule), RESOLVE (FP64
+// reduction, FP64 rule).
+enum RowsMode { kRowsPlain, kRowsMixed, kRowsResolve };
+
+template <int TP, typename S, typename In, bool REV>
+static void launch_tri(double* gram, long long nb, int T, hipStream_t st) {
+  if (TP <= 64)  // one wavefront per matrix; above, four
+    hipLaunchKernelGGL((k_tri_grid<TP <= 64 ? TP : 16, S, In, REV>), dim3((unsigned)nb), dim3(64),
+                       0, st, gram, T, nb);
+  else
+    hipLaunchKernelGGL((k_tri_grid4<TP <= 64 ? 80 : TP, S, In, REV>), dim3((unsigned)nb),
+                       dim3(256), 0, st, gram, T, nb);
+}
+
 template <int TP>
 static int launch_rows(double* gram, long long nb, int T, int N, float* proj, float* fac,
                        int* rank, float* weight, float* margin, float* lam_desc, int* info,
-                       hipStream_t st) {
-  if (TP <= 64)  // one wavefront per matrix; above, four
-    hipLaunchKernelGGL((k_tri_grid<TP <= 64 ? TP : 16>), dim3((unsigned)nb), dim3(64), 0, st,
-                       gram, T, nb);
+                       hipStream_t st, RowsMode mode = kRowsPlain) {
+  // the precision form's solves reduce from the low-energy end of the frames
+  if (mode == kRowsMixed)
+    launch_tri<TP, float, float, true>(gram, nb, T, st);
+  else if (mode == kRowsResolve)
+    launch_tri<TP, double, double, true>(gram, nb, T, st);
   else
-    hipLaunchKernelGGL((k_tri_grid4<TP <= 64 ? 80 : TP>), dim3((unsigned)nb), dim3(256), 0, st,
-                       gram, T, nb);
+    launch_tri<TP, double, double, false>(gram, nb, T, st);
   int rc = launch_status("tridiagonalisation kernel");
   if (rc) return rc;
   // k_eig_vals hands the eigenvalues over in the projector or factor block
@@ -1661,9 +1812,36 @@ static int launch_rows(double* gram, long long nb, int T, int N, float* proj, fl
   hipLaunchKernelGGL((k_eig_vals<TP>), dim3((unsigned)nwave), dim3(64), 0, st, gram, T, nb, proj,
                      fac);
   if ((rc = launch_status("eigenvalue kernel"))) return rc;
-  hipLaunchKernelGGL((k_eig_vecs<TP>), dim3((unsigned)nb), dim3(64), 0, st, gram, T, N, nb, proj,
-                     fac, rank, weight, margin, lam_desc, info);
+  if (mode == kRowsPlain)
+    hipLaunchKernelGGL((k_eig_vecs<TP, false>), dim3((unsigned)nb), dim3(64), 0, st, gram, T, N,
+                       nb, proj, fac, rank, weight, margin, lam_desc, info);
+  else
+    hipLaunchKernelGGL((k_eig_vecs<TP, true>), dim3((unsigned)nb), dim3(64), 0, st, gram, T, N,
+                       nb, proj, fac, rank, weight, margin, lam_desc, info);
   return launch_status("eigenvector kernel");
+}
+
+// The precision form's solves: fp32_reduction = the float32 Gram of every
+// patch (k_gram_h3's output, float32 at the start of each double slot);
+// otherwise the exact FP64 Gram of the re-solved patches. Both with the FP64
+// MP rule.
+int launch_eig_mixed(double* gram, long long nb, int T, int N, float* proj, float* fac, int* rank,
+                     float* weight, float* margin, int* info, hipStream_t st,
+                     bool fp32_reduction) {
+  if (fac && T > kFactorMaxT)
+    return set_err(MPPCA_EUNSUPPORTED, "eig: factors need T <= %d (T = %d)", kFactorMaxT, T);
+  const RowsMode m = fp32_reduction ? kRowsMixed : kRowsResolve;
+#define MPPCA_ROWS(TPV) \
+  return launch_rows<TPV>(gram, nb, T, N, proj, fac, rank, weight, margin, nullptr, info, st, m)
+  if (T <= 16) MPPCA_ROWS(16);
+  if (T <= 32) MPPCA_ROWS(32);
+  if (T <= 48) MPPCA_ROWS(48);
+  if (T <= 64) MPPCA_ROWS(64);
+  if (T <= 80) MPPCA_ROWS(80);
+  if (T <= 96) MPPCA_ROWS(96);
+  if (T <= 112) MPPCA_ROWS(112);
+  MPPCA_ROWS(128);
+#undef MPPCA_ROWS
 }
 
 int launch_eig64_mp(double* gram, long long nb, int T, int N, float* proj, float* fac, int* rank,

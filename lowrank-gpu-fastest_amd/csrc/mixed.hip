@@ -1,0 +1,379 @@
+// mixed.hip -- the precision form's own stages: the FP16 hi/lo three-product
+// Gram on MFMA, the selection of patches whose MP decision is too close to
+// call in that arithmetic, and the hand-back of their exact FP64 re-solve.
+// The reference computes its Gram in FP32 (gram_kernel, step8_tuned.py:32-56);
+// this form keeps FP32 accumulation but feeds the matrix cores FP16 operands.
+#include "common.h"
+
+namespace mppca {
+
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+
+// ---------------------------------------------------------------------------
+// Power-of-two scale of each volume: the max over the kAmaxSlots slots of
+// volume v of max |xc| over its finite entries (float bits; order-preserving
+// as unsigned for values >= 0), written by the centring kernel. With m that
+// max, 2^e * m lies in [2^13, 2^14): the
+// hi halves stay below FP16's largest value (65504) and the lo halves of all
+// but the smallest entries stay normal. e is clamped so that 2^(-2e), the
+// unscaling of the Gram, stays a normal float.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int h3_scale_exp(const unsigned* amax, long long vb, int lane) {
+  static_assert(kAmaxSlots <= 64, "one slot per lane");
+  unsigned u = lane < kAmaxSlots ? amax[vb * kAmaxSlots + lane] : 0u;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned x = __shfl_xor(u, o);
+    u = x > u ? x : u;
+  }
+  const float m = __uint_as_float(u);
+  if (!(m > 0.f) || !(m < INFINITY)) return 0;
+  const int e = 14 - __builtin_amdgcn_frexp_expf(m);  // m = f 2^x, f in [1/2, 1)
+  return e < -60 ? -60 : (e > 60 ? 60 : e);
+}
+
+// ---------------------------------------------------------------------------
+// Gram G_b = Xc_b^T Xc_b of each patch on v_mfma_f32_16x16x32_f16. Each float
+// x of the scaled patch splits as hi = fp16(x), lo = fp16(x - hi) (x = hi + lo
+// to 2^-22 relative); G = hi lo^T + lo hi^T + hi hi^T, the lo lo^T term
+// (2^-22 relative) dropped.
+// FP32 accumulation, compensated: the MFMAs accumulate kH3Run steps of 32
+// voxels into a fresh register set, which is then added to the running sum
+// with an error-free two-sum whose rounding errors collect in a third set.
+// Accumulating all of N (42 steps, 126 MFMAs at 11^3) in one register set
+// carried ~sqrt(126) u of the entries' size, which moved the output by 2e-6
+// against the FP64 oracle; compensated, the Gram is within ~u of exact.
+//
+// NW wavefronts per patch, 4 / NW patches per 256-thread block: each wave of
+// a patch forms every NW-th 16 x 16 block of the upper block triangle (all
+// waves load the whole slice; the extra reads hit the caches). MFMA 16x16x32
+// operand layout: lane l supplies 8 consecutive k of row l % 16 of A and of
+// column l % 16 of B, at the same k for both, so for G = X^T X both operands
+// are one register: lane l holds 8 voxels (n0 + 8 (l / 16) + e) of frame
+// 16 tb + l % 16. The 16 x 16 FP32 result sits in 4 registers per lane:
+// register r holds row 4 (l / 16) + r, column l % 16.
+// Output: the float32 Gram at the start of each patch's double slot
+// (T * T doubles), which the FP32 Householder reduction reads in place.
+// ---------------------------------------------------------------------------
+constexpr int kH3Pad = 64;  // table padding: the prefetch past the last step reads up to N + 63
+constexpr int kH3Run = 4;   // MFMA steps per uncompensated run
+
+__host__ __device__ constexpr int h3_waves(int ntb) { return ntb <= 4 ? 1 : ntb <= 6 ? 2 : 4; }
+
+template <int NTB>
+__global__ __launch_bounds__(256) void k_gram_h3(const float* __restrict__ xc,
+                                                 const unsigned* __restrict__ amax,
+                                                 double* __restrict__ gram, Geom g, int per_xcd) {
+  constexpr int NW = h3_waves(NTB), PPB = 4 / NW;  // waves per patch, patches per block
+  constexpr int NBLK = NTB * (NTB + 1) / 2, NACC = (NBLK + NW - 1) / NW;
+  extern __shared__ int s_off[];
+  for (int n = threadIdx.x; n < g.N + kH3Pad; n += blockDim.x) {
+    const int oz = n % g.pz, r = n / g.pz;
+    const int oy = r % g.py, ox = r / g.py;
+    s_off[n] = n < g.N ? (ox * g.Y + oy) * g.Z + oz : 0;
+  }
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int wsub = wave % NW;  // this wave's share of the blocks: k % NW == wsub
+  // XCD-contiguous runs of patches (neighbours share most voxels, and an L2)
+  const long long blk = (long long)(blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+  const long long bl = blk * PPB + wave / NW;
+  if (bl >= (long long)g.P * g.nvol) return;
+  const int T = g.T, N = g.N;
+  const int i = lane & 15, kq = lane >> 4;
+  const long long vb = bl / g.P;
+  const float* xb = xc + vb * g.nvox * T + patch_base(g, bl - vb * g.P) * T;
+  const int ex = h3_scale_exp(amax, vb, lane);
+  const float scale = __builtin_amdgcn_ldexpf(1.f, ex);
+  float tmask[NTB];  // the scale, 0 past T
+#pragma unroll
+  for (int tb = 0; tb < NTB; ++tb) tmask[tb] = tb * 16 + i < T ? scale : 0.f;
+
+  floatx4 run[NACC], sum[NACC], cmp[NACC];
+#pragma unroll
+  for (int a = 0; a < NACC; ++a) {
+    run[a] = floatx4{0.f, 0.f, 0.f, 0.f};
+    sum[a] = floatx4{0.f, 0.f, 0.f, 0.f};
+    cmp[a] = floatx4{0.f, 0.f, 0.f, 0.f};
+  }
+  auto fold = [&]() {  // sum + run -> sum, exactly up to the error kept in cmp
+#pragma unroll
+    for (int a = 0; a < NACC; ++a)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const float x = sum[a][r], y = run[a][r];
+        const float s = x + y, bp = s - x;
+        cmp[a][r] += (x - (s - bp)) + (y - bp);
+        sum[a][r] = s;
+        run[a][r] = 0.f;
+      }
+  };
+  // Branch-free loads (padded voxel table, frame clamped to T-1); the masks
+  // apply when a step is consumed, so the next step's loads stay in flight.
+  float nxt[8][NTB];
+  auto load = [&](int n) {  // n + 7 < N + kH3Pad
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const float* row = xb + (long long)s_off[n + e] * T;
+#pragma unroll
+      for (int tb = 0; tb < NTB; ++tb) {
+        const int t = tb * 16 + i;
+        nxt[e][tb] = row[t < T ? t : T - 1];
+      }
+    }
+  };
+  load(8 * kq);
+  int step = 0;
+  for (int n0 = 0; n0 < N; n0 += 32) {
+    half8 hi[NTB], lo[NTB];
+    const int nv = n0 + 8 * kq;
+#pragma unroll
+    for (int e = 0; e < 8; ++e)
+#pragma unroll
+      for (int tb = 0; tb < NTB; ++tb) {
+        const float x = nxt[e][tb] * (nv + e < N ? tmask[tb] : 0.f);
+        const _Float16 h = (_Float16)x;
+        hi[tb][e] = h;
+        lo[tb][e] = (_Float16)(x - (float)h);
+      }
+    load(n0 + 32 + 8 * kq);  // in flight under the MFMAs
+    int k = 0;
+#pragma unroll
+    for (int tb1 = 0; tb1 < NTB; ++tb1)
+#pragma unroll
+      for (int tb2 = tb1; tb2 < NTB; ++tb2, ++k) {
+        if (NW > 1 && k % NW != wsub) continue;
+        floatx4& c = run[k / NW];
+        c = __builtin_amdgcn_mfma_f32_16x16x32_f16(hi[tb1], lo[tb2], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_f16(lo[tb1], hi[tb2], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_f16(hi[tb1], hi[tb2], c, 0, 0, 0);
+      }
+    if (++step == kH3Run) {
+      fold();
+      step = 0;
+    }
+  }
+  fold();
+
+  const float unscale = __builtin_amdgcn_ldexpf(1.f, -2 * ex);
+  float* Gb = (float*)(gram + bl * (long long)T * T);
+  int k = 0;
+#pragma unroll
+  for (int tb1 = 0; tb1 < NTB; ++tb1)
+#pragma unroll
+    for (int tb2 = tb1; tb2 < NTB; ++tb2, ++k) {
+      if (NW > 1 && k % NW != wsub) continue;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int ti = tb1 * 16 + 4 * kq + r, tj = tb2 * 16 + i;
+        if (ti < T && tj < T) {
+          const float v = (sum[k / NW][r] + cmp[k / NW][r]) * unscale;
+          Gb[ti * T + tj] = v;
+          if (tb1 != tb2) Gb[tj * T + ti] = v;
+        }
+      }
+    }
+}
+
+template <int NTB>
+static int launch_h3_t(const float* xc, const unsigned* amax, double* gram, const Geom& g,
+                       hipStream_t st) {
+  constexpr int PPB = 4 / h3_waves(NTB);
+  const long long blocks = ((long long)g.P * g.nvol + PPB - 1) / PPB;
+  const long long per_xcd = (blocks + 7) / 8;
+  if (per_xcd > (1LL << 28)) return set_err(MPPCA_EUNSUPPORTED, "gram: too many patches");
+  hipLaunchKernelGGL((k_gram_h3<NTB>), dim3((unsigned)(8 * per_xcd)), dim3(256),
+                     (g.N + kH3Pad) * sizeof(int), st, xc, amax, gram, g, (int)per_xcd);
+  return launch_status("fp16x3 gram kernel");
+}
+
+int launch_gram_h3(const float* xc, const unsigned* amax, double* gram, const Geom& g,
+                   hipStream_t st) {
+  switch (ntb_of(g.T)) {
+    case 1: return launch_h3_t<1>(xc, amax, gram, g, st);
+    case 2: return launch_h3_t<2>(xc, amax, gram, g, st);
+    case 3: return launch_h3_t<3>(xc, amax, gram, g, st);
+    case 4: return launch_h3_t<4>(xc, amax, gram, g, st);
+    case 5: return launch_h3_t<5>(xc, amax, gram, g, st);
+    case 6: return launch_h3_t<6>(xc, amax, gram, g, st);
+    case 7: return launch_h3_t<7>(xc, amax, gram, g, st);
+    default: return launch_h3_t<8>(xc, amax, gram, g, st);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Exact FP64 Gram of the re-solved patches: patch list[s] into slot s, for
+// s < *count; slots from *count to cap get a NaN first entry (flagged and
+// skipped by the eigensolver). A list holds few patches, so each gets a whole
+// workgroup of NWG wavefronts: wave w takes the 4-voxel slices w, w + NWG, ...
+// (v_mfma_f64_16x16x4f64 on exact float -> double products, as the per-patch
+// FP64 Gram), and the NWG partial tiles are summed through LDS in wave order,
+// so the result does not depend on timing.
+// ---------------------------------------------------------------------------
+__host__ __device__ constexpr int list_waves(int ntb) { return ntb <= 4 ? 16 : 4; }
+
+template <int NTB>
+__global__ __launch_bounds__(64 * list_waves(NTB)) void k_gram64_list(
+    const float* __restrict__ xc, double* __restrict__ gram, Geom g,
+    const int* __restrict__ list, const int* __restrict__ count, int cap) {
+  constexpr int NWG = list_waves(NTB), NACC = NTB * (NTB + 1) / 2;
+  __shared__ double s_t[NWG][256];  // one 16 x 16 partial tile per wave
+  const int s = blockIdx.x;
+  if (s >= cap) return;
+  const int T = g.T, N = g.N;
+  double* Gb = gram + (long long)s * T * T;
+  if (s >= *count) {
+    if (threadIdx.x == 0) Gb[0] = NAN;
+    return;
+  }
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int i = lane & 15, kq = lane >> 4;
+  const long long q = list[s];
+  const long long vb = q / g.P;
+  const float* xb = xc + vb * g.nvox * T + patch_base(g, q - vb * g.P) * T;
+  doublex4 acc[NACC];
+#pragma unroll
+  for (int k = 0; k < NACC; ++k) acc[k] = doublex4{0.0, 0.0, 0.0, 0.0};
+  for (int n0 = 4 * wave; n0 < N; n0 += 4 * NWG) {
+    const int n = n0 + kq;
+    double a[NTB];
+    if (n < N) {
+      const int oz = n % g.pz, r = n / g.pz;
+      const float* row = xb + (long long)((r / g.py * g.Y + r % g.py) * g.Z + oz) * T;
+#pragma unroll
+      for (int tb = 0; tb < NTB; ++tb) a[tb] = tb * 16 + i < T ? (double)row[tb * 16 + i] : 0.0;
+    } else {
+#pragma unroll
+      for (int tb = 0; tb < NTB; ++tb) a[tb] = 0.0;
+    }
+    int k = 0;
+#pragma unroll
+    for (int tb1 = 0; tb1 < NTB; ++tb1)
+#pragma unroll
+      for (int tb2 = tb1; tb2 < NTB; ++tb2, ++k)
+        acc[k] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[tb1], a[tb2], acc[k], 0, 0, 0);
+  }
+  // f64 16x16 result: register r of lane l holds row l / 16 + 4 r, column l % 16
+  int k = 0;
+#pragma unroll
+  for (int tb1 = 0; tb1 < NTB; ++tb1)
+#pragma unroll
+    for (int tb2 = tb1; tb2 < NTB; ++tb2, ++k) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) s_t[wave][(kq + 4 * r) * 16 + i] = acc[k][r];
+      __syncthreads();
+      for (int e = threadIdx.x; e < 256; e += 64 * NWG) {
+        double v = s_t[0][e];
+        for (int w = 1; w < NWG; ++w) v += s_t[w][e];
+        const int ti = tb1 * 16 + e / 16, tj = tb2 * 16 + e % 16;
+        if (ti < T && tj < T) {
+          Gb[ti * T + tj] = v;
+          if (tb1 != tb2) Gb[tj * T + ti] = v;
+        }
+      }
+      __syncthreads();
+    }
+}
+
+template <int NTB>
+static int launch_list_t(const float* xc, double* gram, const Geom& g, const int* list,
+                         const int* count, int cap, hipStream_t st) {
+  hipLaunchKernelGGL((k_gram64_list<NTB>), dim3((unsigned)cap), dim3(64 * list_waves(NTB)), 0,
+                     st, xc, gram, g, list, count, cap);
+  return launch_status("re-solve gram kernel");
+}
+
+static int launch_gram64_list(const float* xc, double* gram, const Geom& g,
+                                    const int* list, const int* count, int cap,
+                                    hipStream_t st) {
+  switch (ntb_of(g.T)) {
+    case 1: return launch_list_t<1>(xc, gram, g, list, count, cap, st);
+    case 2: return launch_list_t<2>(xc, gram, g, list, count, cap, st);
+    case 3: return launch_list_t<3>(xc, gram, g, list, count, cap, st);
+    case 4: return launch_list_t<4>(xc, gram, g, list, count, cap, st);
+    case 5: return launch_list_t<5>(xc, gram, g, list, count, cap, st);
+    case 6: return launch_list_t<6>(xc, gram, g, list, count, cap, st);
+    case 7: return launch_list_t<7>(xc, gram, g, list, count, cap, st);
+    default: return launch_list_t<8>(xc, gram, g, list, count, cap, st);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Re-solve selection: every patch solved without fault (status 0) whose
+// relative MP margin is below the threshold joins the list (order of arrival;
+// each patch's result does not depend on its slot). Past the list's capacity
+// a patch keeps its precision-form result and gets status -3.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_resolve_select(const float* __restrict__ margin,
+                                                        int* __restrict__ info, long long P,
+                                                        float thr, int* __restrict__ list,
+                                                        int* __restrict__ count, int cap) {
+  const long long b = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= P) return;
+  if (info[b] != 0 || !(margin[b] < thr)) return;
+  const int slot = atomicAdd(count, 1);
+  if (slot < cap)
+    list[slot] = (int)b;
+  else
+    info[b] = -3;
+}
+
+// The re-solved patches' outputs back to their own slots: one block per list
+// slot copies the factor (or projector) block of blk floats and the per-patch
+// values; status 1 marks a re-solved patch.
+__global__ __launch_bounds__(256) void k_resolve_scatter(
+    const int* __restrict__ list, const int* __restrict__ count, int cap,
+    const float* __restrict__ rblk, const int* __restrict__ rrank,
+    const float* __restrict__ rweight, const float* __restrict__ rmargin,
+    const int* __restrict__ rinfo, long long blk, float* __restrict__ out_blk,
+    int* __restrict__ rank, float* __restrict__ weight, float* __restrict__ margin,
+    int* __restrict__ info) {
+  const int s = blockIdx.x;
+  const int n = *count < cap ? *count : cap;
+  if (s >= n) return;
+  const long long b = list[s];
+  const float* src = rblk + s * blk;
+  float* dst = out_blk + b * blk;
+  for (long long e = threadIdx.x; e < blk; e += blockDim.x) dst[e] = src[e];
+  if (threadIdx.x == 0) {
+    rank[b] = rrank[s];
+    weight[b] = rweight[s];
+    margin[b] = rmargin[s];
+    info[b] = rinfo[s] == 0 ? 1 : rinfo[s];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// The precision form from the centred volume to the rank-p factors (T <= 64)
+// or projectors: FP16x3 Gram, FP32 reduction + FP64 MP rule, selection, exact
+// FP64 Gram and FP64 solve of the selected patches, hand-back.
+// ---------------------------------------------------------------------------
+int run_mixed_solve(const Geom& g, const Workspace& w, int* rank, float* margin, int* info,
+                    hipStream_t st) {
+  int rc;
+  const long long P = (long long)g.P * g.nvol;
+  const int T = g.T;
+  if ((rc = launch_gram_h3(w.xc, w.amax, (double*)w.gram, g, st))) return rc;
+  if ((rc = launch_eig_mixed((double*)w.gram, P, T, g.N, w.proj, w.fac, rank, w.weight, margin,
+                             info, st, true)))
+    return rc;
+  // the list counter starts at 0 (a fill kernel, so that the sequence can be
+  // captured in a graph)
+  if ((rc = zero_fill((float*)w.rcount, 1, st))) return rc;
+  const int cap = w.rcap;
+  hipLaunchKernelGGL(k_resolve_select, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, st,
+                     margin, info, P, (float)MPPCA_MIXED_RESOLVE_MARGIN, w.rlist, w.rcount, cap);
+  if ((rc = launch_status("re-solve selection"))) return rc;
+  if ((rc = launch_gram64_list(w.xc, w.rgram, g, w.rlist, w.rcount, cap, st))) return rc;
+  const bool fac = w.fac != nullptr;
+  if ((rc = launch_eig_mixed(w.rgram, cap, T, g.N, fac ? nullptr : w.rblk, fac ? w.rblk : nullptr,
+                             w.rrank, w.rweight, w.rmargin, w.rinfo, st, false)))
+    return rc;
+  const long long blk = fac ? factor_floats(T) : (long long)T * T;
+  hipLaunchKernelGGL(k_resolve_scatter, dim3((unsigned)cap), dim3(256), 0, st, w.rlist, w.rcount,
+                     cap, w.rblk, w.rrank, w.rweight, w.rmargin, w.rinfo, blk,
+                     fac ? w.fac : w.proj, rank, w.weight, margin, info);
+  return launch_status("re-solve hand-back");
+}
+
+}  // namespace mppca

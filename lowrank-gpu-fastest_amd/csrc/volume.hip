@@ -41,14 +41,25 @@ int zero_fill(float* p, long long n, hipStream_t st) {
 // Temporal centring. A block owns 64 consecutive voxels = 64*T contiguous
 // floats: staged through LDS with coalesced loads, one thread per voxel forms
 // the mean, then xc = x - mean streams out coalesced.
+// amax (optional, the precision form): max |xc| over the finite entries of
+// each volume of nvox1 voxels, as float bits (atomicMax on unsigned keeps the
+// order of values >= 0), spread over kAmaxSlots slots per volume by block
+// index so that the atomics do not queue on one address; zeroed beforehand.
 // ---------------------------------------------------------------------------
 constexpr int kCentreVox = 64;
+
+__device__ __forceinline__ float wave_max_f(float x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x = fmaxf(x, __shfl_xor(x, o));
+  return x;
+}
 
 template <bool VEC>
 __global__ __launch_bounds__(256) void k_centre(const float* __restrict__ vol,
                                                 float* __restrict__ xc,
                                                 float* __restrict__ mean, long long nvox,
-                                                int T) {
+                                                int T, unsigned* __restrict__ amax,
+                                                long long nvox1) {
   extern __shared__ float s_x[];  // kCentreVox * T (+ kCentreVox means)
   float* s_m = s_x + kCentreVox * T;
   const long long v0 = (long long)blockIdx.x * kCentreVox;
@@ -93,28 +104,57 @@ __global__ __launch_bounds__(256) void k_centre(const float* __restrict__ vol,
   }
   __syncthreads();
   float* dst = xc + v0 * T;
+  float tmax = 0.f;  // max |xc| of this thread's finite entries
+  auto fin_abs = [](float x) { return fabsf(x) < INFINITY ? fabsf(x) : 0.f; };
   if (VEC) {
     const float4* x4 = (const float4*)s_x;
     float4* d4 = (float4*)dst;
     for (int e = threadIdx.x; e < ne / 4; e += blockDim.x) {
       const float m = s_m[(4 * e) / T];
       const float4 q = x4[e];
-      d4[e] = make_float4(q.x - m, q.y - m, q.z - m, q.w - m);
+      const float4 c = make_float4(q.x - m, q.y - m, q.z - m, q.w - m);
+      d4[e] = c;
+      tmax = fmaxf(fmaxf(tmax, fmaxf(fin_abs(c.x), fin_abs(c.y))),
+                   fmaxf(fin_abs(c.z), fin_abs(c.w)));
     }
   } else {
-    for (int e = threadIdx.x; e < ne; e += blockDim.x) dst[e] = s_x[e] - s_m[e / T];
+    for (int e = threadIdx.x; e < ne; e += blockDim.x) {
+      const float c = s_x[e] - s_m[e / T];
+      dst[e] = c;
+      tmax = fmaxf(tmax, fin_abs(c));
+    }
+  }
+  if (amax) {
+    const long long va = v0 / nvox1, vz = (v0 + nv - 1) / nvox1;
+    if (va == vz) {
+      tmax = wave_max_f(tmax);
+      if ((threadIdx.x & 63) == 0)
+        atomicMax(amax + va * kAmaxSlots + (blockIdx.x & (kAmaxSlots - 1)),
+                  __float_as_uint(tmax));
+    } else {  // the block straddles volumes: per entry (at most nvol - 1 such blocks)
+      for (int e = threadIdx.x; e < ne; e += blockDim.x)
+        atomicMax(amax + (v0 + e / T) / nvox1 * kAmaxSlots,
+                  __float_as_uint(fin_abs(s_x[e] - s_m[e / T])));
+    }
   }
 }
 
 int launch_centre(const float* vol, float* xc, float* mean, long long nvox, int T,
-                  hipStream_t st) {
+                  hipStream_t st, unsigned* amax, long long nvox1) {
   const unsigned blocks = (unsigned)((nvox + kCentreVox - 1) / kCentreVox);
   const size_t lds = (size_t)kCentreVox * (T + 1) * sizeof(float);
   const bool vec = T % 4 == 0 && (uintptr_t)vol % 16 == 0 && (uintptr_t)xc % 16 == 0;
+  if (amax) {
+    if (nvox1 <= 0) return set_err(MPPCA_EINVAL, "centre: volume size %lld", nvox1);
+    const int rc = zero_fill((float*)amax, nvox / nvox1 * kAmaxSlots, st);
+    if (rc) return rc;
+  }
   if (vec)
-    hipLaunchKernelGGL(k_centre<true>, dim3(blocks), dim3(256), lds, st, vol, xc, mean, nvox, T);
+    hipLaunchKernelGGL(k_centre<true>, dim3(blocks), dim3(256), lds, st, vol, xc, mean, nvox, T,
+                       amax, nvox1);
   else
-    hipLaunchKernelGGL(k_centre<false>, dim3(blocks), dim3(256), lds, st, vol, xc, mean, nvox, T);
+    hipLaunchKernelGGL(k_centre<false>, dim3(blocks), dim3(256), lds, st, vol, xc, mean, nvox, T,
+                       amax, nvox1);
   return launch_status("centre kernel");
 }
 

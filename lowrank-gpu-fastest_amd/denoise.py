@@ -28,7 +28,13 @@ from .geometry import check_geometry, num_patches
 # "plain": FP64-accumulated MFMA Gram + FP64 eigensolver, FP32 reconstruction
 #          (matches step 0 to ~1e-7 on any input); the default.
 # "fp32":  the reference's all-FP32 arithmetic (FP32 MFMA Gram, FP32 Jacobi).
-FORMS = ("plain", "fp32")
+# "mixed": the precision form: FP16 hi/lo three-product Gram on MFMA (FP32
+#          accumulation), FP32 Householder reduction, FP64 MP decision, and an
+#          exact FP64 re-solve of every patch whose relative MP margin is below
+#          1e-4; rank-p factors and FP32 reconstruction as "plain". Stated
+#          against the FP64 oracle (step 0 on float64 data).
+FORMS = ("plain", "fp32", "mixed")
+FACTOR_FORMS = ("plain", "mixed")  # forms with the rank-p factor route (T <= 64)
 FACTOR_MAX_T = 64
 
 
@@ -77,7 +83,8 @@ class Denoiser:
         self.out_d = torch.empty(self.shape, dtype=torch.float32, device=dev)
         # plain form, T <= 64: eigensolver -> rank-p factors -> fused recon + normalise;
         # otherwise projector -> atomic scatter -> normalise
-        self.route = "factors" if form == "plain" and self.shape[3] <= FACTOR_MAX_T else "projector"
+        self.route = ("factors" if form in FACTOR_FORMS and self.shape[3] <= FACTOR_MAX_T
+                      else "projector")
         self.rank = torch.empty(self.n_patches, dtype=torch.int32, device=dev)
         self.margin = torch.empty(self.n_patches, dtype=torch.float32, device=dev)
         self.sweeps = torch.empty(self.n_patches, dtype=torch.int32, device=dev)
@@ -124,13 +131,16 @@ class Denoiser:
         return ("centre", "gram", "eigh_mp", "zero", "recon", "normalize")
 
     def run_staged(self, vol_d, out_d, mark=None, max_sweeps=20):
-        """Same kernels as :meth:`run_device`, launched stage by stage.
+        """Same kernels as :meth:`run_device`, launched stage by stage (the
+        plain and fp32 forms; the mixed form runs as one call).
 
         ``mark(name)`` (optional) is called before each stage and once more
         with ``None`` after the last, so a caller can record events between
         stages on the current stream. Intermediates stay available in
         ``self.stage_buffers`` for inspection.
         """
+        if self.form == "mixed":
+            raise NotImplementedError("the mixed form runs as one call (run_device)")
         torch = self.torch
         x, y, z, t = self.shape
         nvox = x * y * z
@@ -203,7 +213,9 @@ class Denoiser:
     # -- per-patch status --------------------------------------------------
     def check_status(self, on_bad="raise"):
         """Check the per-patch solver status of the last call (a LAPACK-style
-        ``info``: -1 the Jacobi sweeps did not converge, -2 a non-finite patch).
+        ``info``: -1 the Jacobi sweeps did not converge, -2 a non-finite patch,
+        -3 a mixed-form patch whose MP margin asked for an FP64 re-solve past
+        the re-solve list's capacity; 1 marks a patch that was re-solved).
 
         ``on_bad``: "raise" (:class:`SolverStatusError`), "warn" or "ignore".
         Waits for the device work of the last call. Step 0 raises on such
@@ -217,9 +229,10 @@ class Denoiser:
         if worst >= 0:
             return
         st = self.sweeps.cpu().numpy()
-        nan, nconv = int((st == -2).sum()), int((st == -1).sum())
-        msg = (f"{nan} patch(es) hold NaN or Inf, {nconv} did not converge "
-               f"(of {self.n_patches}; first at patch {int(np.flatnonzero(st < 0)[0])})")
+        nan, nconv, over = int((st == -2).sum()), int((st == -1).sum()), int((st == -3).sum())
+        msg = (f"{nan} patch(es) hold NaN or Inf, {nconv} did not converge, {over} past the "
+               f"re-solve capacity (of {self.n_patches}; first at patch "
+               f"{int(np.flatnonzero(st < 0)[0])})")
         if on_bad == "raise":
             raise SolverStatusError(msg, st)
         warnings.warn(msg, RuntimeWarning, stacklevel=3)
