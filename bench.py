@@ -250,13 +250,13 @@ def validate(vol, clean, res, info, patch, stride, workers):
     return out, oracle_s
 
 
-def serve_run(pkg, volumes, rate_hz, n_requests, burst_n=None):
+def serve_run(pkg, volumes, rate_hz, n_requests, burst_n=None, form="plain"):
     """Continuous-batching server (BatchServer) over ``volumes`` (a list of
     (vol, patch, stride)): burst throughput (every request submitted at once)
     and latency under Poisson arrivals at ``rate_hz`` (arrival to the result in
     host memory)."""
     burst_n = burst_n or max(24, 4 * len(volumes))
-    with pkg.BatchServer() as srv:
+    with pkg.BatchServer(form=form) as srv:
         for f in [srv.submit(v, p, s, keep=False) for v, p, s in volumes for _ in range(2)]:
             f.result()  # lanes allocated, graphs captured
         t0 = time.perf_counter()
@@ -274,8 +274,104 @@ def serve_run(pkg, volumes, rate_hz, n_requests, burst_n=None):
             "max_ms": round(r["latency_ms"]["max"], 3),
             "volumes_per_s": round(r["volumes_per_s"], 2),
             "burst_volumes_per_s": round(burst_vps, 2), "burst_requests": burst_n,
-            "mean_batch": round(nv / max(nb, 1), 2),
+            "mean_batch": round(nv / max(nb, 1), 2), "form": form,
             "latency": "arrival to host result, host->host"}
+
+
+def precision_run(pkg, name, K, W, workers, plain=None, eig_sample=400):
+    """The precision form (form="mixed") on a named workload: host->host and
+    device-resident ms per volume beside the plain form's, and its error
+    against the FP64 oracle (step 0 on float64 data, outside the timed
+    region): output relative L2 error, per-patch rank agreement, and the
+    eigenvalues of the form's FP32 solve against FP64 singular values on a
+    random sample of patches."""
+    import torch
+
+    import oracle
+
+    vol, _, patch, stride = pkg.workload_volume(name)
+    out = {"workload": name, "shape": list(vol.shape), "patch": list(patch), "stride": stride}
+
+    def time_form(form):
+        d = pkg.Denoiser(vol.shape, patch, stride, form=form)
+        for _ in range(max(W, 1)):
+            r = d(vol)
+        lat = []
+        for _ in range(K):
+            t0 = time.perf_counter()
+            r = d(vol)
+            lat.append((time.perf_counter() - t0) * 1e3)
+        vd = torch.from_numpy(vol).cuda()
+        od = torch.empty_like(vd)
+        for _ in range(max(W, 1)):
+            d.run_device(vd, od)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(K):
+            d.run_device(vd, od)
+        torch.cuda.synchronize()
+        dev = (time.perf_counter() - t0) * 1e3 / K
+        return d, r, float(np.mean(lat)), float(np.percentile(lat, 99)), dev
+
+    d, res, h2h, h2h99, dev = time_form("mixed")
+    info = d.info()
+    out["mixed"] = {"host_to_host_ms": round(h2h, 4), "host_to_host_p99_ms": round(h2h99, 3),
+                    "device_resident_ms": round(dev, 4),
+                    "device_resident_volumes_per_s": round(1e3 / dev, 2),
+                    "resolved_fp64": int((info["sweeps"] == 1).sum()),
+                    "status_negative": int((info["sweeps"] < 0).sum())}
+    if plain is None:
+        _, _, ph, ph99, pdev = time_form("plain")
+        plain = {"host_to_host_ms": round(ph, 4), "device_resident_ms": round(pdev, 4)}
+    out["plain"] = plain
+    out["device_speedup_vs_plain"] = round(plain["device_resident_ms"] / dev, 3)
+    out["host_to_host_speedup_vs_plain"] = round(plain["host_to_host_ms"] / h2h, 3)
+
+    # error against the FP64 oracle
+    v64 = vol.astype(np.float64)
+    ref, ranks, margins = oracle.denoise_parallel(v64, patch, stride, workers=workers)
+    bad = np.flatnonzero(info["rank"] != ranks)
+    err = {"rel_err_vs_fp64_oracle": float(np.linalg.norm(res - ref) / np.linalg.norm(ref)),
+           "n_patches": int(len(ranks)), "rank_mismatches": int(bad.size),
+           "rank_mismatches_oracle_margin_ge_1e-9": int((margins[bad] >= 1e-9).sum())}
+    # eigenvalues of the first (FP32) solve against FP64 singular values
+    lib = pkg._lib.load()
+    x, y, z, t = vol.shape
+    ws = int(lib.mppca_workspace_bytes(x, y, z, t, *patch, stride, 2))
+    wbuf = torch.empty(ws + 256, dtype=torch.uint8, device="cuda")
+    wptr = (wbuf.data_ptr() + 255) // 256 * 256
+    lam = torch.empty((d.n_patches, t), dtype=torch.float32, device="cuda")
+    vd = torch.from_numpy(vol).cuda()
+    pkg._lib.check(lib.mppca_mixed_eigenvalues(vd.data_ptr(), x, y, z, t, *patch, stride, wptr,
+                                               ws, lam.data_ptr(),
+                                               torch.cuda.current_stream().cuda_stream),
+                   "mppca_mixed_eigenvalues")
+    lam = lam.cpu().numpy().astype(np.float64)
+    corners = pkg.patch_corners(vol.shape, patch, stride)
+    rng = np.random.default_rng(0)
+    pick = rng.choice(len(corners), size=min(eig_sample, len(corners)), replace=False)
+    px, py, pz = patch
+    worst_abs, sig_rel = 0.0, []
+    for b in pick:
+        i, j, k = corners[b]
+        xp = v64[i:i + px, j:j + py, k:k + pz].reshape(-1, t)
+        sv = np.linalg.svd(xp - xp.mean(axis=1, keepdims=True), compute_uv=False)
+        m = min(len(sv), t - 1)
+        l64 = sv[:m] ** 2
+        dl = np.abs(lam[b, :m] - l64)
+        worst_abs = max(worst_abs, float(dl.max() / l64[0]))
+        p = int(ranks[b])
+        if p:
+            sig_rel.append(float((dl[:p] / l64[:p]).max()))
+    err["eigenvalues"] = {
+        "sample_patches": int(len(pick)),
+        "max_abs_err_over_lambda_max": worst_abs,
+        "signal_max_rel_err": max(sig_rel) if sig_rel else None,
+        "signal_median_rel_err": float(np.median(sig_rel)) if sig_rel else None,
+        "note": "the form's first solve (FP16x3 Gram, FP32 reduction) before any FP64 "
+                "re-solve, against the squared singular values of the float64-centred patch"}
+    out["error_vs_fp64_oracle"] = err
+    return out
 
 
 # ---- HBM traffic: rocprofv3 --pmc child runs ---------------------------------
@@ -419,6 +515,8 @@ def main():
     ap.add_argument("--mixed-rate", type=float, default=40.0,
                     help="Poisson arrival rate of the six-shape serving run")
     ap.add_argument("--mixed-requests", type=int, default=80)
+    ap.add_argument("--no-precision", action="store_true",
+                    help="skip the precision-form (mixed) lines")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.pmc_child:
@@ -524,7 +622,7 @@ def main():
         roofline["traffic"] = round(traffic[dom]["fetch_bytes"] + traffic[dom]["write_bytes"])
 
     # ---- serving: continuous batching, Poisson arrivals at a fixed rate -----
-    srv_res = mix_res = None
+    srv_res = mix_res = mix_res_m = None
     if args.serve_requests > 0:
         srv_res = serve_run(pkg, [(vol, patch, stride)], args.serve_rate, args.serve_requests)
         log(f"serving at {args.serve_rate:.0f}/s: p50 {srv_res['p50_ms']:.2f} p99 "
@@ -537,6 +635,12 @@ def main():
             log(f"mixed-shape serving at {args.mixed_rate:.0f}/s: p50 {mix_res['p50_ms']:.2f} "
                 f"p99 {mix_res['p99_ms']:.2f} ms; burst {mix_res['burst_volumes_per_s']:.1f} "
                 "volumes/s")
+            if not args.no_precision:
+                mix_res_m = serve_run(pkg, mixed, args.mixed_rate, args.mixed_requests,
+                                      form="mixed")
+                log(f"mixed-shape serving, precision form: p50 {mix_res_m['p50_ms']:.2f} p99 "
+                    f"{mix_res_m['p99_ms']:.2f} ms; burst "
+                    f"{mix_res_m['burst_volumes_per_s']:.1f} volumes/s")
             del mixed
 
     cpu = host_cpu()
@@ -567,6 +671,7 @@ def main():
         "device_resident_volumes_per_s": round(1e3 / dev_ms, 2),
         "serving_at_fixed_load": srv_res,
         "serving_mixed_shapes": mix_res,
+        "serving_mixed_shapes_precision_form": mix_res_m,
         "kernels": kernels,
         "roofline": roofline,
         "traffic_note": traffic_note if traffic is None else
@@ -584,6 +689,18 @@ def main():
         line["validation"], oracle_s = validate(vol, clean, res, info, patch, stride, workers)
         log(f"rel err vs step 0 {line['validation']['rel_err_vs_step0']:.2e}, rank mismatches "
             f"{line['validation']['rank_mismatches']}")
+    if not args.no_precision and args.form == "plain":
+        prec = {}
+        for name in ("default", "t96"):
+            log(f"precision form on {name}")
+            plain = ({"host_to_host_ms": round(h2h_ms, 4), "device_resident_ms": round(dev_ms, 4)}
+                     if name == args.workload else None)
+            prec[name] = precision_run(pkg, name, K, W, workers, plain)
+            e = prec[name]["error_vs_fp64_oracle"]
+            log(f"  mixed {name}: device {prec[name]['mixed']['device_resident_ms']:.3f} ms "
+                f"(plain {prec[name]['plain']['device_resident_ms']:.3f}), rel err vs FP64 "
+                f"{e['rel_err_vs_fp64_oracle']:.2e}, rank mismatches {e['rank_mismatches']}")
+        line["precision_form"] = prec
     if not args.no_cpu_baseline:
         log(f"cpu baseline (step-0 oracle, 1 BLAS thread, ~{args.cpu_budget:.0f} s)")
         cb = cpu_baseline(vol, patch, stride, args.cpu_budget, threads=1)

@@ -210,6 +210,17 @@ int mppca_normalize(const float* acc, const float* mean, const float* weight,
  * reads them afterwards.
  * ------------------------------------------------------------------------- */
 
+/* Diagnostic of the precision form: the descending eigenvalues of every
+ * patch's Gram as the form's first solve finds them (FP16x3 Gram, FP32
+ * reduction, before any FP64 re-solve), into lam_desc[b * T + j] (float32,
+ * P * T entries), for the error statement against an FP64 eigensolver.
+ * workspace: mppca_workspace_bytes(..., MPPCA_FORM_MIXED) bytes, 256-byte
+ * aligned. No counterpart in the reference (its eigenvalues stay inside
+ * torch.linalg.eigh, step8_tuned.py:134). */
+int mppca_mixed_eigenvalues(const float* vol, int X, int Y, int Z, int T, int px, int py,
+                            int pz, int stride, void* workspace, int64_t workspace_bytes,
+                            float* lam_desc, void* stream);
+
 /* Floats of one patch's factor block for T frames (< 0 if T > 64). */
 int64_t mppca_factor_floats(int T);
 

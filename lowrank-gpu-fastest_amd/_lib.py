@@ -12,7 +12,7 @@ import pathlib
 
 PKG = pathlib.Path(__file__).resolve().parent
 LIB_PATH = PKG / "libmppca.so"
-ABI_VERSION = 104
+ABI_VERSION = 105
 FORM_CODES = {"plain": 0, "fp32": 1, "mixed": 2}
 
 _i32, _i64, _vp, _f = ctypes.c_int, ctypes.c_int64, ctypes.c_void_p, ctypes.c_float
@@ -34,6 +34,7 @@ SIGNATURES = {
     "mppca_normalize": (_i32, [_vp] * 4 + [_i32] * 8 + [_vp]),
     "mppca_factor_floats": (_i64, [_i32]),
     "mppca_eig64_factors": (_i32, [_vp, _i64, _i32, _i32] + [_vp] * 7),
+    "mppca_mixed_eigenvalues": (_i32, [_vp] + [_i32] * 8 + [_vp, _i64, _vp, _vp]),
     "mppca_recon_factors": (_i32, [_vp] * 6 + [_i32] * 8 + [_vp]),
     "mppca_workspace_bytes": (_i64, [_i32] * 9),
     "mppca_denoise_device": (_i32, [_vp, _vp] + [_i32] * 9 + [_vp, _i64, _vp, _vp, _vp, _vp]),

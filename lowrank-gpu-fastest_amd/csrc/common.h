@@ -15,7 +15,7 @@
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef double doublex4 __attribute__((ext_vector_type(4)));
 
-#define MPPCA_ABI_VERSION 104
+#define MPPCA_ABI_VERSION 105
 #define MAX_T 128
 #define MAX_N 4096
 
@@ -274,7 +274,7 @@ int launch_eig64_mp(double* gram, long long nb, int T, int N, float* proj, float
                     int variant = 0);
 // The precision form (mixed.hip, and its solves in eig_tridiag.hip).
 int launch_eig_mixed(double* gram, long long nb, int T, int N, float* proj, float* fac, int* rank,
-                     float* weight, float* margin, int* info, hipStream_t st,
+                     float* weight, float* margin, float* lam_desc, int* info, hipStream_t st,
                      bool fp32_reduction);
 int launch_gram_h3(const float* xc, const unsigned* amax, double* gram, const Geom& g,
                    hipStream_t st);

@@ -1826,13 +1826,13 @@ static int launch_rows(double* gram, long long nb, int T, int N, float* proj, fl
 // otherwise the exact FP64 Gram of the re-solved patches. Both with the FP64
 // MP rule.
 int launch_eig_mixed(double* gram, long long nb, int T, int N, float* proj, float* fac, int* rank,
-                     float* weight, float* margin, int* info, hipStream_t st,
+                     float* weight, float* margin, float* lam_desc, int* info, hipStream_t st,
                      bool fp32_reduction) {
   if (fac && T > kFactorMaxT)
     return set_err(MPPCA_EUNSUPPORTED, "eig: factors need T <= %d (T = %d)", kFactorMaxT, T);
   const RowsMode m = fp32_reduction ? kRowsMixed : kRowsResolve;
 #define MPPCA_ROWS(TPV) \
-  return launch_rows<TPV>(gram, nb, T, N, proj, fac, rank, weight, margin, nullptr, info, st, m)
+  return launch_rows<TPV>(gram, nb, T, N, proj, fac, rank, weight, margin, lam_desc, info, st, m)
   if (T <= 16) MPPCA_ROWS(16);
   if (T <= 32) MPPCA_ROWS(32);
   if (T <= 48) MPPCA_ROWS(48);

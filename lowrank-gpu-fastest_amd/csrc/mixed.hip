@@ -355,7 +355,7 @@ int run_mixed_solve(const Geom& g, const Workspace& w, int* rank, float* margin,
   const int T = g.T;
   if ((rc = launch_gram_h3(w.xc, w.amax, (double*)w.gram, g, st))) return rc;
   if ((rc = launch_eig_mixed((double*)w.gram, P, T, g.N, w.proj, w.fac, rank, w.weight, margin,
-                             info, st, true)))
+                             nullptr, info, st, true)))
     return rc;
   // the list counter starts at 0 (a fill kernel, so that the sequence can be
   // captured in a graph)
@@ -367,7 +367,7 @@ int run_mixed_solve(const Geom& g, const Workspace& w, int* rank, float* margin,
   if ((rc = launch_gram64_list(w.xc, w.rgram, g, w.rlist, w.rcount, cap, st))) return rc;
   const bool fac = w.fac != nullptr;
   if ((rc = launch_eig_mixed(w.rgram, cap, T, g.N, fac ? nullptr : w.rblk, fac ? w.rblk : nullptr,
-                             w.rrank, w.rweight, w.rmargin, w.rinfo, st, false)))
+                             w.rrank, w.rweight, w.rmargin, nullptr, w.rinfo, st, false)))
     return rc;
   const long long blk = fac ? factor_floats(T) : (long long)T * T;
   hipLaunchKernelGGL(k_resolve_scatter, dim3((unsigned)cap), dim3(256), 0, st, w.rlist, w.rcount,
@@ -377,3 +377,29 @@ int run_mixed_solve(const Geom& g, const Workspace& w, int* rank, float* margin,
 }
 
 }  // namespace mppca
+
+extern "C" {
+
+int mppca_mixed_eigenvalues(const float* vol, int X, int Y, int Z, int T, int px, int py, int pz,
+                            int stride, void* workspace, int64_t workspace_bytes,
+                            float* lam_desc, void* stream) {
+  using namespace mppca;
+  Geom g;
+  int rc = make_geom(&g, X, Y, Z, T, px, py, pz, stride);
+  if (rc) return rc;
+  if (!vol || !workspace || !lam_desc) return set_err(MPPCA_EINVAL, "mixed_eigenvalues: null pointer");
+  if ((uintptr_t)workspace % 256 != 0)
+    return set_err(MPPCA_EINVAL, "mixed_eigenvalues: workspace must be 256-byte aligned");
+  const Workspace need = carve_workspace(nullptr, g, MPPCA_FORM_MIXED);
+  if (workspace_bytes < (int64_t)need.bytes)
+    return set_err(MPPCA_ENOSPACE, "mixed_eigenvalues: workspace %lld < %zu bytes",
+                   (long long)workspace_bytes, need.bytes);
+  const Workspace w = carve_workspace((char*)workspace, g, MPPCA_FORM_MIXED);
+  hipStream_t st = (hipStream_t)stream;
+  if ((rc = launch_centre(vol, w.xc, w.mean, g.nvox, T, st, w.amax, g.nvox))) return rc;
+  if ((rc = launch_gram_h3(w.xc, w.amax, (double*)w.gram, g, st))) return rc;
+  return launch_eig_mixed((double*)w.gram, g.P, T, g.N, w.proj, w.fac, w.rank, w.weight, w.margin,
+                          lam_desc, w.info, st, true);
+}
+
+}  // extern "C"

@@ -148,3 +148,20 @@ def test_batch_server_status_and_poisson(pkg):
     want = _alone(pkg, *good)
     for f in res["futures"]:
         np.testing.assert_array_equal(f.result(), want)
+
+
+def test_batch_server_precision_form(pkg):
+    """The precision form through the continuous-batching server: every
+    result equals the volume denoised alone in that form (the scale is per
+    volume and every patch, re-solved or not, is solved on its own)."""
+    vols = [(_volume(s, 80 + i), p, st) for i, (s, p, st) in enumerate(GEOMS)]
+    alone = [pkg.denoise(v, p, st, form="mixed") for v, p, st in vols]
+    order = [0, 1, 2, 3, 1, 0, 2, 0, 0, 3]
+    with pkg.BatchServer(max_batch=3, form="mixed", batch_patches=10 ** 9) as srv:
+        futs = [srv.submit(*vols[k]) for k in order]
+        outs = [f.result(timeout=120) for f in futs]
+    for k, out in zip(order, outs):
+        if GEOMS[k][0][3] <= 64:
+            np.testing.assert_array_equal(out, alone[k])
+        else:
+            np.testing.assert_allclose(out, alone[k], rtol=0, atol=1e-6 * np.abs(alone[k]).max())
