@@ -46,8 +46,8 @@ extern "C" {
  *          for the sliding window over x-y columns, else FP64 MFMA on exact
  *          products of the float32 centred data),
  *          FP64 eigensolver (Householder + bisection + twisted factorisation),
- *          FP32 MFMA reconstruction from the rank-p factors (T <= 64; above
- *          that from the projector). Matches step 0 to ~1e-7.
+ *          FP32 MFMA reconstruction from the rank-p factors (every T up to
+ *          128). Matches step 0 to ~1e-7.
  *   FP32 : the reference's all-FP32 arithmetic (FP32 MFMA Gram, FP32 Jacobi
  *          eigensolver), as step8_tuned.py computes it.
  *   MIXED: the precision form. Gram on FP16 MFMA with each float split into
@@ -191,7 +191,7 @@ int mppca_normalize(const float* acc, const float* mean, const float* weight,
                     int pz, int stride, void* stream);
 
 /* ---------------------------------------------------------------------------
- * Rank-p factor route (T <= 64)
+ * Rank-p factor route (the whole path's reconstruction, T <= 128)
  *
  * Instead of the T x T projector P_b = V_p V_p^T, the eigensolver can write
  * the top-p eigenvectors themselves, which is all the reconstruction needs:
@@ -226,7 +226,7 @@ int64_t mppca_factor_floats(int T);
 
 /* As mppca_eig64_mp, writing the rank-p factors of every patch into
  * factors + b * mppca_factor_floats(T) (16-byte aligned) instead of the
- * projector. gram is clobbered. T <= 64.
+ * projector. gram is clobbered. T <= 128.
  * Replaces eigh + mp_keep + the projector bmm (step8_tuned.py:134-137). */
 int mppca_eig64_factors(double* gram, int64_t nb, int T, int N, float* factors,
                         int32_t* rank, float* weight, float* margin, float* lam_desc,
@@ -238,7 +238,7 @@ int mppca_eig64_factors(double* gram, int64_t nb, int T, int N, float* factors,
  *             / (sum_{b contains v} weight[b]),  0 where no patch covers v,
  * with W summed in increasing patch index (step 0's order). A wavefront owns
  * a 2 x 4 x 4 voxel tile and keeps its sums in registers: no atomics, no
- * zero-initialised accumulator. xc and factors 16-byte aligned; T <= 64.
+ * zero-initialised accumulator. xc and factors 16-byte aligned; T <= 128.
  * Replaces recon_scatter_kernel + the final normalisation
  * (step8_tuned.py:59-86, 139-145; step0_numpy_loop.py:36-39). */
 int mppca_recon_factors(const float* xc, const float* factors, const int32_t* rank,
@@ -284,8 +284,8 @@ int mppca_denoise_device(const float* vol, float* out, int X, int Y, int Z,
  *                     NULL uploads from vol directly (vol pinned).
  *   h_stage_out       pinned host buffer the result is downloaded to and then
  *                     copied from into out; NULL downloads into out (pinned).
- *   nslabs            x-slabs of the pipeline (>= 1; the plain form with
- *                     T <= 64 pipelines, other routes run whole-volume).
+ *   nslabs            x-slabs of the pipeline (>= 1; the plain form
+ *                     pipelines, the other forms run whole-volume).
  *   nthreads          host copy threads (1..256).
  *   s_compute, s_in, s_out  three streams (hipStream_t) for the kernels, the
  *                     uploads and the downloads; distinct non-blocking streams
@@ -311,7 +311,7 @@ int mppca_denoise_host(const float* vol, float* out, int X, int Y, int Z, int T,
  * Per-patch arrays (rank, margin, info) hold nvol * P entries, volume v's
  * from v * P. Volumes of different geometries form different batches.
  * Each volume's result is the one mppca_denoise_device gives it alone, bit
- * for bit on the factor route (plain form, T <= 64).
+ * for bit on the factor route (plain and mixed forms).
  * ------------------------------------------------------------------------- */
 
 #define MPPCA_MAX_BATCH 256

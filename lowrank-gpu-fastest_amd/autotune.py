@@ -85,9 +85,9 @@ def tune(shape, patch=(11, 11, 11), stride=3, batch=1, reps=5, form="plain", vol
     try:
         for field, values in (("gram", [_lib.GRAM_KERNELS[g] for g in GRAMS]),
                               ("eig", [_lib.EIG_LAYOUTS[e] for e in EIGS]),
-                              ("recon_sx", list(TILES) if form == "plain" and T <= 64 else [])):
+                              ("recon_sx", list(TILES) if form != "fp32" else [])):
             if form != "plain" and field != "recon_sx":
-                continue  # the fp32 form has one Gram kernel and one eigensolver
+                continue  # the fp32 and mixed forms have one Gram kernel and one eigensolver
             results = {v: trial(**{field: v}) for v in values}
             ok = {v: t for v, t in results.items() if t is not None}
             if ok:

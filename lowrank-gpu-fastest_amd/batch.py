@@ -15,8 +15,8 @@ per geometry (:func:`denoise_many`).
   (:mod:`.autotune`), or from an explicit :class:`~._lib.Tuning`.
 
 Each volume's result equals ``denoise`` on it alone: bit for bit on the
-factor route (plain form, T <= 64), to float atomic summation order on the
-projector route.
+factor route (plain and mixed forms), to float atomic summation order on
+the projector route (fp32 form).
 """
 
 from __future__ import annotations

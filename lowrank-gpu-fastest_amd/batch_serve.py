@@ -23,7 +23,7 @@ requests overlap the kernels of others and requests that wait are batched:
   checked and its Future resolved.
 
 Each result equals the volume denoised alone (bit for bit on the factor route,
-plain form with T <= 64; up to float atomic summation order otherwise): the
+plain and mixed forms; up to float atomic summation order in the fp32 form): the
 batched kernels compute every volume of a batch exactly as alone.
 """
 

@@ -163,7 +163,7 @@ __device__ inline void mp_rule(const float* lam, float* e, float* rc, int T, int
 // so every filled slot has c%4 < S2; empty slots and frames t >= T hold zeros.
 __host__ __device__ inline int factor_tf(int T) { return 16 * ((T + 15) / 16); }
 __host__ __device__ inline long long factor_floats(int T) { return 2LL * factor_tf(T) * factor_tf(T); }
-constexpr int kFactorMaxT = 64;
+constexpr int kFactorMaxT = 128;
 
 // Patches of rank p <= kFactorPairMaxRank use the compact layout: their q = p
 // vectors fill slots 0..7 only, two groups of S2 = ceil(q/2), so that the
@@ -179,35 +179,35 @@ __device__ __forceinline__ int factor_slot(int g16, int q, int p) {
 }
 
 // Writes vectors g0 .. g0+gn-1 (unit, x_g[t] = z[g*ldz + t]) of a patch of rank
-// p and weight w into its factor block F. Lane t writes frame t (TF <= 64).
+// p and weight w into its factor block F. Lane l writes frames l, l + 64.
 __device__ inline void write_factors(float* F, const double* z, int ldz, int g0, int gn, int p,
                                      int T, int TF, double w, int lane) {
-  if (lane >= TF) return;
-  for (int g = 0; g < gn; ++g) {
-    const int G = g0 + g, nc = G >> 4;
-    const int q = p - 16 * nc < 16 ? p - 16 * nc : 16;
-    const int c = factor_slot(G & 15, q, p);
-    const double v = lane < T ? z[g * ldz + lane] : 0.0;
-    F[nc * 16 * TF + c * TF + lane] = (float)v;
-    F[TF * TF + nc * 16 * TF + lane * 16 + c] = (float)(w * v);
-  }
+  for (int t = lane; t < TF; t += 64)
+    for (int g = 0; g < gn; ++g) {
+      const int G = g0 + g, nc = G >> 4;
+      const int q = p - 16 * nc < 16 ? p - 16 * nc : 16;
+      const int c = factor_slot(G & 15, q, p);
+      const double v = t < T ? z[g * ldz + t] : 0.0;
+      F[nc * 16 * TF + c * TF + t] = (float)v;
+      F[TF * TF + nc * 16 * TF + t * 16 + c] = (float)(w * v);
+    }
 }
 
 // Zeros the slots of the written tiles that no vector fills; a rank-0 patch
 // gets its first tile zeroed, since the reconstruction reads it in a pair.
 __device__ inline void zero_factor_slots(float* F, int p, int TF, int lane) {
-  if (lane >= TF) return;
   const int nct = p > 0 ? (p + 15) >> 4 : 1;
-  for (int nc = 0; nc < nct; ++nc) {
-    const int q = p - 16 * nc < 16 ? p - 16 * nc : 16;
-    const int S2 = factor_groups(q, p);
-    for (int c = 0; c < 16; ++c) {
-      const int a = c >> 2, s = c & 3;
-      if (s < S2 && a * S2 + s < q) continue;  // slot of vector a*S2 + s
-      F[nc * 16 * TF + c * TF + lane] = 0.f;
-      F[TF * TF + nc * 16 * TF + lane * 16 + c] = 0.f;
+  for (int t = lane; t < TF; t += 64)
+    for (int nc = 0; nc < nct; ++nc) {
+      const int q = p - 16 * nc < 16 ? p - 16 * nc : 16;
+      const int S2 = factor_groups(q, p);
+      for (int c = 0; c < 16; ++c) {
+        const int a = c >> 2, s = c & 3;
+        if (s < S2 && a * S2 + s < q) continue;  // slot of vector a*S2 + s
+        F[nc * 16 * TF + c * TF + t] = 0.f;
+        F[TF * TF + nc * 16 * TF + t * 16 + c] = 0.f;
+      }
     }
-  }
 }
 
 // Error models of the two routes (see MpErr).
@@ -221,7 +221,7 @@ struct Workspace {
   float* mean;
   void* gram;   // P*T*T doubles (plain form) or floats (fp32 form)
   float* proj;  // P*T*T floats: the fp32 form (in place of gram) and the plain form above T = 64
-  float* fac;   // P*factor_floats(T) floats: rank-p factors, plain form with T <= 64
+  float* fac;   // P*factor_floats(T) floats: rank-p factors (plain and mixed forms)
   float* weight;
   int* rank;
   float* margin;

@@ -344,8 +344,7 @@ __global__ __launch_bounds__(256) void k_resolve_scatter(
 }
 
 // ---------------------------------------------------------------------------
-// The precision form from the centred volume to the rank-p factors (T <= 64)
-// or projectors: FP16x3 Gram, FP32 reduction + FP64 MP rule, selection, exact
+// The precision form from the centred volume to the rank-p factors: FP16x3 Gram, FP32 reduction + FP64 MP rule, selection, exact
 // FP64 Gram and FP64 solve of the selected patches, hand-back.
 // ---------------------------------------------------------------------------
 int run_mixed_solve(const Geom& g, const Workspace& w, int* rank, float* margin, int* info,

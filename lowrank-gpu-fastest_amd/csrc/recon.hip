@@ -457,8 +457,12 @@ static int launch_recon_lr_sx(const float* xc, const float* fac, const int* rank
     case 2: return launch_recon_lr_t<2, SX>(xc, fac, rank, weight, mean, out, g, x0, x1, st);
     case 3: return launch_recon_lr_t<3, SX>(xc, fac, rank, weight, mean, out, g, x0, x1, st);
     case 4: return launch_recon_lr_t<4, SX>(xc, fac, rank, weight, mean, out, g, x0, x1, st);
+    case 5: return launch_recon_lr_t<5, SX>(xc, fac, rank, weight, mean, out, g, x0, x1, st);
+    case 6: return launch_recon_lr_t<6, SX>(xc, fac, rank, weight, mean, out, g, x0, x1, st);
+    case 7: return launch_recon_lr_t<7, SX>(xc, fac, rank, weight, mean, out, g, x0, x1, st);
+    case 8: return launch_recon_lr_t<8, SX>(xc, fac, rank, weight, mean, out, g, x0, x1, st);
     default:
-      return set_err(MPPCA_EUNSUPPORTED, "recon (factors): T = %d above 64", g.T);
+      return set_err(MPPCA_EUNSUPPORTED, "recon (factors): T = %d above 128", g.T);
   }
 }
 

@@ -34,8 +34,8 @@ from .geometry import check_geometry, num_patches
 #          1e-4; rank-p factors and FP32 reconstruction as "plain". Stated
 #          against the FP64 oracle (step 0 on float64 data).
 FORMS = ("plain", "fp32", "mixed")
-FACTOR_FORMS = ("plain", "mixed")  # forms with the rank-p factor route (T <= 64)
-FACTOR_MAX_T = 64
+FACTOR_FORMS = ("plain", "mixed")  # forms with the rank-p factor route
+FACTOR_MAX_T = 128
 
 
 class SolverStatusError(RuntimeError):
@@ -81,8 +81,8 @@ class Denoiser:
         self._ws_bytes = ws
         self.vol_d = torch.empty(self.shape, dtype=torch.float32, device=dev)
         self.out_d = torch.empty(self.shape, dtype=torch.float32, device=dev)
-        # plain form, T <= 64: eigensolver -> rank-p factors -> fused recon + normalise;
-        # otherwise projector -> atomic scatter -> normalise
+        # plain and mixed forms: eigensolver -> rank-p factors -> fused recon +
+        # normalise; fp32 form: projector -> atomic scatter -> normalise
         self.route = ("factors" if form in FACTOR_FORMS and self.shape[3] <= FACTOR_MAX_T
                       else "projector")
         self.rank = torch.empty(self.n_patches, dtype=torch.int32, device=dev)

@@ -58,11 +58,11 @@ def test_workspace_bytes(lib):
     assert lib.mppca_factor_floats(t) == 2 * 48 * 48
     assert 4 * (nvox * t + nvox + 2 * p * t * t + 2 * p * 48 * 48 + 4 * p) <= ws
     assert ws < 4 * (nvox * t + nvox + 2 * p * t * t + 2 * p * 48 * 48 + 4 * p) + 256 * 16
-    # plain, T > 64: FP64 Gram + float32 projectors
+    # plain, T > 64: FP64 Gram + rank-p factor blocks (TF = 96)
     nv, pp, tt = 16 * 16 * 16, 6 ** 3, 96
     ws = lib.mppca_workspace_bytes(16, 16, 16, tt, 6, 6, 6, 2, PLAIN)
-    assert 4 * (nv * tt + nv + 3 * pp * tt * tt + 4 * pp) <= ws
-    assert ws < 4 * (nv * tt + nv + 3 * pp * tt * tt + 4 * pp) + 256 * 16
+    assert 4 * (nv * tt + nv + 4 * pp * tt * tt + 4 * pp) <= ws
+    assert ws < 4 * (nv * tt + nv + 4 * pp * tt * tt + 4 * pp) + 256 * 16
     assert lib.mppca_workspace_bytes(x, y, z, t, 11, 11, 11, 3, 7) < 0  # unknown form
     assert lib.mppca_workspace_bytes(x, y, z, 129, 11, 11, 11, 3, 0) < 0  # T > 128
     assert lib.mppca_workspace_bytes(x, y, z, 1, 11, 11, 11, 3, 0) < 0   # T < 2
@@ -72,15 +72,17 @@ def test_workspace_bytes(lib):
 def test_factor_entry_points_validate(lib):
     assert lib.mppca_factor_floats(17) == 2 * 32 * 32
     assert lib.mppca_factor_floats(64) == 2 * 64 * 64
-    assert lib.mppca_factor_floats(65) < 0 and lib.mppca_factor_floats(1) < 0
+    assert lib.mppca_factor_floats(65) == 2 * 80 * 80
+    assert lib.mppca_factor_floats(128) == 2 * 128 * 128
+    assert lib.mppca_factor_floats(129) < 0 and lib.mppca_factor_floats(1) < 0
     assert lib.mppca_eig64_factors(None, 4, 48, 100, None, None, None, None, None, None,
                                    None) < 0
-    assert lib.mppca_eig64_factors(16, 4, 80, 100, 16, 16, 16, None, None, None, None) == -2
+    assert lib.mppca_eig64_factors(16, 4, 130, 100, 16, 16, 16, None, None, None, None) == -2
     assert lib.mppca_eig64_factors(16, 4, 48, 100, 18, 16, 16, None, None, None, None) == -1
     assert b"aligned" in lib.mppca_last_error()
     assert lib.mppca_recon_factors(None, None, None, None, None, None, 16, 16, 16, 48,
                                    5, 5, 5, 2, None) < 0
-    assert lib.mppca_recon_factors(16, 16, 16, 16, 16, 32, 16, 16, 16, 96,
+    assert lib.mppca_recon_factors(16, 16, 16, 16, 16, 32, 16, 16, 16, 130,
                                    5, 5, 5, 2, None) == -2
     assert lib.mppca_recon_factors(16, 16, 16, 16, 16, 16, 16, 16, 16, 48,
                                    5, 5, 5, 2, None) == -1  # out aliases xc

@@ -34,7 +34,7 @@ def test_every_schedule_same_result(pkg, shape, patch, stride):
     want_rank = bd.rank.cpu().numpy().copy()
     for gram in (1, 2, 3):
         for eig in (1, 2):
-            for sx in ((1, 2, 4) if shape[3] <= 64 else (0,)):
+            for sx in (1, 2, 4):
                 bd.tuning = lib.Tuning(tuning.N, tuning.T, gram, eig, sx)
                 try:
                     bd.run(1)

@@ -46,7 +46,7 @@ def _device_result(d, vol):
     ((40, 22, 18, 24), (5, 5, 5), 2, "plain"),
     ((33, 20, 16, 16), (7, 6, 5), 3, "plain"),   # odd X, shifted last corners
     ((24, 12, 12, 12), (3, 3, 3), 5, "plain"),   # stride > patch: uncovered planes
-    ((20, 12, 12, 80), (4, 4, 4), 2, "plain"),   # T > 64: whole-volume route
+    ((20, 12, 12, 80), (4, 4, 4), 2, "plain"),   # T > 64: per-patch FP64 Gram on slabs
     ((24, 16, 16, 16), (5, 5, 5), 2, "fp32"),    # projector route: whole-volume
 ])
 def test_host_pipeline_matches_device_path(pkg, shape, patch, stride, form):

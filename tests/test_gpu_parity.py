@@ -76,12 +76,12 @@ def test_configs(pkg, shape, patch, stride, rank, noise):
 
 
 @pytest.mark.parametrize("shape,patch,stride,rank,noise", [
-    # the projector route (T > 64): per-patch FP64 Gram, FP64 eigensolver in LDS,
-    # atomic scatter of w * Xc P_b, gathered normalisation
+    # T above 64: per-patch FP64 Gram, FP64 eigensolver on the four-wavefront
+    # lane grid, rank-p factors and the fused reconstruction
     ((17, 17, 17, 96), (11, 11, 11), 3, 8, 0.03),     # the T = 96 variant's patch
     ((15, 15, 15, 100), (9, 9, 9), 3, 6, 0.05),       # mixed shape c: 9^3, stride 3
     ((12, 12, 12, 128), (5, 5, 5), 2, 6, 0.04),       # mixed shape f: N = 125 < T - 1
-    ((13, 12, 11, 65), (5, 4, 6), 2, 5, 0.05),        # first T past the factor route
+    ((13, 12, 11, 65), (5, 4, 6), 2, 5, 0.05),        # first T past 64 (80-frame tiles)
     # the other mixed geometries at reduced X, Y, Z (their patch, stride and T)
     ((16, 16, 14, 30), (7, 7, 7), 2, 5, 0.06),        # shape a: 7^3 / 2, T = 30
     ((14, 14, 12, 64), (7, 7, 7), 2, 6, 0.04),        # shape d: 7^3 / 2, T = 64
@@ -109,8 +109,8 @@ def test_default_workload_full_size(pkg):
 
 
 def test_t96_workload_full_size(pkg):
-    """96x96x48x96, 11^3 patches, stride 3 (the T = 96 variant, 12,600 patches,
-    projector route) against step 0."""
+    """96x96x48x96, 11^3 patches, stride 3 (the T = 96 variant, 12,600 patches)
+    against step 0."""
     vol, _, patch, stride = pkg.workload_volume("t96")
     out, info = pkg.denoise(vol, patch, stride, return_info=True)
     _parity(out, info, vol, patch, stride, parallel=True)

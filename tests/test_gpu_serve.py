@@ -14,7 +14,7 @@ def _volume(shape, seed, rank=4):
     return (maps @ curves + 0.05 * rng.standard_normal(shape)).astype(np.float32)
 
 
-# (shape, patch, stride): factor route (T <= 64) and one projector route (T > 64)
+# (shape, patch, stride): T up to 64 and one above (all on the factor route)
 GEOMS = [((32, 24, 20, 16), (5, 5, 5), 2),
          ((28, 28, 16, 30), (7, 7, 7), 2),
          ((24, 20, 20, 48), (5, 5, 5), 3),
@@ -35,10 +35,7 @@ def test_results_same_alone_and_in_mix(pkg):
         assert srv.admitted >= len(GEOMS)
     for k, out in zip(order, outs):
         print("request on geometry", GEOMS[k], flush=True)
-        if GEOMS[k][0][3] <= 64:  # factor route: no atomics, bit for bit
-            np.testing.assert_array_equal(out, alone[k])
-        else:  # projector route: float atomics, same up to summation order
-            np.testing.assert_allclose(out, alone[k], rtol=0, atol=1e-6 * np.abs(alone[k]).max())
+        np.testing.assert_array_equal(out, alone[k])  # factor route: no atomics
 
 
 def test_admission_within_budget(pkg):
@@ -88,10 +85,7 @@ def test_batch_server_same_alone_and_in_mix(pkg):
         assert srv.batched_volumes == len(order)
         print(f"{srv.batches} batches for {len(order)} requests", flush=True)
     for k, out in zip(order, outs):
-        if GEOMS[k][0][3] <= 64:
-            np.testing.assert_array_equal(out, alone[k])
-        else:
-            np.testing.assert_allclose(out, alone[k], rtol=0, atol=1e-6 * np.abs(alone[k]).max())
+        np.testing.assert_array_equal(out, alone[k])
 
 
 def test_batch_server_mixed_shapes_small(pkg):
@@ -115,10 +109,7 @@ def test_batch_server_mixed_shapes_small(pkg):
         futs = [srv.submit(*r) for r in reqs]
         outs = [f.result(timeout=120) for f in futs]
     for (vol, patch, stride), out, want in zip(reqs, outs, alone):
-        if vol.shape[3] <= 64:
-            np.testing.assert_array_equal(out, want)
-        else:
-            np.testing.assert_allclose(out, want, rtol=0, atol=1e-6 * np.abs(want).max())
+        np.testing.assert_array_equal(out, want)
 
 
 def test_batch_server_budget_and_eviction(pkg):
@@ -161,7 +152,4 @@ def test_batch_server_precision_form(pkg):
         futs = [srv.submit(*vols[k]) for k in order]
         outs = [f.result(timeout=120) for f in futs]
     for k, out in zip(order, outs):
-        if GEOMS[k][0][3] <= 64:
-            np.testing.assert_array_equal(out, alone[k])
-        else:
-            np.testing.assert_allclose(out, alone[k], rtol=0, atol=1e-6 * np.abs(alone[k]).max())
+        np.testing.assert_array_equal(out, alone[k])

@@ -242,8 +242,6 @@ def test_recon_and_normalize_vs_fp64(pkg, lib, shape, patch, stride):
     ref_out = mean[..., None] + ref_acc / wsum[..., None]
     np.testing.assert_allclose(out.cpu().numpy(), ref_out, rtol=0,
                                atol=4 * EPS32 * (np.abs(ref_out).max() + scale))
-    if t > 64:
-        return
     # fused factor form on P_b = q q^T: same result, no zeroed accumulator needed
     fused = torch.full_like(acc, float("nan"))
     fac = _dev(_factor_blocks([q[b].astype(np.float32) for b in range(P)], wts, t))
@@ -322,14 +320,13 @@ def test_recon_factors_shapes(pkg, lib, shape, patch, stride):
 
 def test_recon_factors_rejects_large_t(pkg, lib):
     torch = _torch()
-    shape = (6, 6, 6, 96)
+    shape = (6, 6, 6, 130)
     d = torch.zeros(shape, dtype=torch.float32, device="cuda")
     m = torch.zeros(shape[:3], dtype=torch.float32, device="cuda")
-    assert lib.mppca_factor_floats(96) < 0
+    assert lib.mppca_factor_floats(130) < 0
     rc = lib.mppca_recon_factors(d.data_ptr(), d.data_ptr(), d.data_ptr(), d.data_ptr(),
                                  m.data_ptr(), m.data_ptr(), *shape, 3, 3, 3, 2, _stream())
-    assert rc == -2  # MPPCA_EUNSUPPORTED
-    assert b"64" in lib.mppca_last_error()
+    assert rc == -2  # MPPCA_EUNSUPPORTED: T above 128
 
 
 def _slide_bound(xc, patch, stride):
@@ -565,7 +562,8 @@ def test_eig64_structured(pkg, lib, t):
 
 
 @pytest.mark.parametrize("n_vox,t", [(1331, 48), (125, 16), (343, 30), (60, 17), (40, 64),
-                                     (200, 40), (343, 64), (150, 52)])
+                                     (200, 40), (343, 64), (150, 52), (1331, 96), (343, 80),
+                                     (125, 128), (729, 100)])
 def test_eig64_factors_match_projector(pkg, lib, n_vox, t):
     """mppca_eig64_factors: same rank / weight as mppca_eig64_mp, sum of the
     slot outer products equals the projector, Uw = w * Ut^T, empty slots and

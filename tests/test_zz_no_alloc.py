@@ -2,9 +2,10 @@
 scratch, no allocation) against mppca_denoise_device (caller workspace) on the
 same inputs, through the C ABI: two calls in a row, the second on a smaller
 volume, so the second call runs over the first one's leftovers in the shared
-scratch. The factor route (T <= 64) is deterministic, so the outputs, ranks,
-margins and status must be identical bit for bit; the projector route
-(T > 64) sums with float atomics and is held to 1e-6 of the largest value."""
+scratch. The plain form's factor route is deterministic, so the outputs,
+ranks, margins and status must be identical bit for bit; the fp32 form's
+projector route sums with float atomics and is held to 1e-6 of the largest
+value."""
 
 import numpy as np
 import pytest
@@ -46,8 +47,8 @@ def _run(pkg, lib, vol, patch, stride, form, noalloc):
     # factor route: a larger volume, then a smaller one
     ([((32, 32, 24, 16), (5, 5, 5), 2, 0), ((16, 16, 12, 10), (5, 5, 5), 2, 1)], PLAIN, True),
     ([((24, 20, 18, 48), (11, 11, 11), 3, 2), ((14, 13, 12, 17), (3, 5, 4), 2, 3)], PLAIN, True),
-    # projector route (T > 64), then the factor route on a smaller volume
-    ([((12, 12, 12, 128), (5, 5, 5), 2, 4), ((13, 12, 11, 24), (5, 4, 6), 2, 5)], PLAIN, False),
+    # T = 128 (N < T - 1), then T = 24 on a smaller volume
+    ([((12, 12, 12, 128), (5, 5, 5), 2, 4), ((13, 12, 11, 24), (5, 4, 6), 2, 5)], PLAIN, True),
     # the all-FP32 form
     ([((20, 18, 16, 30), (7, 7, 7), 2, 6), ((12, 12, 10, 12), (5, 5, 5), 2, 7)], FP32, False),
 ])
@@ -58,7 +59,7 @@ def test_noalloc_matches_device_path(pkg, lib, cases, form, exact):
         want = _run(pkg, lib, vol, patch, stride, form, noalloc=False)
         got = _run(pkg, lib, vol, patch, stride, form, noalloc=True)
         assert np.isfinite(got[0]).all()
-        if exact or shape[3] <= 64 and form == PLAIN:
+        if exact:
             np.testing.assert_array_equal(got[0], want[0])
         else:
             np.testing.assert_allclose(got[0], want[0], rtol=0,
