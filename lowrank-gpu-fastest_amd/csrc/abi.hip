@@ -129,7 +129,7 @@ Workspace carve_workspace(char* base, const Geom& g, int form) {
     const size_t cap = (P + MPPCA_MIXED_RESOLVE_FRAC - 1) / MPPCA_MIXED_RESOLVE_FRAC;
     const size_t blk = g.T <= kFactorMaxT ? (size_t)factor_floats(g.T) : (size_t)g.T * g.T;
     w.rcap = (int)cap;
-    w.amax = (unsigned*)take((size_t)g.nvol * kAmaxSlots * sizeof(unsigned));
+    w.amax = (unsigned*)take((size_t)g.nvol * g.X * sizeof(unsigned));
     w.rlist = (int*)take(cap * sizeof(int));
     w.rcount = (int*)take(sizeof(int));
     w.rgram = (double*)take(cap * g.T * g.T * sizeof(double));
@@ -469,7 +469,9 @@ int run_device_path(const float* vol, float* out, const Geom& g, int form, const
   // the voxel-wise and matrix-wise stages see the batch's volumes as one
   // array; the others map their grids onto the volumes (Geom::nvol)
   const long long P = g.P * g.nvol;
-  if ((rc = launch_centre(vol, w.xc, w.mean, g.nvox * g.nvol, T, st, w.amax, g.nvox))) return rc;
+  if ((rc = launch_centre(vol, w.xc, w.mean, g.nvox * g.nvol, T, st, w.amax,
+                          (long long)g.Y * g.Z)))
+    return rc;
   if (form == MPPCA_FORM_MIXED) {
     if ((rc = run_mixed_solve(g, w, rk, mg, inf, st))) return rc;
     if (w.fac) return launch_recon_factors(w.xc, w.fac, rk, w.weight, w.mean, out, g, 0, g.X, st);

@@ -249,11 +249,10 @@ int run_device_path(const float* vol, float* out, const Geom& g, int form, const
 
 // ---- launchers (defined per stage file) ----------------------------------------
 int zero_fill(float* p, long long n, hipStream_t st);
-// amax (optional): per-volume max |xc| of finite entries, volumes of nvox1
-// voxels, spread over kAmaxSlots slots per volume (the max of the slots)
-constexpr int kAmaxSlots = 64;
+// amax (optional): per-x-plane max |xc| of finite entries, planes of `plane`
+// voxels (nvox a multiple of it)
 int launch_centre(const float* vol, float* xc, float* mean, long long nvox, int T, hipStream_t st,
-                  unsigned* amax = nullptr, long long nvox1 = 0);
+                  unsigned* amax = nullptr, long long plane = 0);
 int launch_normalize(const float* acc, const float* mean, const float* weight, float* out,
                      const Geom& g, hipStream_t st);
 int launch_gram(const float* xc, float* gram, const Geom& g, long long b0, long long nb,
@@ -277,7 +276,11 @@ int launch_eig_mixed(double* gram, long long nb, int T, int N, float* proj, floa
                      float* weight, float* margin, float* lam_desc, int* info, hipStream_t st,
                      bool fp32_reduction);
 int launch_gram_h3(const float* xc, const unsigned* amax, double* gram, const Geom& g,
-                   hipStream_t st);
+                   long long b0, long long nb, hipStream_t st);
+int mixed_first_pass(const Geom& g, const Workspace& w, long long b0, long long nb, int* rank,
+                     float* margin, int* info, hipStream_t st);
+int mixed_resolve(const Geom& g, const Workspace& w, long long b0, long long nb, int* rank,
+                  float* margin, int* info, hipStream_t st);
 int run_mixed_solve(const Geom& g, const Workspace& w, int* rank, float* margin, int* info,
                     hipStream_t st);
 int launch_recon(const float* xc, const float* proj, const float* weight, float* acc,

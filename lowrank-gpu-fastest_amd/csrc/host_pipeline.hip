@@ -278,7 +278,10 @@ extern "C" int mppca_denoise_host(const float* vol, float* out, int X, int Y, in
                    (long long)workspace_bytes, w.bytes);
   hipStream_t sc = (hipStream_t)s_compute, si = (hipStream_t)s_in, so = (hipStream_t)s_out;
   const size_t plane = (size_t)Y * Z * T;  // floats per x-plane
-  const bool pipelined = form == MPPCA_FORM_PLAIN && w.fac != nullptr;
+  // the plain and precision forms pipeline (rank-p factors); the fp32 form's
+  // atomic projector route runs whole-volume
+  const bool mixed = form == MPPCA_FORM_MIXED;
+  const bool pipelined = (form == MPPCA_FORM_PLAIN || mixed) && w.fac != nullptr;
   const int K = pipelined ? (nslabs < X ? nslabs : X) : 1;
   // Slab boundaries. The last slab is narrower (about a third of the others):
   // everything after the last upload -- its Gram and eigensolves, the
@@ -427,16 +430,27 @@ extern "C" int mppca_denoise_host(const float* vol, float* out, int X, int Y, in
     }
     const size_t off = (size_t)xa[i] * plane;
     const long long vox0 = (long long)xa[i] * Y * Z, nv = (long long)(xa[i + 1] - xa[i]) * Y * Z;
-    if (nv > 0 && (rc = launch_centre(d_vol + off, w.xc + off, w.mean + vox0, nv, T, S))) break;
+    // (the precision form: per-x-plane maxima of the slab's planes, for the
+    // FP16 scale of the patches over them)
+    if (nv > 0 && (rc = launch_centre(d_vol + off, w.xc + off, w.mean + vox0, nv, T, S,
+                                      mixed ? w.amax + xa[i] : nullptr, (long long)Y * Z)))
+      break;
     int ix_new = ix_done;
     while (ix_new < g.ncx && corner_of(ix_new, g.nsx, X, px, stride) + px <= xa[i + 1]) ++ix_new;
+    long long rb0 = 0, rnb = 0;  // this slab's patches
     if (ix_new > ix_done) {
       const long long b0 = ix_done * per_ix, nb = (ix_new - ix_done) * per_ix;
-      if ((rc = launch_gram64(w.xc, (double*)w.gram + b0 * T * T, g, b0, nb, S))) break;
-      if ((rc = launch_eig64_mp((double*)w.gram + b0 * T * T, nb, T, g.N, nullptr,
-                                w.fac + b0 * ff, rk + b0, w.weight + b0, mg + b0, nullptr,
-                                inf + b0, S, g.tune.eig)))
-        break;
+      if (mixed) {
+        if ((rc = mixed_first_pass(g, w, b0, nb, rk, mg, inf, S))) break;
+      } else {
+        if ((rc = launch_gram64(w.xc, (double*)w.gram + b0 * T * T, g, b0, nb, S))) break;
+        if ((rc = launch_eig64_mp((double*)w.gram + b0 * T * T, nb, T, g.N, nullptr,
+                                  w.fac + b0 * ff, rk + b0, w.weight + b0, mg + b0, nullptr,
+                                  inf + b0, S, g.tune.eig)))
+          break;
+      }
+      rb0 = b0;
+      rnb = nb;
       ix_done = ix_new;
     }
     // the reconstruction stream takes over: it reads the centred planes and the
@@ -444,6 +458,10 @@ extern "C" int mppca_denoise_host(const float* vol, float* out, int X, int Y, in
     if ((rc = hip_ok(hipEventRecord(ev_eig.ev[i], S), "event")) ||
         (rc = hip_ok(hipStreamWaitEvent(rs, ev_eig.ev[i], 0), "wait")))
       break;
+    // the precision form's FP64 re-solve of the slab's small-margin patches
+    // runs there too, ahead of the reconstruction that reads their factors,
+    // off the compute stream's path (its launches are few and latency-bound)
+    if (mixed && rnb > 0 && (rc = mixed_resolve(g, w, rb0, rnb, rk, mg, inf, rs))) break;
     // planes below the first unfinished corner (and uploaded) are final
     int x_hi = ix_done == g.ncx ? X : corner_of(ix_done, g.nsx, X, px, stride);
     if (x_hi > xa[i + 1]) x_hi = xa[i + 1];

@@ -10,17 +10,20 @@ namespace mppca {
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 
 // ---------------------------------------------------------------------------
-// Power-of-two scale of each volume: the max over the kAmaxSlots slots of
-// volume v of max |xc| over its finite entries (float bits; order-preserving
-// as unsigned for values >= 0), written by the centring kernel. With m that
-// max, 2^e * m lies in [2^13, 2^14): the
-// hi halves stay below FP16's largest value (65504) and the lo halves of all
-// but the smallest entries stay normal. e is clamped so that 2^(-2e), the
-// unscaling of the Gram, stays a normal float.
+// Power-of-two scale of each patch from the per-x-plane maxima of max |xc|
+// over finite entries that the centring kernel writes (float bits;
+// order-preserving as unsigned for values >= 0): with m the max over the
+// patch's planes cx .. cx + px - 1 (planes of its volume, from amax), 2^e * m
+// lies in [2^13, 2^14): the hi halves stay below FP16's largest value (65504)
+// and the lo halves of all but the smallest entries stay normal. e is clamped
+// so that 2^(-2e), the unscaling of the Gram, stays a normal float.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ int h3_scale_exp(const unsigned* amax, long long vb, int lane) {
-  static_assert(kAmaxSlots <= 64, "one slot per lane");
-  unsigned u = lane < kAmaxSlots ? amax[vb * kAmaxSlots + lane] : 0u;
+__device__ __forceinline__ int h3_scale_exp(const unsigned* amax, int cx, int px, int lane) {
+  unsigned u = 0u;
+  for (int l = lane; l < px; l += 64) {
+    const unsigned x = amax[cx + l];
+    u = x > u ? x : u;
+  }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
     const unsigned x = __shfl_xor(u, o);
@@ -60,10 +63,13 @@ constexpr int kH3Run = 4;   // MFMA steps per uncompensated run
 
 __host__ __device__ constexpr int h3_waves(int ntb) { return ntb <= 4 ? 1 : ntb <= 6 ? 2 : 4; }
 
+// Patches [b0, b0 + nb) of each volume of the batch, into consecutive slots
+// of gram (nb per volume).
 template <int NTB>
 __global__ __launch_bounds__(256) void k_gram_h3(const float* __restrict__ xc,
                                                  const unsigned* __restrict__ amax,
-                                                 double* __restrict__ gram, Geom g, int per_xcd) {
+                                                 double* __restrict__ gram, Geom g, long long b0,
+                                                 long long nb, int per_xcd) {
   constexpr int NW = h3_waves(NTB), PPB = 4 / NW;  // waves per patch, patches per block
   constexpr int NBLK = NTB * (NTB + 1) / 2, NACC = (NBLK + NW - 1) / NW;
   extern __shared__ int s_off[];
@@ -78,12 +84,13 @@ __global__ __launch_bounds__(256) void k_gram_h3(const float* __restrict__ xc,
   // XCD-contiguous runs of patches (neighbours share most voxels, and an L2)
   const long long blk = (long long)(blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
   const long long bl = blk * PPB + wave / NW;
-  if (bl >= (long long)g.P * g.nvol) return;
+  if (bl >= nb * g.nvol) return;
   const int T = g.T, N = g.N;
   const int i = lane & 15, kq = lane >> 4;
-  const long long vb = bl / g.P;
-  const float* xb = xc + vb * g.nvox * T + patch_base(g, bl - vb * g.P) * T;
-  const int ex = h3_scale_exp(amax, vb, lane);
+  const long long vb = bl / nb, pb = b0 + (bl - vb * nb);
+  const float* xb = xc + vb * g.nvox * T + patch_base(g, pb) * T;
+  const int cx = corner_of((int)(pb / ((long long)g.ncy * g.ncz)), g.nsx, g.X, g.px, g.stride);
+  const int ex = h3_scale_exp(amax + vb * g.X, cx, g.px, lane);
   const float scale = __builtin_amdgcn_ldexpf(1.f, ex);
   float tmask[NTB];  // the scale, 0 past T
 #pragma unroll
@@ -177,27 +184,28 @@ __global__ __launch_bounds__(256) void k_gram_h3(const float* __restrict__ xc,
 
 template <int NTB>
 static int launch_h3_t(const float* xc, const unsigned* amax, double* gram, const Geom& g,
-                       hipStream_t st) {
+                       long long b0, long long nb, hipStream_t st) {
   constexpr int PPB = 4 / h3_waves(NTB);
-  const long long blocks = ((long long)g.P * g.nvol + PPB - 1) / PPB;
+  const long long blocks = (nb * g.nvol + PPB - 1) / PPB;
   const long long per_xcd = (blocks + 7) / 8;
   if (per_xcd > (1LL << 28)) return set_err(MPPCA_EUNSUPPORTED, "gram: too many patches");
   hipLaunchKernelGGL((k_gram_h3<NTB>), dim3((unsigned)(8 * per_xcd)), dim3(256),
-                     (g.N + kH3Pad) * sizeof(int), st, xc, amax, gram, g, (int)per_xcd);
+                     (g.N + kH3Pad) * sizeof(int), st, xc, amax, gram, g, b0, nb, (int)per_xcd);
   return launch_status("fp16x3 gram kernel");
 }
 
 int launch_gram_h3(const float* xc, const unsigned* amax, double* gram, const Geom& g,
-                   hipStream_t st) {
+                   long long b0, long long nb, hipStream_t st) {
+  if (nb <= 0) return MPPCA_OK;
   switch (ntb_of(g.T)) {
-    case 1: return launch_h3_t<1>(xc, amax, gram, g, st);
-    case 2: return launch_h3_t<2>(xc, amax, gram, g, st);
-    case 3: return launch_h3_t<3>(xc, amax, gram, g, st);
-    case 4: return launch_h3_t<4>(xc, amax, gram, g, st);
-    case 5: return launch_h3_t<5>(xc, amax, gram, g, st);
-    case 6: return launch_h3_t<6>(xc, amax, gram, g, st);
-    case 7: return launch_h3_t<7>(xc, amax, gram, g, st);
-    default: return launch_h3_t<8>(xc, amax, gram, g, st);
+    case 1: return launch_h3_t<1>(xc, amax, gram, g, b0, nb, st);
+    case 2: return launch_h3_t<2>(xc, amax, gram, g, b0, nb, st);
+    case 3: return launch_h3_t<3>(xc, amax, gram, g, b0, nb, st);
+    case 4: return launch_h3_t<4>(xc, amax, gram, g, b0, nb, st);
+    case 5: return launch_h3_t<5>(xc, amax, gram, g, b0, nb, st);
+    case 6: return launch_h3_t<6>(xc, amax, gram, g, b0, nb, st);
+    case 7: return launch_h3_t<7>(xc, amax, gram, g, b0, nb, st);
+    default: return launch_h3_t<8>(xc, amax, gram, g, b0, nb, st);
   }
 }
 
@@ -305,11 +313,13 @@ static int launch_gram64_list(const float* xc, double* gram, const Geom& g,
 // a patch keeps its precision-form result and gets status -3.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_resolve_select(const float* __restrict__ margin,
-                                                        int* __restrict__ info, long long P,
-                                                        float thr, int* __restrict__ list,
+                                                        int* __restrict__ info, long long b0,
+                                                        long long n, float thr,
+                                                        int* __restrict__ list,
                                                         int* __restrict__ count, int cap) {
-  const long long b = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= P) return;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const long long b = b0 + i;
   if (info[b] != 0 || !(margin[b] < thr)) return;
   const int slot = atomicAdd(count, 1);
   if (slot < cap)
@@ -344,24 +354,37 @@ __global__ __launch_bounds__(256) void k_resolve_scatter(
 }
 
 // ---------------------------------------------------------------------------
-// The precision form from the centred volume to the rank-p factors: FP16x3 Gram, FP32 reduction + FP64 MP rule, selection, exact
-// FP64 Gram and FP64 solve of the selected patches, hand-back.
+// The precision form from the centred volume to the rank-p factors, over
+// patches [b0, b0 + nb) of each volume (the whole batch: b0 = 0, nb = P; an
+// x-slab of one volume in the host pipeline). First pass: FP16x3 Gram, FP32
+// reduction + FP64 MP rule. Re-solve: selection, exact FP64 Gram and FP64
+// solve of the selected patches, hand-back. Both stream-ordered; the re-solve
+// may run on another stream after the first pass.
 // ---------------------------------------------------------------------------
-int run_mixed_solve(const Geom& g, const Workspace& w, int* rank, float* margin, int* info,
-                    hipStream_t st) {
+int mixed_first_pass(const Geom& g, const Workspace& w, long long b0, long long nb, int* rank,
+                     float* margin, int* info, hipStream_t st) {
   int rc;
-  const long long P = (long long)g.P * g.nvol;
   const int T = g.T;
-  if ((rc = launch_gram_h3(w.xc, w.amax, (double*)w.gram, g, st))) return rc;
-  if ((rc = launch_eig_mixed((double*)w.gram, P, T, g.N, w.proj, w.fac, rank, w.weight, margin,
-                             nullptr, info, st, true)))
-    return rc;
+  const long long tt = (long long)T * T;
+  if ((rc = launch_gram_h3(w.xc, w.amax, (double*)w.gram + b0 * tt, g, b0, nb, st))) return rc;
+  return launch_eig_mixed((double*)w.gram + b0 * tt, nb * g.nvol, T, g.N,
+                          w.proj ? w.proj + b0 * tt : nullptr,
+                          w.fac ? w.fac + b0 * factor_floats(T) : nullptr, rank + b0,
+                          w.weight + b0, margin + b0, nullptr, info + b0, st, true);
+}
+
+int mixed_resolve(const Geom& g, const Workspace& w, long long b0, long long nb, int* rank,
+                  float* margin, int* info, hipStream_t st) {
+  int rc;
+  const long long n = nb * g.nvol;
+  const int T = g.T;
   // the list counter starts at 0 (a fill kernel, so that the sequence can be
   // captured in a graph)
   if ((rc = zero_fill((float*)w.rcount, 1, st))) return rc;
   const int cap = w.rcap;
-  hipLaunchKernelGGL(k_resolve_select, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, st,
-                     margin, info, P, (float)MPPCA_MIXED_RESOLVE_MARGIN, w.rlist, w.rcount, cap);
+  hipLaunchKernelGGL(k_resolve_select, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                     margin, info, b0, n, (float)MPPCA_MIXED_RESOLVE_MARGIN, w.rlist, w.rcount,
+                     cap);
   if ((rc = launch_status("re-solve selection"))) return rc;
   if ((rc = launch_gram64_list(w.xc, w.rgram, g, w.rlist, w.rcount, cap, st))) return rc;
   const bool fac = w.fac != nullptr;
@@ -373,6 +396,12 @@ int run_mixed_solve(const Geom& g, const Workspace& w, int* rank, float* margin,
                      cap, w.rblk, w.rrank, w.rweight, w.rmargin, w.rinfo, blk,
                      fac ? w.fac : w.proj, rank, w.weight, margin, info);
   return launch_status("re-solve hand-back");
+}
+
+int run_mixed_solve(const Geom& g, const Workspace& w, int* rank, float* margin, int* info,
+                    hipStream_t st) {
+  const int rc = mixed_first_pass(g, w, 0, g.P, rank, margin, info, st);
+  return rc ? rc : mixed_resolve(g, w, 0, g.P, rank, margin, info, st);
 }
 
 }  // namespace mppca
@@ -395,8 +424,9 @@ int mppca_mixed_eigenvalues(const float* vol, int X, int Y, int Z, int T, int px
                    (long long)workspace_bytes, need.bytes);
   const Workspace w = carve_workspace((char*)workspace, g, MPPCA_FORM_MIXED);
   hipStream_t st = (hipStream_t)stream;
-  if ((rc = launch_centre(vol, w.xc, w.mean, g.nvox, T, st, w.amax, g.nvox))) return rc;
-  if ((rc = launch_gram_h3(w.xc, w.amax, (double*)w.gram, g, st))) return rc;
+  if ((rc = launch_centre(vol, w.xc, w.mean, g.nvox, T, st, w.amax, (long long)g.Y * g.Z)))
+    return rc;
+  if ((rc = launch_gram_h3(w.xc, w.amax, (double*)w.gram, g, 0, g.P, st))) return rc;
   return launch_eig_mixed((double*)w.gram, g.P, T, g.N, w.proj, w.fac, w.rank, w.weight, w.margin,
                           lam_desc, w.info, st, true);
 }

@@ -42,9 +42,10 @@ int zero_fill(float* p, long long n, hipStream_t st) {
 // floats: staged through LDS with coalesced loads, one thread per voxel forms
 // the mean, then xc = x - mean streams out coalesced.
 // amax (optional, the precision form): max |xc| over the finite entries of
-// each volume of nvox1 voxels, as float bits (atomicMax on unsigned keeps the
-// order of values >= 0), spread over kAmaxSlots slots per volume by block
-// index so that the atomics do not queue on one address; zeroed beforehand.
+// each x-plane of `plane` voxels (amax[v / plane] for voxel v), as float bits
+// (atomicMax on unsigned keeps the order of values >= 0); zeroed here first.
+// Per plane, so that a patch's scale (the max over its planes) is the same
+// whether the volume is centred whole or slab by slab.
 // ---------------------------------------------------------------------------
 constexpr int kCentreVox = 64;
 
@@ -59,7 +60,7 @@ __global__ __launch_bounds__(256) void k_centre(const float* __restrict__ vol,
                                                 float* __restrict__ xc,
                                                 float* __restrict__ mean, long long nvox,
                                                 int T, unsigned* __restrict__ amax,
-                                                long long nvox1) {
+                                                long long plane) {
   extern __shared__ float s_x[];  // kCentreVox * T (+ kCentreVox means)
   float* s_m = s_x + kCentreVox * T;
   const long long v0 = (long long)blockIdx.x * kCentreVox;
@@ -125,36 +126,34 @@ __global__ __launch_bounds__(256) void k_centre(const float* __restrict__ vol,
     }
   }
   if (amax) {
-    const long long va = v0 / nvox1, vz = (v0 + nv - 1) / nvox1;
-    if (va == vz) {
+    const long long pa = v0 / plane, pz = (v0 + nv - 1) / plane;
+    if (pa == pz) {
       tmax = wave_max_f(tmax);
-      if ((threadIdx.x & 63) == 0)
-        atomicMax(amax + va * kAmaxSlots + (blockIdx.x & (kAmaxSlots - 1)),
-                  __float_as_uint(tmax));
-    } else {  // the block straddles volumes: per entry (at most nvol - 1 such blocks)
+      if ((threadIdx.x & 63) == 0) atomicMax(amax + pa, __float_as_uint(tmax));
+    } else {  // the block straddles planes: per entry
       for (int e = threadIdx.x; e < ne; e += blockDim.x)
-        atomicMax(amax + (v0 + e / T) / nvox1 * kAmaxSlots,
-                  __float_as_uint(fin_abs(s_x[e] - s_m[e / T])));
+        atomicMax(amax + (v0 + e / T) / plane, __float_as_uint(fin_abs(s_x[e] - s_m[e / T])));
     }
   }
 }
 
 int launch_centre(const float* vol, float* xc, float* mean, long long nvox, int T,
-                  hipStream_t st, unsigned* amax, long long nvox1) {
+                  hipStream_t st, unsigned* amax, long long plane) {
   const unsigned blocks = (unsigned)((nvox + kCentreVox - 1) / kCentreVox);
   const size_t lds = (size_t)kCentreVox * (T + 1) * sizeof(float);
   const bool vec = T % 4 == 0 && (uintptr_t)vol % 16 == 0 && (uintptr_t)xc % 16 == 0;
   if (amax) {
-    if (nvox1 <= 0) return set_err(MPPCA_EINVAL, "centre: volume size %lld", nvox1);
-    const int rc = zero_fill((float*)amax, nvox / nvox1 * kAmaxSlots, st);
+    if (plane <= 0 || nvox % plane != 0)
+      return set_err(MPPCA_EINVAL, "centre: %lld voxels in planes of %lld", nvox, plane);
+    const int rc = zero_fill((float*)amax, nvox / plane, st);
     if (rc) return rc;
   }
   if (vec)
     hipLaunchKernelGGL(k_centre<true>, dim3(blocks), dim3(256), lds, st, vol, xc, mean, nvox, T,
-                       amax, nvox1);
+                       amax, plane);
   else
     hipLaunchKernelGGL(k_centre<false>, dim3(blocks), dim3(256), lds, st, vol, xc, mean, nvox, T,
-                       amax, nvox1);
+                       amax, plane);
   return launch_status("centre kernel");
 }
 
