@@ -94,6 +94,16 @@ def gram_route(shape, patch, form):
     return "patch"
 
 
+def stage_symbol(stage, shape, patch, form, route):
+    """The kernel a single-launch stage runs (for the roofline line)."""
+    if stage == "gram":
+        r = gram_route(shape, patch, form)
+        return "k_gram_h3" if form == "mixed" else "k_gram" if r == "patch" else "k_gram_" + r
+    if stage == "recon":
+        return "k_recon_lr" if route == "factors" else "k_recon"
+    return STAGE_KERNELS[stage][0]
+
+
 def stage_launches(stage, shape, form):
     """Kernel launches of a stage on the device path (the FP64 eigensolver for
     T <= 48 is k_tri_grid + k_eig_vals + k_eig_vecs)."""
@@ -597,7 +607,8 @@ def main():
     peak, unit = PEAK[peak_key]
     sec = kernel_ms[dom] * 1e-3
     achieved = amount / sec / (1e9 if unit == "GB/s" else 1e12)
-    roofline = {"kernel": dom, "symbol": STAGE_KERNELS[dom][0], "bound": bound,
+    roofline = {"kernel": dom, "symbol": stage_symbol(dom, shape, patch, args.form, d.route),
+                "bound": bound,
                 "achieved": round(achieved, 3), "peak": peak, "unit": unit,
                 "frac": round(achieved / peak, 4), "traffic": None, "work_per_launch": amount,
                 "peak_of": peak_key, "longest_stage": max(kernel_ms, key=kernel_ms.get)}
