@@ -129,7 +129,7 @@ Workspace carve_workspace(char* base, const Geom& g, int form) {
     const size_t cap = (P + MPPCA_MIXED_RESOLVE_FRAC - 1) / MPPCA_MIXED_RESOLVE_FRAC;
     const size_t blk = g.T <= kFactorMaxT ? (size_t)factor_floats(g.T) : (size_t)g.T * g.T;
     w.rcap = (int)cap;
-    w.amax = (unsigned*)take((size_t)g.nvol * g.X * sizeof(unsigned));
+    w.amax = (unsigned*)take((size_t)g.nvol * g.X * kAmaxSpread * sizeof(unsigned));
     w.rlist = (int*)take(cap * sizeof(int));
     w.rcount = (int*)take(sizeof(int));
     w.rgram = (double*)take(cap * g.T * g.T * sizeof(double));

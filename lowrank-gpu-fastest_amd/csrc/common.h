@@ -250,7 +250,9 @@ int run_device_path(const float* vol, float* out, const Geom& g, int form, const
 // ---- launchers (defined per stage file) ----------------------------------------
 int zero_fill(float* p, long long n, hipStream_t st);
 // amax (optional): per-x-plane max |xc| of finite entries, planes of `plane`
-// voxels (nvox a multiple of it)
+// voxels (nvox a multiple of it), kAmaxSpread slots per plane (the max of
+// the slots), so that the blocks of one plane do not queue on one address
+constexpr int kAmaxSpread = 8;
 int launch_centre(const float* vol, float* xc, float* mean, long long nvox, int T, hipStream_t st,
                   unsigned* amax = nullptr, long long plane = 0);
 int launch_normalize(const float* acc, const float* mean, const float* weight, float* out,

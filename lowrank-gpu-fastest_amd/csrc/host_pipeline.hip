@@ -433,7 +433,8 @@ extern "C" int mppca_denoise_host(const float* vol, float* out, int X, int Y, in
     // (the precision form: per-x-plane maxima of the slab's planes, for the
     // FP16 scale of the patches over them)
     if (nv > 0 && (rc = launch_centre(d_vol + off, w.xc + off, w.mean + vox0, nv, T, S,
-                                      mixed ? w.amax + xa[i] : nullptr, (long long)Y * Z)))
+                                      mixed ? w.amax + (size_t)xa[i] * kAmaxSpread : nullptr,
+                                      (long long)Y * Z)))
       break;
     int ix_new = ix_done;
     while (ix_new < g.ncx && corner_of(ix_new, g.nsx, X, px, stride) + px <= xa[i + 1]) ++ix_new;

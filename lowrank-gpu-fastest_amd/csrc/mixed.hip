@@ -3,6 +3,8 @@
 // call in that arithmetic, and the hand-back of their exact FP64 re-solve.
 // The reference computes its Gram in FP32 (gram_kernel, step8_tuned.py:32-56);
 // this form keeps FP32 accumulation but feeds the matrix cores FP16 operands.
+#include <type_traits>
+
 #include "common.h"
 
 namespace mppca {
@@ -20,8 +22,8 @@ typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int h3_scale_exp(const unsigned* amax, int cx, int px, int lane) {
   unsigned u = 0u;
-  for (int l = lane; l < px; l += 64) {
-    const unsigned x = amax[cx + l];
+  for (int l = lane; l < px * kAmaxSpread; l += 64) {
+    const unsigned x = amax[cx * kAmaxSpread + l];
     u = x > u ? x : u;
   }
 #pragma unroll
@@ -90,7 +92,7 @@ __global__ __launch_bounds__(256) void k_gram_h3(const float* __restrict__ xc,
   const long long vb = bl / nb, pb = b0 + (bl - vb * nb);
   const float* xb = xc + vb * g.nvox * T + patch_base(g, pb) * T;
   const int cx = corner_of((int)(pb / ((long long)g.ncy * g.ncz)), g.nsx, g.X, g.px, g.stride);
-  const int ex = h3_scale_exp(amax + vb * g.X, cx, g.px, lane);
+  const int ex = h3_scale_exp(amax + vb * g.X * kAmaxSpread, cx, g.px, lane);
   const float scale = __builtin_amdgcn_ldexpf(1.f, ex);
   float tmask[NTB];  // the scale, 0 past T
 #pragma unroll
@@ -194,9 +196,326 @@ static int launch_h3_t(const float* xc, const unsigned* amax, double* gram, cons
   return launch_status("fp16x3 gram kernel");
 }
 
+// ---------------------------------------------------------------------------
+// The same Gram as a sliding window. The patches of one (x-corner, y-corner)
+// column share their x-y footprint and their scale (the scale depends on the
+// x-planes only) and differ in their z window [cz, cz + pz). A workgroup
+// walks a run of consecutive windows (4; 2 above 80 frames) of one column through their
+// planes, accumulating the prefix P(z) = sum over planes z' < z of the
+// footprint plane's FP16x3 product hi lo^T + lo hi^T + hi hi^T, each plane's
+// product in FP32 and the prefix as a compensated pair (hi, lo) of FP32 sums
+// (error-free two-sum per plane). At a window's first plane the pair is kept
+// in registers (one snapshot slot per window of the run);
+// after its last plane G = (P_hi - S_hi) + (P_lo - S_lo). Every plane's
+// product is formed once per run instead of once per patch: at 11^3 / stride 3
+// a run of 4 windows walks 20 planes where its 4 patches hold 44.
+// Runs are a fixed partition of each column's windows, so a patch's Gram does
+// not depend on the launch (whole volume or an x-slab of the host pipeline).
+//
+// NW wavefronts per workgroup, each forming NBW of the NBLK 16 x 16 blocks of
+// the upper block triangle. The plane is staged in LDS as two FP16 tiles (hi,
+// lo), frame-major: row f holds 128 voxel slots of frame f (the footprint,
+// zeros past it), rows kH3sRS = 136 halves apart (16 bytes of padding put
+// the 16 rows one operand read touches on distinct banks). MFMA operand of
+// lane (i, kq) at voxel step c: the 8 halves of row 16 tb + i at voxels
+// 32 c + 8 kq .. + 7, one 16-byte read; the four steps' reads are issued
+// before their MFMAs. Staging stores of lanes with nothing to store go to a
+// slot of their own (no branch).
+// ---------------------------------------------------------------------------
+// windows per run (and snapshot slots): 4, and 2 above 80 frames, where each
+// wavefront holds the prefixes and snapshots of two or three blocks
+__host__ __device__ constexpr int h3s_run(int ntb) { return ntb <= 5 ? 4 : 2; }
+constexpr int kH3sMaxF = 128;  // largest footprint (staging and LDS sizes)
+
+__host__ __device__ constexpr int h3s_blk(int ntb) { return ntb * (ntb + 1) / 2; }
+// wavefronts per workgroup: every block gets a wave (NTB <= 5), else two or
+// three blocks per wave; 64 * waves is a multiple of the 16 * NTB frames, so
+// that each thread stages one fixed frame
+__host__ __device__ constexpr int h3s_waves(int ntb) {
+  return ntb <= 5 ? h3s_blk(ntb) : ntb == 6 ? 12 : ntb == 7 ? 14 : 12;
+}
+__host__ __device__ constexpr int h3s_nbw(int ntb) {
+  return (h3s_blk(ntb) + h3s_waves(ntb) - 1) / h3s_waves(ntb);
+}
+constexpr int kH3sRS = kH3sMaxF + 8;  // tile row stride (halves)
+__host__ __device__ constexpr size_t h3s_lds(int ntb, int R) {
+  // hi and lo tiles, a dummy store slot per thread and tile, row offsets
+  return 2 * (size_t)(16 * ntb) * kH3sRS * 2 + 2 * 4 * 1024 + (size_t)R * 4;
+}
+
+template <int NTB>
+__global__ __launch_bounds__(64 * h3s_waves(NTB)) void k_gram_h3s(
+    const float* __restrict__ xc, const unsigned* __restrict__ amax, double* __restrict__ gram,
+    Geom g, int ix0, int nix, int per_xcd) {
+  constexpr int NBLK = h3s_blk(NTB), NBW = h3s_nbw(NTB), NW = h3s_waves(NTB);
+  constexpr int NT = 64 * NW, TP = 16 * NTB;
+  extern __shared__ __align__(16) unsigned char s_raw[];
+  const int T = g.T, F = g.px * g.py, R = (F + 31) / 32 * 32;
+  constexpr int RS = kH3sRS;
+  _Float16* s_hi = (_Float16*)s_raw;
+  _Float16* s_lo = s_hi + TP * RS;
+  unsigned* s_dummy = (unsigned*)(s_lo + TP * RS);  // 2 x 1024 words
+  unsigned* s_row = s_dummy + 2 * 1024;             // element offset of footprint row n at z = 0
+  constexpr int RUN = h3s_run(NTB);
+  const int nrun = (g.ncz + RUN - 1) / RUN;
+  const long long ncol = (long long)nix * g.ncy;
+  const long long nunit = ncol * g.nvol * nrun;
+  const long long unit = (long long)(blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+  if (unit >= nunit) return;  // the whole workgroup
+  const int run = (int)(unit % nrun);
+  const long long colb = unit / nrun;
+  const long long vb = colb / ncol, col = colb - vb * ncol;
+  const int ix = ix0 + (int)(col / g.ncy), iy = (int)(col % g.ncy);
+  const int cx = corner_of(ix, g.nsx, g.X, g.px, g.stride);
+  const int cy = corner_of(iy, g.nsy, g.Y, g.py, g.stride);
+  const int w0 = run * RUN, w1 = min(w0 + RUN, g.ncz);
+  const int z0 = corner_of(w0, g.nsz, g.Z, g.pz, g.stride);
+  const int z1 = corner_of(w1 - 1, g.nsz, g.Z, g.pz, g.stride) + g.pz;
+  const float* xb = xc + vb * g.nvox * T + (((long long)cx * g.Y + cy) * g.Z) * T;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int i = lane & 15, kq = lane >> 4;
+
+  // zero tiles (frames past T and voxels past F stay zero), row offsets
+  for (int e = tid; e < TP * RS; e += NT) {
+    s_hi[e] = (_Float16)0.f;
+    s_lo[e] = (_Float16)0.f;
+  }
+  for (int n = tid; n < R; n += NT) {
+    const int m = n < F ? n : F - 1;
+    const int dx = m / g.py, dy = m - dx * g.py;
+    s_row[n] = (unsigned)((dx * g.Y + dy) * g.Z) * (unsigned)T;
+  }
+  const int ex = h3_scale_exp(amax + vb * g.X * kAmaxSpread, cx, g.px, lane);
+  const float scale = __builtin_amdgcn_ldexpf(1.f, ex);
+  __syncthreads();
+
+  // staging: thread -> frame ft = tid % TP of the voxel pairs
+  // (2 n2, 2 n2 + 1), n2 = tid / TP + RP k: consecutive threads on consecutive
+  // frames (coalesced loads), one 4-byte LDS store per tile and pair
+  constexpr int RP = NT / TP;  // pairs per pass
+  static_assert(RP * TP == NT, "a thread stages one fixed frame");
+  constexpr int NK = (kH3sMaxF / 2 + RP - 1) / RP;
+  const int ft = tid % TP, r2 = tid / TP;
+  const int nk = (R / 2 - r2 + RP - 1) / RP;  // this thread's pairs (R / 2 of them in all)
+  const bool ft_ok = ft < T;
+  const unsigned fo = (unsigned)(ft_ok ? ft : T - 1);
+  unsigned ro0[NK], ro1[NK];  // element offsets of the pair's rows at z = 0 (clamped)
+  unsigned fmask = 0;         // pair k: bit 2k voxel 2 n2 < F, bit 2k + 1 voxel 2 n2 + 1 < F
+  // the pairs' values of the next two planes (A: even steps, B: odd), loaded
+  // two steps before they are staged
+  float a0[NK], a1[NK], b0[NK], b1[NK];
+#pragma unroll
+  for (int k = 0; k < NK; ++k) {
+    const int n2 = min(r2 + RP * k, R / 2 - 1);
+    ro0[k] = s_row[2 * n2] + fo;
+    ro1[k] = s_row[2 * n2 + 1] + fo;
+    fmask |= (2 * n2 < F ? 1u : 0u) << (2 * k);
+    fmask |= (2 * n2 + 1 < F ? 1u : 0u) << (2 * k + 1);
+  }
+  static_assert(2 * NK <= 32, "mask bits");
+  // the register set of parity PAR (0: A, 1: B), chosen at compile time
+  auto fetch = [&](int z, auto par) __attribute__((always_inline)) {
+    constexpr int PAR = decltype(par)::value;
+    const unsigned zo = (unsigned)z * (unsigned)T;  // every address is in the footprint
+#pragma unroll
+    for (int k = 0; k < NK; ++k) {
+      const float x0 = xb[ro0[k] + zo], x1 = xb[ro1[k] + zo];
+      if constexpr (PAR == 0) {
+        a0[k] = x0;
+        a1[k] = x1;
+      } else {
+        b0[k] = x0;
+        b1[k] = x1;
+      }
+    }
+  };
+  typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+  half2_t* s_hi2 = (half2_t*)(s_hi + ft * RS + 2 * r2);
+  half2_t* s_lo2 = (half2_t*)(s_lo + ft * RS + 2 * r2);
+  half2_t* d_hi2 = (half2_t*)(s_dummy + tid);
+  half2_t* d_lo2 = (half2_t*)(s_dummy + 1024 + tid);
+  auto put = [&](auto par) __attribute__((always_inline)) {
+    constexpr int PAR = decltype(par)::value;
+#pragma unroll
+    for (int k = 0; k < NK; ++k) {
+      const float x0 = PAR == 0 ? a0[k] : b0[k], x1 = PAR == 0 ? a1[k] : b1[k];
+      const float a = ((fmask >> (2 * k)) & 1u) ? x0 * scale : 0.f;
+      const float b = ((fmask >> (2 * k + 1)) & 1u) ? x1 * scale : 0.f;
+      const _Float16 ha = (_Float16)a, hb = (_Float16)b;
+      const _Float16 la = (_Float16)(a - (float)ha), lb = (_Float16)(b - (float)hb);
+      const bool ok = ft_ok && k < nk;
+      *(ok ? s_hi2 + RP * k : d_hi2) = half2_t{ha, hb};
+      *(ok ? s_lo2 + RP * k : d_lo2) = half2_t{la, lb};
+    }
+  };
+
+  // this wave's blocks: k = wave * NBW + j of the triangle (tb1 <= tb2)
+  int tb1[NBW], tb2[NBW];
+  bool live[NBW];
+#pragma unroll
+  for (int j = 0; j < NBW; ++j) {
+    int k = wave * NBW + j;
+    live[j] = k < NBLK;
+    tb1[j] = 0;
+    tb2[j] = 0;
+    for (int t1 = 0; t1 < NTB && k >= 0; ++t1) {
+      if (k < NTB - t1) {
+        tb1[j] = t1;
+        tb2[j] = t1 + k;
+        k = -1;
+      } else {
+        k -= NTB - t1;
+      }
+    }
+  }
+  floatx4 Ph[NBW], Pl[NBW], Sh[NBW][RUN], Sl[NBW][RUN];
+#pragma unroll
+  for (int j = 0; j < NBW; ++j) {
+    Ph[j] = floatx4{0.f, 0.f, 0.f, 0.f};
+    Pl[j] = Ph[j];
+  }
+  const float unscale = __builtin_amdgcn_ldexpf(1.f, -2 * ex);
+  const long long slot0 = vb * ncol * g.ncz + col * g.ncz;  // slot of window 0 of this column
+  int is = w0, ie = w0;
+  auto step = [&](int z, auto par) __attribute__((always_inline)) {
+    while (is < w1 && corner_of(is, g.nsz, g.Z, g.pz, g.stride) == z) {
+      const int sl = is - w0;
+      // selects with constant indices (a branch per slot would index the
+      // slot arrays at run time, in scratch memory)
+#pragma unroll
+      for (int j = 0; j < NBW; ++j)
+#pragma unroll
+        for (int u = 0; u < RUN; ++u)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            Sh[j][u][r] = u == sl ? Ph[j][r] : Sh[j][u][r];
+            Sl[j][u][r] = u == sl ? Pl[j][r] : Sl[j][u][r];
+          }
+      ++is;
+    }
+    const bool act = is > ie;  // plane z lies in a window
+    __syncthreads();           // every wave is done with plane z - 1
+    put(par);
+    __syncthreads();
+    // in flight for two steps; unconditional (the last plane again past the
+    // run's end), since a load issued in a branch makes its consumer wait
+    // for every later load too
+    fetch(min(z + 2, z1 - 1), par);
+    if (act) {
+#pragma unroll
+      for (int j = 0; j < NBW; ++j) {
+        if (!live[j]) continue;
+        const _Float16* aH = s_hi + (tb1[j] * 16 + i) * RS + 8 * kq;
+        const _Float16* aL = s_lo + (tb1[j] * 16 + i) * RS + 8 * kq;
+        const _Float16* bH = s_hi + (tb2[j] * 16 + i) * RS + 8 * kq;
+        const _Float16* bL = s_lo + (tb2[j] * 16 + i) * RS + 8 * kq;
+        constexpr int NSTEP = kH3sMaxF / 32;
+        half8 ah[NSTEP], al[NSTEP], bh[NSTEP], bl[NSTEP];
+#pragma unroll
+        for (int c = 0; c < NSTEP; ++c) {
+          ah[c] = *(const half8*)(aH + 32 * c);
+          al[c] = *(const half8*)(aL + 32 * c);
+          bh[c] = *(const half8*)(bH + 32 * c);
+          bl[c] = *(const half8*)(bL + 32 * c);
+        }
+        floatx4 c4 = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int c = 0; c < NSTEP; ++c) {
+          c4 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[c], bl[c], c4, 0, 0, 0);
+          c4 = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[c], bh[c], c4, 0, 0, 0);
+          c4 = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[c], bh[c], c4, 0, 0, 0);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {  // P += c, exactly up to the error kept in Pl
+          const float x = Ph[j][r], y = c4[r];
+          const float s = x + y, bp = s - x;
+          Pl[j][r] += (x - (s - bp)) + (y - bp);
+          Ph[j][r] = s;
+        }
+      }
+    }
+    if (ie < is && corner_of(ie, g.nsz, g.Z, g.pz, g.stride) + g.pz == z + 1) {
+      const int sl = ie - w0;
+      float* Gb = (float*)(gram + (slot0 + ie) * (long long)T * T);
+#pragma unroll
+      for (int j = 0; j < NBW; ++j) {
+        if (!live[j]) continue;
+        floatx4 sh = Sh[j][0], slo = Sl[j][0];
+#pragma unroll
+        for (int u = 1; u < RUN; ++u)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            sh[r] = u == sl ? Sh[j][u][r] : sh[r];
+            slo[r] = u == sl ? Sl[j][u][r] : slo[r];
+          }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int ti = tb1[j] * 16 + 4 * kq + r, tj = tb2[j] * 16 + i;
+          if (ti < T && tj < T) {
+            const float v = ((Ph[j][r] - sh[r]) + (Pl[j][r] - slo[r])) * unscale;
+            Gb[ti * T + tj] = v;
+            if (tb1[j] != tb2[j]) Gb[tj * T + ti] = v;
+          }
+        }
+      }
+      ++ie;
+    }
+  };
+  const std::integral_constant<int, 0> A{};
+  const std::integral_constant<int, 1> B{};
+  fetch(z0, A);
+  fetch(min(z0 + 1, z1 - 1), B);
+  for (int z = z0; z < z1; z += 2) {
+    step(z, A);
+    if (z + 1 < z1) step(z + 1, B);
+  }
+}
+
+template <int NTB>
+static int launch_h3s_t(const float* xc, const unsigned* amax, double* gram, const Geom& g,
+                        int ix0, int nix, hipStream_t st) {
+  const int F = g.px * g.py, R = (F + 31) / 32 * 32;
+  const size_t lds = h3s_lds(NTB, R);
+  int rc = prepare_lds((const void*)k_gram_h3s<NTB>, lds);
+  if (rc) return rc;
+  const long long units =
+      (long long)nix * g.ncy * g.nvol * ((g.ncz + h3s_run(NTB) - 1) / h3s_run(NTB));
+  const long long per_xcd = (units + 7) / 8;
+  if (per_xcd > (1LL << 28)) return set_err(MPPCA_EUNSUPPORTED, "gram: too many columns");
+  hipLaunchKernelGGL((k_gram_h3s<NTB>), dim3((unsigned)(8 * per_xcd)), dim3(64 * h3s_waves(NTB)),
+                     lds, st, xc, amax, gram, g, ix0, nix, (int)per_xcd);
+  return launch_status("fp16x3 sliding gram kernel");
+}
+
+// The sliding kernel takes launches over whole x-corner columns with a
+// footprint of at most kH3sMaxF voxels; other launches take the per-patch one.
+static int launch_gram_h3_slide(const float* xc, const unsigned* amax, double* gram,
+                                const Geom& g, int ix0, int nix, hipStream_t st) {
+  switch (ntb_of(g.T)) {
+    case 1: return launch_h3s_t<1>(xc, amax, gram, g, ix0, nix, st);
+    case 2: return launch_h3s_t<2>(xc, amax, gram, g, ix0, nix, st);
+    case 3: return launch_h3s_t<3>(xc, amax, gram, g, ix0, nix, st);
+    case 4: return launch_h3s_t<4>(xc, amax, gram, g, ix0, nix, st);
+    case 5: return launch_h3s_t<5>(xc, amax, gram, g, ix0, nix, st);
+    case 6: return launch_h3s_t<6>(xc, amax, gram, g, ix0, nix, st);
+    case 7: return launch_h3s_t<7>(xc, amax, gram, g, ix0, nix, st);
+    default: return launch_h3s_t<8>(xc, amax, gram, g, ix0, nix, st);
+  }
+}
+
 int launch_gram_h3(const float* xc, const unsigned* amax, double* gram, const Geom& g,
                    long long b0, long long nb, hipStream_t st) {
   if (nb <= 0) return MPPCA_OK;
+  const long long per_ix = (long long)g.ncy * g.ncz;
+  // the sliding kernel above 64 frames (T = 96: 1.12 ms against 1.93 per
+  // patch); at 48 frames the per-patch kernel is faster (0.37 against 0.46 ms):
+  // a plane's staging, barriers and LDS operand reads cost more than the
+  // products they save. The choice depends on the geometry only, so a patch's
+  // Gram is the same in every launch.
+  if (ntb_of(g.T) >= 5 && g.px * g.py <= kH3sMaxF && b0 % per_ix == 0 &&
+      (nb % per_ix == 0 || b0 + nb == g.P) && (g.nvol == 1 || (b0 == 0 && nb == g.P)))
+    return launch_gram_h3_slide(xc, amax, gram, g, (int)(b0 / per_ix),
+                                (int)((nb + per_ix - 1) / per_ix), st);
   switch (ntb_of(g.T)) {
     case 1: return launch_h3_t<1>(xc, amax, gram, g, b0, nb, st);
     case 2: return launch_h3_t<2>(xc, amax, gram, g, b0, nb, st);
@@ -242,18 +561,24 @@ __global__ __launch_bounds__(64 * list_waves(NTB)) void k_gram64_list(
   doublex4 acc[NACC];
 #pragma unroll
   for (int k = 0; k < NACC; ++k) acc[k] = doublex4{0.0, 0.0, 0.0, 0.0};
+  // unconditional loads of clamped, in-bounds addresses, one slice ahead (the
+  // masks apply when a slice is consumed), so the next slice's loads are in
+  // flight under the MFMAs
+  float nxt[NTB];
+  auto load = [&](int n0) {
+    const int n = min(n0 + kq, N - 1);
+    const int oz = n % g.pz, r = n / g.pz;
+    const float* row = xb + (long long)((r / g.py * g.Y + r % g.py) * g.Z + oz) * T;
+#pragma unroll
+    for (int tb = 0; tb < NTB; ++tb) nxt[tb] = row[min(tb * 16 + i, T - 1)];
+  };
+  load(4 * wave);
   for (int n0 = 4 * wave; n0 < N; n0 += 4 * NWG) {
-    const int n = n0 + kq;
+    const bool vox = n0 + kq < N;
     double a[NTB];
-    if (n < N) {
-      const int oz = n % g.pz, r = n / g.pz;
-      const float* row = xb + (long long)((r / g.py * g.Y + r % g.py) * g.Z + oz) * T;
 #pragma unroll
-      for (int tb = 0; tb < NTB; ++tb) a[tb] = tb * 16 + i < T ? (double)row[tb * 16 + i] : 0.0;
-    } else {
-#pragma unroll
-      for (int tb = 0; tb < NTB; ++tb) a[tb] = 0.0;
-    }
+    for (int tb = 0; tb < NTB; ++tb) a[tb] = vox && tb * 16 + i < T ? (double)nxt[tb] : 0.0;
+    load(n0 + 4 * NWG);
     int k = 0;
 #pragma unroll
     for (int tb1 = 0; tb1 < NTB; ++tb1)

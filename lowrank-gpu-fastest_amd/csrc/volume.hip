@@ -42,10 +42,11 @@ int zero_fill(float* p, long long n, hipStream_t st) {
 // floats: staged through LDS with coalesced loads, one thread per voxel forms
 // the mean, then xc = x - mean streams out coalesced.
 // amax (optional, the precision form): max |xc| over the finite entries of
-// each x-plane of `plane` voxels (amax[v / plane] for voxel v), as float bits
-// (atomicMax on unsigned keeps the order of values >= 0); zeroed here first.
-// Per plane, so that a patch's scale (the max over its planes) is the same
-// whether the volume is centred whole or slab by slab.
+// each x-plane of `plane` voxels, as float bits (atomicMax on unsigned keeps
+// the order of values >= 0), over kAmaxSpread slots per plane (slot
+// blockIdx % kAmaxSpread of plane v / plane; one atomic per block); zeroed
+// here first. Per plane, so that a patch's scale (the max over its planes) is
+// the same whether the volume is centred whole or slab by slab.
 // ---------------------------------------------------------------------------
 constexpr int kCentreVox = 64;
 
@@ -127,12 +128,20 @@ __global__ __launch_bounds__(256) void k_centre(const float* __restrict__ vol,
   }
   if (amax) {
     const long long pa = v0 / plane, pz = (v0 + nv - 1) / plane;
-    if (pa == pz) {
+    if (pa == pz) {  // uniform over the block
       tmax = wave_max_f(tmax);
-      if ((threadIdx.x & 63) == 0) atomicMax(amax + pa, __float_as_uint(tmax));
+      __syncthreads();  // s_m is read no more: its first 4 words take the wave maxima
+      if ((threadIdx.x & 63) == 0) s_m[threadIdx.x >> 6] = tmax;
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        float m = s_m[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) m = fmaxf(m, s_m[w]);
+        atomicMax(amax + pa * kAmaxSpread + (blockIdx.x % kAmaxSpread), __float_as_uint(m));
+      }
     } else {  // the block straddles planes: per entry
       for (int e = threadIdx.x; e < ne; e += blockDim.x)
-        atomicMax(amax + (v0 + e / T) / plane, __float_as_uint(fin_abs(s_x[e] - s_m[e / T])));
+        atomicMax(amax + (v0 + e / T) / plane * kAmaxSpread,
+                  __float_as_uint(fin_abs(s_x[e] - s_m[e / T])));
     }
   }
 }
@@ -145,7 +154,7 @@ int launch_centre(const float* vol, float* xc, float* mean, long long nvox, int 
   if (amax) {
     if (plane <= 0 || nvox % plane != 0)
       return set_err(MPPCA_EINVAL, "centre: %lld voxels in planes of %lld", nvox, plane);
-    const int rc = zero_fill((float*)amax, nvox / plane, st);
+    const int rc = zero_fill((float*)amax, nvox / plane * kAmaxSpread, st);
     if (rc) return rc;
   }
   if (vec)
