@@ -291,14 +291,22 @@ int mppca_denoise_device(const float* vol, float* out, int X, int Y, int Z,
  *                     uploads and the downloads; distinct non-blocking streams
  *                     overlap, one stream (or NULL) serialises.
  *   rank, margin, info  device arrays of P entries, or NULL.
- * The library creates (and destroys) its events and copy threads per call. */
+ *   h_status          host int, or NULL: receives the lowest per-patch status
+ *                     (info) of the call, 0 when every patch solved, so that
+ *                     a host caller need not read the device array back.
+ * Transfers run on the DMA engines (HSA SDMA copies over several engines)
+ * when the HSA runtime knows the buffers, else as hipMemcpyAsync on s_in and
+ * s_out. The host threads, events and signals of a call are kept in a
+ * per-device context that the next call reuses (calls in parallel each take
+ * their own). */
 int mppca_denoise_host(const float* vol, float* out, int X, int Y, int Z, int T,
                        int px, int py, int pz, int stride, int form,
                        void* workspace, int64_t workspace_bytes,
                        float* d_vol, float* d_out, float* h_stage_in,
                        float* h_stage_out, int nslabs, int nthreads,
                        void* s_compute, void* s_in, void* s_out,
-                       int32_t* rank, float* margin, int32_t* info);
+                       int32_t* rank, float* margin, int32_t* info,
+                       int32_t* h_status);
 
 /* ---------------------------------------------------------------------------
  * Batches of volumes of one geometry

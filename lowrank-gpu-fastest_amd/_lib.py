@@ -12,7 +12,7 @@ import pathlib
 
 PKG = pathlib.Path(__file__).resolve().parent
 LIB_PATH = PKG / "libmppca.so"
-ABI_VERSION = 105
+ABI_VERSION = 106
 FORM_CODES = {"plain": 0, "fp32": 1, "mixed": 2}
 
 _i32, _i64, _vp, _f = ctypes.c_int, ctypes.c_int64, ctypes.c_void_p, ctypes.c_float
@@ -39,7 +39,7 @@ SIGNATURES = {
     "mppca_workspace_bytes": (_i64, [_i32] * 9),
     "mppca_denoise_device": (_i32, [_vp, _vp] + [_i32] * 9 + [_vp, _i64, _vp, _vp, _vp, _vp]),
     "mppca_denoise_host": (_i32, [_vp, _vp] + [_i32] * 9 + [_vp, _i64] + [_vp] * 4
-                           + [_i32, _i32] + [_vp] * 6),
+                           + [_i32, _i32] + [_vp] * 7),
     "mppca_batch_workspace_bytes": (_i64, [_i32] * 10),
     "mppca_denoise_device_batch": (_i32, [_vp, _vp] + [_i32] * 10 + [_vp, _vp, _i64] + [_vp] * 4),
     "mppca_graph_create": (_i32, [_vp, _vp, _vp] + [_i32] * 10 + [_vp, _vp, _i64] + [_vp] * 3),

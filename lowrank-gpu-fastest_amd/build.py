@@ -13,12 +13,14 @@ import pathlib
 import shutil
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 PKG = pathlib.Path(__file__).resolve().parent
 ROOT = PKG.parent
 # HIP sources, plus host-only C++ (compiled by hipcc as plain C++)
 SRCS = sorted((PKG / "csrc").glob("*.hip")) + sorted((PKG / "csrc").glob("*.cpp"))
 OUT = PKG / "libmppca.so"
+OBJ_DIR = PKG / "_build"  # per-source objects (kept out of git and of GPU runs)
 ARCH = "gfx950"
 
 HIPCC_FLAGS = [
@@ -26,7 +28,6 @@ HIPCC_FLAGS = [
     "-O3",
     "-std=c++17",
     "-fPIC",
-    "-shared",
     # packed FP32 VALU forms are kept out of the generated code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-munsafe-fp-atomics",
@@ -45,13 +46,7 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found (ROCm toolchain required to build)")
 
 
-def build(verbose: bool = False, extra_flags=()) -> pathlib.Path:
-    tmp = OUT.with_suffix(".so.tmp")
-    rocm = pathlib.Path(hipcc()).resolve().parent.parent
-    # the host pipeline's DMA downloads call the HSA runtime (linked by SONAME,
-    # libhsa-runtime64.so.1, so the process keeps the one torch's HIP runtime loaded)
-    cmd = [hipcc(), *HIPCC_FLAGS, *extra_flags, f"-I{ROOT / 'include'}",
-           *[str(s) for s in SRCS], f"-L{rocm / 'lib'}", "-lhsa-runtime64", "-o", str(tmp)]
+def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
@@ -62,6 +57,23 @@ def build(verbose: bool = False, extra_flags=()) -> pathlib.Path:
         lines = [ln for ln in res.stderr.splitlines() if "packed-fp32-ops" not in ln]
         if any(ln.strip() for ln in lines):
             print("\n".join(lines), file=sys.stderr)
+
+
+def build(verbose: bool = False, extra_flags=(), jobs: int | None = None) -> pathlib.Path:
+    """Compile every source to an object (in parallel), then link the library."""
+    tmp = OUT.with_suffix(".so.tmp")
+    rocm = pathlib.Path(hipcc()).resolve().parent.parent
+    OBJ_DIR.mkdir(exist_ok=True)
+    objs = [OBJ_DIR / (s.name + ".o") for s in SRCS]
+    jobs = jobs or max(1, min(len(SRCS), 8, os.cpu_count() or 1))
+    with ThreadPoolExecutor(jobs) as ex:
+        list(ex.map(lambda so: _run([hipcc(), *HIPCC_FLAGS, *extra_flags, f"-I{ROOT / 'include'}",
+                                     "-c", str(so[0]), "-o", str(so[1])], verbose),
+                    zip(SRCS, objs)))
+    # the host pipeline's DMA copies call the HSA runtime (linked by SONAME,
+    # libhsa-runtime64.so.1, so the process keeps the one torch's HIP runtime loaded)
+    _run([hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", *[str(o) for o in objs],
+          f"-L{rocm / 'lib'}", "-lhsa-runtime64", "-o", str(tmp)], verbose)
     os.replace(tmp, OUT)
     return OUT
 

@@ -15,7 +15,7 @@
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef double doublex4 __attribute__((ext_vector_type(4)));
 
-#define MPPCA_ABI_VERSION 105
+#define MPPCA_ABI_VERSION 106
 #define MAX_T 128
 #define MAX_N 4096
 

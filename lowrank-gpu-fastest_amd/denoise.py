@@ -16,6 +16,7 @@ All device work is enqueued on torch's current stream through the C ABI of
 
 from __future__ import annotations
 
+import ctypes
 import os
 import threading
 import warnings
@@ -256,15 +257,18 @@ class Denoiser:
         cur = torch.cuda.current_stream(self.device)
         sc.wait_stream(cur)  # earlier work on the buffers
         so.wait_stream(cur)  # (the output stream computes too when downloads use DMA)
+        worst = ctypes.c_int32(0)  # the lowest per-patch status, read by the call
         rc = self.lib.mppca_denoise_host(
             a.ctypes.data, host.data_ptr(), *self._geom_args(), self.form_code,
             self._ws_ptr, self._ws_bytes, self.vol_d.data_ptr(), self.out_d.data_ptr(),
             self._pin_in.data_ptr(), None, self.nslabs, self.nthreads,
             sc.cuda_stream, si.cuda_stream, so.cuda_stream,
             self.rank.data_ptr(), self.margin.data_ptr(), self.sweeps.data_ptr(),
+            ctypes.byref(worst),
         )
         _lib.check(rc, "mppca_denoise_host")
-        self.check_status(on_bad)
+        if worst.value < 0:  # some patch failed: the per-patch report
+            self.check_status(on_bad)
         return host.numpy()
 
     def info(self) -> dict:

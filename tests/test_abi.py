@@ -92,17 +92,17 @@ def test_denoise_host_validates_before_any_call(lib):
     args = [16, 16, 16, 8, 5, 5, 5, 2]
     # null host / device pointers, bad slab and thread counts, unknown form
     assert lib.mppca_denoise_host(None, 1, *args, 0, 256, 1 << 30, 1, 2, None, None, 4, 2,
-                                  None, None, None, None, None, None) == -1
+                                  None, None, None, None, None, None, None) == -1
     assert lib.mppca_denoise_host(1, 1, *args, 0, 256, 1 << 30, 1, 1, None, None, 4, 2,
-                                  None, None, None, None, None, None) == -1  # d_vol == d_out
+                                  None, None, None, None, None, None, None) == -1  # d_vol == d_out
     assert lib.mppca_denoise_host(1, 1, *args, 0, 256, 1 << 30, 1, 2, None, None, 0, 2,
-                                  None, None, None, None, None, None) == -1
+                                  None, None, None, None, None, None, None) == -1
     assert lib.mppca_denoise_host(1, 1, *args, 0, 256, 1 << 30, 1, 2, None, None, 4, 0,
-                                  None, None, None, None, None, None) == -1
+                                  None, None, None, None, None, None, None) == -1
     assert lib.mppca_denoise_host(1, 1, *args, 9, 256, 1 << 30, 1, 2, None, None, 4, 2,
-                                  None, None, None, None, None, None) == -1
+                                  None, None, None, None, None, None, None) == -1
     assert lib.mppca_denoise_host(1, 1, *args, 0, 256, 16, 1, 2, None, None, 4, 2,
-                                  None, None, None, None, None, None) == -3  # workspace
+                                  None, None, None, None, None, None, None) == -3  # workspace
 
 
 def test_host_side_validation_launches_nothing(lib):

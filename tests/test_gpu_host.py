@@ -83,7 +83,7 @@ def test_host_pipeline_pageable_output_through_staging(pkg, lib):
     rc = lib.mppca_denoise_host(
         vol.ctypes.data, out.ctypes.data, *shape, *patch, stride, 0, d._ws_ptr, d._ws_bytes,
         d.vol_d.data_ptr(), d.out_d.data_ptr(), stage_in.data_ptr(), stage_out.data_ptr(),
-        4, 3, *(s.cuda_stream for s in streams), None, None, None)
+        4, 3, *(s.cuda_stream for s in streams), None, None, None, None)
     pkg._lib.check(rc, "mppca_denoise_host")
     _assert_same(out, want, "factors")
 
@@ -100,6 +100,22 @@ def test_host_pipeline_one_stream(pkg, lib):
     rc = lib.mppca_denoise_host(
         vol.ctypes.data, out.data_ptr(), *shape, *patch, stride, 0, d._ws_ptr, d._ws_bytes,
         d.vol_d.data_ptr(), d.out_d.data_ptr(), pin_in.data_ptr(), None, 5, 2,
-        None, None, None, None, None, None)
+        None, None, None, None, None, None, None)
     pkg._lib.check(rc, "mppca_denoise_host")
     _assert_same(out.numpy(), want, "factors")
+
+
+def test_host_pipeline_alternating_volumes(pkg):
+    """Calls that alternate between volumes, reusing the kept threads, events
+    and signals: each result is its own volume's (no data of the previous
+    call survives in the device buffers or in the caches the uploads bypass);
+    the status the call reports agrees with the per-patch array."""
+    shape, patch, stride = (40, 22, 18, 24), (5, 5, 5), 2
+    vols = [_volume(shape, s) for s in (21, 22, 23)]
+    d = pkg.Denoiser(shape, patch, stride)
+    wants = [_device_result(d, v)[0] for v in vols]
+    for k in range(9):
+        print("alternating call", k, flush=True)
+        got = d(vols[k % 3])
+        _assert_same(got, wants[k % 3], "factors")
+        assert int(d.info()["sweeps"].min()) >= 0
