@@ -502,14 +502,27 @@ extern "C" int mppca_denoise_host(const float* vol, float* out, int X, int Y, in
         err.store((int)e, std::memory_order_release);
         return;
       }
+      // a download of more than 1 MiB is split over the download engines
+      const int parts = jb.bytes > ((size_t)1 << 20) && C.eng_dn.size() > 1 ? (int)C.eng_dn.size() : 1;
+      hsa_signal_store_relaxed(C.sig_dn[j], parts);
       if (jb.bytes == 0) {
         hsa_signal_store_screlease(C.sig_dn[j], 0);
       } else {
-        const uint32_t eg = C.eng_dn.empty() ? 0 : C.eng_dn[j % C.eng_dn.size()];
-        if (!dma_copy(jb.dst, cpu_agent, jb.src, gpu_agent, jb.bytes, C.sig_dn[j], eg)) {
-          dn_msg = "download: hsa_amd_memory_async_copy refused the copy";
-          err.store(MPPCA_EINVAL, std::memory_order_release);
-          return;
+        const size_t per = (jb.bytes / parts + 4095) & ~(size_t)4095;
+        for (int k = 0; k < parts; ++k) {
+          const size_t a = std::min(jb.bytes, per * k), b = k + 1 < parts ? std::min(jb.bytes, a + per) : jb.bytes;
+          const uint32_t eg = C.eng_dn.empty() ? 0 : C.eng_dn[(j + k) % C.eng_dn.size()];
+          if (b <= a) {  // (an empty part: counted done)
+            hsa_signal_subtract_screlease(C.sig_dn[j], 1);
+            continue;
+          }
+          if (!dma_copy(jb.dst + a, cpu_agent, jb.src + a, gpu_agent, b - a, C.sig_dn[j], eg)) {
+            dn_msg = "download: hsa_amd_memory_async_copy refused the copy";
+            hsa_signal_subtract_screlease(C.sig_dn[j], parts - k);
+            err.store(MPPCA_EINVAL, std::memory_order_release);
+            issued.store(j + 1, std::memory_order_release);
+            return;
+          }
         }
       }
       issued.store(j + 1, std::memory_order_release);
@@ -523,7 +536,7 @@ extern "C" int mppca_denoise_host(const float* vol, float* out, int X, int Y, in
     posted.store(j + 1, std::memory_order_release);
   };
   int ix_done = 0, x_done = 0;
-  int deferred_hi = -1;  // planes whose reconstruction waits for the next slab's Gram
+  int deferred_hi = -1;  // planes whose reconstruction waits for the next set's Gram
   const long long per_ix = (long long)g.ncy * g.ncz;
   const long long ff = pipelined ? factor_floats(T) : 0;
   const int tw = recon_factor_tile_x(g);
@@ -559,7 +572,14 @@ extern "C" int mppca_denoise_host(const float* vol, float* out, int X, int Y, in
     }
     return r;
   };
-  for (int i = 0; i < K; ++i) {
+  // Slab i counts as uploaded (without waiting) once its copies have landed.
+  auto uploaded = [&](int i) {
+    if (up_slabs.load(std::memory_order_acquire) <= i) return false;
+    if (dma_up) return hsa_signal_load_scacquire(C.sig_up[i]) == 0;
+    return hipEventQuery(C.ev_in[i]) == hipSuccess;
+  };
+  hipEvent_t prev_gram = nullptr;  // the last Gram launch of the plain form
+  for (int i = 0, set = 0; i < K; ++set) {
     while (up_slabs.load(std::memory_order_acquire) <= i && !err.load(std::memory_order_acquire))
       relax();
     if (err.load(std::memory_order_acquire)) break;  // reported after the join
@@ -571,34 +591,53 @@ extern "C" int mppca_denoise_host(const float* vol, float* out, int X, int Y, in
         rc = set_err(MPPCA_EINVAL, "upload: a DMA copy reported an error");
         break;
       }
-    } else if ((rc = hip_ok(hipStreamWaitEvent(S, C.ev_in[i], 0), "wait"))) {
-      break;
     }
+    // The plain form takes the slabs in sets: before enqueuing the next set the
+    // host waits until the previous set's Gram has run (its eigensolves, ~170
+    // us, still ahead on the GPU), then takes every slab uploaded by then. When
+    // the kernels fall behind the uploads, sets grow and their launches fill
+    // the GPU better; when they keep up, a set is one slab.
+    int j = i;  // the set's last slab
+    if (pipelined && !mixed) {
+      if (prev_gram) {
+        hipError_t q;
+        while ((q = hipEventQuery(prev_gram)) == hipErrorNotReady) relax();
+        if ((rc = hip_ok(q, "Gram event"))) break;
+      }
+      // (the last slab stays a set of its own: what follows it is the tail)
+      while (j + 2 < K && uploaded(j + 1)) ++j;
+    }
+    if (!dma_up && (rc = hip_ok(hipStreamWaitEvent(S, C.ev_in[j], 0), "wait"))) break;
+    const bool last = j + 1 == K;
+    const int xlo = xa[i], xhi = xa[j + 1];
+    i = j + 1;
     if (!pipelined) {  // whole volume: one slab, the device path, one download
       if ((rc = run_device_path(d_vol, d_out, g, form, w, rank, margin, info, S))) break;
       if ((rc = finish_planes(X, S, 1))) break;
       continue;
     }
-    const size_t off = (size_t)xa[i] * plane;
-    const long long vox0 = (long long)xa[i] * Y * Z, nv = (long long)(xa[i + 1] - xa[i]) * Y * Z;
+    const size_t off = (size_t)xlo * plane;
+    const long long vox0 = (long long)xlo * Y * Z, nv = (long long)(xhi - xlo) * Y * Z;
     // (the precision form: per-x-plane maxima of the slab's planes, for the
     // FP16 scale of the patches over them)
     if (nv > 0 && (rc = launch_centre(d_vol + off, w.xc + off, w.mean + vox0, nv, T, S,
-                                      mixed ? w.amax + (size_t)xa[i] * kAmaxSpread : nullptr,
+                                      mixed ? w.amax + (size_t)xlo * kAmaxSpread : nullptr,
                                       (long long)Y * Z)))
       break;
     int ix_new = ix_done;
-    while (ix_new < g.ncx && corner_of(ix_new, g.nsx, X, px, stride) + px <= xa[i + 1]) ++ix_new;
-    long long rb0 = 0, rnb = 0;  // this slab's patches
+    while (ix_new < g.ncx && corner_of(ix_new, g.nsx, X, px, stride) + px <= xhi) ++ix_new;
+    long long rb0 = 0, rnb = 0;  // this set's patches
     if (ix_new > ix_done) {
       const long long b0 = ix_done * per_ix, nb = (ix_new - ix_done) * per_ix;
       if (mixed) {
         if ((rc = mixed_first_pass(g, w, b0, nb, rk, mg, inf, S))) break;
       } else {
-        if ((rc = launch_gram64(w.xc, (double*)w.gram + b0 * T * T, g, b0, nb, S))) break;
-        if (deferred_hi >= 0) {  // the previous slab's reconstruction, beside these eigensolves
-          if ((rc = hip_ok(hipEventRecord(C.ev_gram[i], S), "event")) ||
-              (rc = hip_ok(hipStreamWaitEvent(rs, C.ev_gram[i], 0), "wait")) ||
+        if ((rc = launch_gram64(w.xc, (double*)w.gram + b0 * T * T, g, b0, nb, S)) ||
+            (rc = hip_ok(hipEventRecord(C.ev_gram[set], S), "event")))
+          break;
+        prev_gram = C.ev_gram[set];
+        if (deferred_hi >= 0) {  // the previous set's reconstruction, beside these eigensolves
+          if ((rc = hip_ok(hipStreamWaitEvent(rs, C.ev_gram[set], 0), "wait")) ||
               (rc = finish_planes(deferred_hi, rs, 1)))
             break;
           deferred_hi = -1;
@@ -613,24 +652,24 @@ extern "C" int mppca_denoise_host(const float* vol, float* out, int X, int Y, in
       ix_done = ix_new;
     }
     // the reconstruction stream takes over: it reads the centred planes and the
-    // factors of every slab up to this one, all written on the compute stream
-    if ((rc = hip_ok(hipEventRecord(C.ev_eig[i], S), "event")) ||
-        (rc = hip_ok(hipStreamWaitEvent(rs, C.ev_eig[i], 0), "wait")))
+    // factors of every set up to this one, all written on the compute stream
+    if ((rc = hip_ok(hipEventRecord(C.ev_eig[set], S), "event")) ||
+        (rc = hip_ok(hipStreamWaitEvent(rs, C.ev_eig[set], 0), "wait")))
       break;
-    // the precision form's FP64 re-solve of the slab's small-margin patches
+    // the precision form's FP64 re-solve of the set's small-margin patches
     // runs there too, ahead of the reconstruction that reads their factors,
     // off the compute stream's path (its launches are few and latency-bound)
     if (mixed && rnb > 0 && (rc = mixed_resolve(g, w, rb0, rnb, rk, mg, inf, rs))) break;
     // planes below the first unfinished corner (and uploaded) are final
     int x_hi = ix_done == g.ncx ? X : corner_of(ix_done, g.nsx, X, px, stride);
-    if (x_hi > xa[i + 1]) x_hi = xa[i + 1];
+    if (x_hi > xhi) x_hi = xhi;
     if (x_hi != X) x_hi -= x_hi % tw;
     const int hi = x_hi > x_done ? x_hi : x_done;
-    if (!mixed && i + 1 < K) {  // (plain form: reconstructed beside the next slab's eigensolves)
+    if (!mixed && !last) {  // (plain form: reconstructed beside the next set's eigensolves)
       deferred_hi = hi;
       continue;
     }
-    if ((rc = finish_planes(hi, rs, i + 1 == K ? kTailPieces : 1))) break;
+    if ((rc = finish_planes(hi, rs, last ? kTailPieces : 1))) break;
   }
   // the per-patch status, final once the last reconstruction (behind every
   // eigensolve and re-solve) has run
