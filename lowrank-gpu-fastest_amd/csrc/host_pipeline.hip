@@ -351,13 +351,19 @@ extern "C" int mppca_denoise_host(const float* vol, float* out, int X, int Y, in
     int x0, x1, slab;
   };
   std::vector<Piece> pieces;
-  const size_t piece_bytes = (size_t)4 << 20;  // about 4 MiB per upload
+  // About 4 MiB per upload; the first pieces are a quarter and a half of
+  // that, so that the first upload starts sooner (the staging of the first
+  // 4 MiB took ~90 us, against ~25 us per piece once the copy threads run).
+  const size_t piece_bytes = (size_t)4 << 20;
   const int piece_planes =
       (int)(plane * sizeof(float) >= piece_bytes ? 1 : piece_bytes / (plane * sizeof(float)));
   std::vector<int> slab_pieces(K, 0);
   for (int i = 0; i < K; ++i)
-    for (int x = xa[i]; x < xa[i + 1]; x += piece_planes) {
-      pieces.push_back({x, x + piece_planes < xa[i + 1] ? x + piece_planes : xa[i + 1], i});
+    for (int x = xa[i]; x < xa[i + 1];) {
+      const int n = std::max(1, pieces.size() == 0 ? piece_planes / 4
+                                : pieces.size() == 1 ? piece_planes / 2 : piece_planes);
+      pieces.push_back({x, x + n < xa[i + 1] ? x + n : xa[i + 1], i});
+      x = pieces.back().x1;
       ++slab_pieces[i];
     }
   const int npieces = (int)pieces.size();
@@ -378,7 +384,7 @@ extern "C" int mppca_denoise_host(const float* vol, float* out, int X, int Y, in
   PipeCtx& C = *lease.c;
   if ((rc = C.events(C.ev_in, K, hipEventDisableTiming)) ||
       (rc = C.events(C.ev_gram, K, hipEventDisableTiming)) ||
-      (rc = C.events(C.ev_eig, K, hipEventDisableTiming)) ||
+      (rc = C.events(C.ev_eig, K + 1, hipEventDisableTiming)) ||
       (rc = C.events(C.ev_rec, NJ + 1, hipEventDisableTiming | hipEventReleaseToSystem)) ||
       (rc = C.events(C.ev_out, NJ + 1, hipEventDisableTiming)))
     return rc;
@@ -541,8 +547,9 @@ extern "C" int mppca_denoise_host(const float* vol, float* out, int X, int Y, in
   const long long ff = pipelined ? factor_floats(T) : 0;
   const int tw = recon_factor_tile_x(g);
   // Reconstruction of planes [x_done, x_hi) on stream st, in npiece pieces,
-  // each downloaded once it has run.
-  auto finish_planes = [&](int x_hi, hipStream_t st, int npiece) -> int {
+  // each downloaded once it has run; the pieces alternate between st and st2
+  // (disjoint planes: the tail's small launches run two at a time).
+  auto finish_planes = [&](int x_hi, hipStream_t st, int npiece, hipStream_t st2 = nullptr) -> int {
     int r = MPPCA_OK;
     const int x0 = x_done, span = x_hi - x0;
     for (int k = 0; k < npiece; ++k) {
@@ -552,13 +559,14 @@ extern "C" int mppca_denoise_host(const float* vol, float* out, int X, int Y, in
       const int j = njobs++;
       out_lo[j] = x_done;
       out_hi[j] = b > x_done ? b : x_done;
+      hipStream_t sk = st2 && (k & 1) ? st2 : st;
       if (b > x_done) {
         if (pipelined && (r = launch_recon_factors(w.xc, w.fac, rk, w.weight, w.mean, d_out, g,
-                                                   x_done, b, st)))
+                                                   x_done, b, sk)))
           return r;
         x_done = b;
       }
-      if ((r = hip_ok(hipEventRecord(C.ev_rec[j], st), "event"))) return r;
+      if ((r = hip_ok(hipEventRecord(C.ev_rec[j], sk), "event"))) return r;
       const size_t o = (size_t)out_lo[j] * plane, m = (size_t)(out_hi[j] - out_lo[j]) * plane;
       if (dma_dn) {
         post(j, {(char*)(dst_out + o), (const char*)(d_out + o), m * sizeof(float), C.ev_rec[j]});
@@ -669,10 +677,16 @@ extern "C" int mppca_denoise_host(const float* vol, float* out, int X, int Y, in
       deferred_hi = hi;
       continue;
     }
-    if ((rc = finish_planes(hi, rs, last ? kTailPieces : 1))) break;
+    // (the tail's pieces alternate with the compute stream, idle by then,
+    // once it has waited for the reconstruction stream's re-solves)
+    const bool two = last && rs != sc;
+    if (two && ((rc = hip_ok(hipEventRecord(C.ev_eig[K], rs), "event")) ||
+                (rc = hip_ok(hipStreamWaitEvent(sc, C.ev_eig[K], 0), "wait"))))
+      break;
+    if ((rc = finish_planes(hi, rs, last ? kTailPieces : 1, two ? sc : nullptr))) break;
   }
   // the per-patch status, final once the last reconstruction (behind every
-  // eigensolve and re-solve) has run
+  // eigensolve and re-solve, on either stream) has run
   const int jstat = njobs;  // the next download job
   const bool want_status = !rc && !err.load() && h_status && njobs > 0;
   if (want_status) {

@@ -147,7 +147,7 @@ __device__ __forceinline__ void touch_range(int a0, int ext, int S, int p, int s
 // 128 VGPRs at T <= 48 give 4 waves per SIMD.
 // ---------------------------------------------------------------------------
 template <int NTB, int SX>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NTB >= 4 ? 1 : 4)))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NTB >= 4 ? 1 : (SX >= 4 ? 2 : 4))))
 void k_recon_lr(const float* __restrict__ xc,
                                                   const float* __restrict__ fac,
                                                   const int* __restrict__ rank,
