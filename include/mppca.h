@@ -42,7 +42,7 @@ extern "C" {
 
 /* Arithmetic forms of the whole path (mppca_workspace_bytes,
  * mppca_denoise_device). Both read and write float32 volumes.
- *   PLAIN: FP64 Gram (int8 MFMA on 46-bit fixed-point limbs of each plane
+ *   PLAIN: FP64 Gram (int8 MFMA on 30-bit fixed-point limbs of each plane
  *          for the sliding window over x-y columns, else FP64 MFMA on exact
  *          products of the float32 centred data),
  *          FP64 eigensolver (Householder + bisection + twisted factorisation),
@@ -122,7 +122,7 @@ int mppca_gram(const float* xc, float* gram, int X, int Y, int Z, int T,
 /* As mppca_gram with FP64 accumulation; gram[b] is T x T doubles. A launch
  * over whole x-corner columns (b0 and nb multiples of ncy * ncz) with
  * T <= 64 slides a window along each x-y column: each plane's values are
- * rounded to integers at 2^-46 of the plane's largest |x| and their products
+ * rounded to integers at 2^-30 of the plane's largest |x| and their products
  * summed exactly in int8-limb MFMA (v_mfma_i32_16x16x64_i8) before the FP64
  * prefix. Other launches sum exact FP64 products (v_mfma_f64_16x16x4f64).
  * Replaces gram_kernel (step8_tuned.py:32-56) in the PLAIN form. */

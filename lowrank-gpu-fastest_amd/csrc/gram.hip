@@ -338,24 +338,26 @@ __global__ __launch_bounds__(64 * NTB * (NTB + 1) / 2 * slide_cols(NTB)) __attri
 // value read back after its last) with every plane's products formed exactly
 // in integers.
 //  - A plane (footprint voxels x frames of one column) is scaled by 2^s,
-//    s = 46 - e where every |x| < 2^e, and rounded to integers X, |X| < 2^46:
-//    at most 2^(e-47) <= 2^-46 * max|x| off per value.
-//  - X has six signed 8-bit limbs, X = sum_a L_a 2^(8a), L_a in [-128, 127]:
-//    the bytes of X + 0x808080808080, less 128 each. They are the low six
-//    bytes of fma(x, 2^s, 1.5 * 2^52 + 0x808080808080), whose mantissa holds
-//    that sum once rounded to an integer.
-//  - sum_v X_i X_j = sum_{a,b} 2^(8(a+b)) sum_v L_a,i L_b,j. The 26 limb
-//    pairs with a + b >= 4 run on v_mfma_i32_16x16x64_i8, exact in one
-//    32-bit accumulator per weight w = a + b (at most 6 pairs x 128 voxels x
-//    2^14 < 2^24 per plane). The 10 pairs below add up to less than 2^40.1
-//    against products up to 2^92 and are left out: at most 2^-51.9 * 2^(2e)
+//    s = 30 - e where every |x| < 2^e, and rounded to integers X, |X| < 2^30:
+//    at most 2^(e-31) <= 2^-30 * max|x| off per value, 2^6 below the float32
+//    rounding of the values themselves (2^-24).
+//  - X has four signed 8-bit limbs, X = sum_a L_a 2^(8a), L_a in [-128, 127]:
+//    the bytes of X + 0x80808080, less 128 each. They are the low four bytes
+//    of fma(x, 2^s, 1.5 * 2^52 + 0x80808080), whose mantissa holds that sum
+//    once rounded to an integer.
+//  - sum_v X_i X_j = sum_{a,b} 2^(8(a+b)) sum_v L_a,i L_b,j. The 13 limb
+//    pairs with a + b >= 2 run on v_mfma_i32_16x16x64_i8, exact in one
+//    32-bit accumulator per weight w = a + b (at most 4 pairs x 128 voxels x
+//    2^14 = 2^23 per plane). The 3 pairs below add up to less than 2^23.1
+//    against products up to 2^60 and are left out: at most 2^-36.9 * 2^(2e)
 //    per voxel.
-//  - The seven weight sums are combined in FP64 (Horner in 2^8) and added
-//    to the prefix scaled by 2^(32 - 2s). The planes of a window enter its
+//  - The five weight sums are combined in FP64 (Horner in 2^8) and added
+//    to the prefix scaled by 2^(16 - 2s). The planes of a window enter its
 //    entries once: a Gram entry is off by its planes' quantisation (at most
-//    2^-46 max|x| (|x_i| + |x_j|) + 2^-49 max|x|^2 per voxel) plus the FP64
-//    rounding of the prefix difference.
-// A block's plane costs 2 x 26 int8 MFMAs of 16 cycles where FP64 takes 32 of
+//    2^-30 max|x| (|x_i| + |x_j|) + 2^-34 max|x|^2 per voxel) plus the FP64
+//    rounding of the prefix difference: ~2^-30 of the entries, against the
+//    2^-24 an FP32 Gram carries.
+// A block's plane costs 2 x 13 int8 MFMAs of 16 cycles where FP64 takes 32 of
 // 64. A plane holding a non-finite value makes the prefix NaN, which flags
 // every later window of the run as k_gram_slide's NaN products do.
 // Plane tiles in LDS: one per limb, a row of R = F rounded up to 64 bytes per
@@ -365,9 +367,12 @@ __global__ __launch_bounds__(64 * NTB * (NTB + 1) / 2 * slide_cols(NTB)) __attri
 // conflicts).
 // ---------------------------------------------------------------------------
 typedef int intx4 __attribute__((ext_vector_type(4)));
-constexpr int kLimbs = 6;                          // signed 8-bit limbs per value
-constexpr int kLimbW0 = 4;                         // lightest limb pair weight formed
-constexpr int kLimbNW = 2 * kLimbs - 1 - kLimbW0;  // weights 4 .. 10
+constexpr int kLimbs = 4;                          // signed 8-bit limbs per value
+constexpr int kLimbBits = 8 * kLimbs - 2;          // |X| < 2^kLimbBits
+constexpr int kLimbW0 = 2;                         // lightest limb pair weight formed
+constexpr int kLimbNW = 2 * kLimbs - 1 - kLimbW0;  // weights 2 .. 6
+// limb pairs formed per 64-voxel half of a tile row
+constexpr int kLimbPairs = kLimbs * kLimbs - kLimbW0 * (kLimbW0 + 1) / 2;
 constexpr int kLimbMaxLoads = 16;                  // plane floats each thread stages (one chunk)
 __host__ __device__ constexpr int limb_rows(int F) { return (F + 63) / 64 * 64; }
 // per column: two sets of kLimbs tiles, the footprint offsets (padded to the
@@ -514,14 +519,14 @@ __global__ __launch_bounds__(64 * NTB * (NTB + 1) / 2 * limb_cols(NTB)) void k_g
             max((unsigned)__builtin_amdgcn_readlane((int)m, 32), (unsigned)__builtin_amdgcn_readlane((int)m, 48)));
     if (lane == 0) s_wmax[(z & 1) * NBLK + (tid >> 6)] = m;
   };
-  constexpr double kMagic = 0x1.8p52 + (double)0x808080808080LL;
+  constexpr double kMagic = 0x1.8p52 + (double)0x80808080LL;
 
   // Software pipeline over the run's planes, one barrier per plane. Step z
   // issues the MFMAs of plane z - 1 (tile set (z - 1) & 1), converts plane z
   // into tile set z & 1 while they run, loads plane z + 2 into the registers
   // plane z leaves, and reduces the maximum of plane z + 1, loaded a step ago.
   int is = w0, ie = w0;  // next window to start, next window to end
-  double sc2 = 0.0;      // 2^(32 - 2s) of plane z - 1 (NaN: non-finite plane)
+  double sc2 = 0.0;      // 2^(8 kLimbW0 - 2s) of plane z - 1 (NaN: non-finite plane)
   auto step = [&](int z, float(&cur)[kLimbMaxLoads], float(&nxt)[kLimbMaxLoads])
       __attribute__((always_inline)) {
     // tile set (z - 1) & 1 complete, set z & 1 free, maxima of plane z written
@@ -546,33 +551,26 @@ __global__ __launch_bounds__(64 * NTB * (NTB + 1) / 2 * limb_cols(NTB)) void k_g
     for (int w = 1; w < NBLK; ++w) M = max(M, s_wmax[(z & 1) * NBLK + w]);
     const bool bad = M >= 0x7f800000u;
     const int e = bad ? 0 : __builtin_amdgcn_frexp_expf(__uint_as_float(M));  // max|x| < 2^e
-    const double scl = st_ok ? __builtin_amdgcn_ldexp(1.0, 46 - e) : 0.0;      // frames past T: 0
-    const double sc2_next = bad ? __builtin_nan("") : __builtin_amdgcn_ldexp(1.0, 32 - 2 * (46 - e));
+    const double scl = st_ok ? __builtin_amdgcn_ldexp(1.0, kLimbBits - e) : 0.0;  // frames past T: 0
+    const double sc2_next =
+        bad ? __builtin_nan("") : __builtin_amdgcn_ldexp(1.0, 8 * kLimbW0 - 2 * (kLimbBits - e));
     const unsigned char* tA = opA + (zm & 1) * kLimbs * TB;
     const unsigned char* tB = opB + (zm & 1) * kLimbs * TB;
     unsigned L[kLimbs][4];  // limb a of voxels 16 sc + 4 d .. + 3 in word d
     auto convert = [&](int d) __attribute__((always_inline)) {
-      unsigned lo[4], hi[4];
+      unsigned lo[4];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const unsigned long long bits =
-            (unsigned long long)__double_as_longlong(fma((double)cur[4 * d + j], scl, kMagic));
-        lo[j] = (unsigned)bits;
-        hi[j] = (unsigned)(bits >> 32);
-      }
+      for (int j = 0; j < 4; ++j)
+        lo[j] = (unsigned)__double_as_longlong(fma((double)cur[4 * d + j], scl, kMagic));
       // byte transpose: limb a of the four voxels into one word
       const unsigned t01l = __builtin_amdgcn_perm(lo[1], lo[0], 0x05010400u);
       const unsigned t01h = __builtin_amdgcn_perm(lo[1], lo[0], 0x07030602u);
       const unsigned t23l = __builtin_amdgcn_perm(lo[3], lo[2], 0x05010400u);
       const unsigned t23h = __builtin_amdgcn_perm(lo[3], lo[2], 0x07030602u);
-      const unsigned u01 = __builtin_amdgcn_perm(hi[1], hi[0], 0x05010400u);
-      const unsigned u23 = __builtin_amdgcn_perm(hi[3], hi[2], 0x05010400u);
       L[0][d] = __builtin_amdgcn_perm(t23l, t01l, 0x05040100u) ^ 0x80808080u;
       L[1][d] = __builtin_amdgcn_perm(t23l, t01l, 0x07060302u) ^ 0x80808080u;
       L[2][d] = __builtin_amdgcn_perm(t23h, t01h, 0x05040100u) ^ 0x80808080u;
       L[3][d] = __builtin_amdgcn_perm(t23h, t01h, 0x07060302u) ^ 0x80808080u;
-      L[4][d] = __builtin_amdgcn_perm(u23, u01, 0x05040100u) ^ 0x80808080u;
-      L[5][d] = __builtin_amdgcn_perm(u23, u01, 0x07060302u) ^ 0x80808080u;
     };
     intx4 acc[kLimbNW];
     // limb pairs (a, b), a + b >= kLimbW0, weight w = a + b - kLimbW0; the
@@ -580,7 +578,7 @@ __global__ __launch_bounds__(64 * NTB * (NTB + 1) / 2 * limb_cols(NTB)) void k_g
     // conversion of plane z runs in four pieces between groups of MFMAs
     // (in order, by scheduling barriers), its VALU work under their 16-cycle
     // matrix-pipe slots.
-    constexpr int NPAIR = 26 * RH;
+    constexpr int NPAIR = kLimbPairs * RH;
     int pi = 0, piece = 0;
 #pragma unroll
     for (int hf = 0; hf < RH; ++hf) {

@@ -348,10 +348,10 @@ def _slide_bound(xc, patch, stride):
 
 def _limb_bound(xc, patch, stride):
     """Quantisation bound of the int8-limb sliding Gram: each plane of a
-    column's footprint is rounded to integers at 2^-46 of its largest |x|
-    (M), and limb pairs worth at most 2^-49 M^2 per voxel are left out, so
+    column's footprint is rounded to integers at 2^-30 of its largest |x|
+    (M), and limb pairs worth at most 2^-34.9 M^2 per voxel are left out, so
     entry (t, s) of a window is off by at most
-    sum_z 2^-46 M_z (S_z[t] + S_z[s]) + F 2^-48 M_z^2, S_z = sum of |x| over
+    sum_z 2^-30 M_z (S_z[t] + S_z[s]) + F 2^-34 M_z^2, S_z = sum of |x| over
     the plane's voxels (the square of the rounding error included)."""
     import oracle as o
 
@@ -362,8 +362,8 @@ def _limb_bound(xc, patch, stride):
         F = A[i:i + px, j:j + py, k:k + pz]          # px, py, pz, T
         M = F.max(axis=(0, 1, 3))                    # per plane
         S = F.sum(axis=(0, 1))                       # pz, T
-        b = 2.0 ** -46 * (M[:, None, None] * (S[:, :, None] + S[:, None, :])).sum(0)
-        out.append(b + px * py * 2.0 ** -48 * (M ** 2).sum())
+        b = 2.0 ** -30 * (M[:, None, None] * (S[:, :, None] + S[:, None, :])).sum(0)
+        out.append(b + px * py * 2.0 ** -34 * (M ** 2).sum())
     return np.stack(out)
 
 
@@ -371,7 +371,7 @@ def test_gram64_exact(pkg, lib):
     """FP64 Gram. T > 64 (per-patch FP64 MFMA sums: products of float32 values
     are exact, sums carry 53 bits) is held to the patch-sum bound; T <= 64
     (sliding window over each x-y column, on int8 limbs of each plane rounded
-    at 2^-46 of its largest value, or on FP64 MFMA) to the prefix-difference
+    at 2^-30 of its largest value, or on FP64 MFMA) to the prefix-difference
     bound plus the quantisation bound; a launch over part of the patches (not
     whole columns) takes the per-patch kernel."""
     torch = _torch()
