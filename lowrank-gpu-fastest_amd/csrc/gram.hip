@@ -484,15 +484,29 @@ __global__ __launch_bounds__(64 * NTB * (NTB + 1) / 2 * limb_cols(NTB)) void k_g
     for (int r = 0; r < 4; ++r)
       if (ent_ok && tb1 * 16 + 4 * kq + r < T) *entry(iz, r) = -P[r];
   };
+  // -P of the window that ends at a step's plane, loaded at the step's start
+  // (after that step's start store, which it may read back when pz = 1) so
+  // that the load's latency hides under the step's MFMAs instead of holding
+  // the wave, and with it the next barrier, at the window's end. The load is
+  // unconditional, of an in-bounds entry (window and indices clamped), so that
+  // no branch merges its wait with the plane loads' (those stay in flight).
+  double snap[4];
+  auto load_snap = [&](int iz) {
+    const int izc = iz < w1 ? iz : w1 - 1, tc = t2 < T ? t2 : T - 1;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int ti = tb1 * 16 + 4 * kq + r;
+      snap[r] = Gc[(long long)izc * T * T + (long long)(ti < T ? ti : T - 1) * T + tc];
+    }
+  };
   auto finish = [&](int iz) {
-    // the stores of -P precede these loads in this wave's program order
+    // the stores of -P precede the loads of snap in this wave's program order
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int ti = tb1 * 16 + 4 * kq + r;
       if (ent_ok && ti < T) {
-        double* e = entry(iz, r);
-        const double v = P[r] + *e;
-        *e = v;
+        const double v = P[r] + snap[r];
+        *entry(iz, r) = v;
         if (!diag) Gc[(long long)iz * T * T + (long long)t2 * T + ti] = v;
       }
     }
@@ -538,6 +552,8 @@ __global__ __launch_bounds__(64 * NTB * (NTB + 1) / 2 * limb_cols(NTB)) void k_g
       act = is > ie;  // plane zm lies in a window
       ends = ie < is && corner_of(ie, g.nsz, g.Z, g.pz, g.stride) + g.pz == zm + 1;
     }
+    load_snap(ie);
+    __builtin_amdgcn_sched_barrier(0);  // (issued here, ahead of the plane loads)
     if (!vfull) {  // voxels past the footprint convert to zero
 #pragma unroll
       for (int k = 0; k < kLimbMaxLoads; ++k) cur[k] = ((vmask >> k) & 1u) ? cur[k] : 0.f;
@@ -617,7 +633,10 @@ __global__ __launch_bounds__(64 * NTB * (NTB + 1) / 2 * limb_cols(NTB)) void k_g
       for (int a = 0; a < kLimbs; ++a)
         *(intx4*)(dst + a * TB) = intx4{(int)L[a][0], (int)L[a][1], (int)L[a][2], (int)L[a][3]};
     }
-    if (z + 2 < z1) fetch(cur, z + 2);  // in flight for two steps
+    // in flight for two steps; unconditional (the last planes reload plane
+    // z1 - 1, unused), so that the wait for the snapshot loads above, which
+    // precede these in issue order, leaves them in flight
+    fetch(cur, z + 2 < z1 ? z + 2 : z1 - 1);
     if (act) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
