@@ -444,12 +444,19 @@ __global__ __launch_bounds__(64 * NTB * (NTB + 1) / 2 * limb_cols(NTB)) void k_g
   // tile rows, which converts nothing, reads chunk 0's)
   const intx4* rob = (const intx4*)(s_rowb + 16 * (sc_ok ? sc : 0));
   auto fetch = [&](float(&stage)[kLimbMaxLoads], int z) {
-    const char* cbz = cb + (size_t)z * 4u * (unsigned)T;
+    // the plane's base in scalar registers: each load is the base plus a
+    // 32-bit lane offset (no 64-bit address arithmetic per load)
+    typedef const __attribute__((address_space(1))) char gchar;
+    const unsigned long long zb = (unsigned long long)(cb + (size_t)z * 4u * (unsigned)T);
+    gchar* cbz = (gchar*)((unsigned long long)(unsigned)__builtin_amdgcn_readfirstlane(
+                              (int)(zb >> 32)) << 32 |
+                          (unsigned)__builtin_amdgcn_readfirstlane((int)zb));
 #pragma unroll
     for (int k4 = 0; k4 < kLimbMaxLoads / 4; ++k4) {
       const intx4 o = rob[k4];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) stage[4 * k4 + j] = *(const float*)(cbz + ((unsigned)o[j] + sto));
+      for (int j = 0; j < 4; ++j)
+        stage[4 * k4 + j] = *(const __attribute__((address_space(1))) float*)(cbz + ((unsigned)o[j] + sto));
     }
   };
 
