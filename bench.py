@@ -50,7 +50,7 @@ METRIC = ("ms/volume host->host (96x96x48x48, 11³, stride 3) + p50/p99 latency 
           "at fixed load")
 PEAK = {"hbm": (8000.0, "GB/s"), "fp32": (157.3, "TFLOP/s"), "fp64": (78.6, "TFLOP/s"),
         "int8": (5000.0, "TOP/s")}
-LIMB_PAIRS = 26  # int8 limb pairs the sliding Gram forms per product (gram.hip)
+LIMB_PAIRS = 13  # int8 limb pairs the sliding Gram forms per product (gram.hip, kLimbPairs)
 # kernel-symbol prefix of each stage, as rocprofv3 reports it
 # kernel-symbol prefixes of each stage, as rocprofv3 reports them (the FP64
 # eigensolver for T <= 48 is three kernels: tridiagonalisation, eigenvalues +
