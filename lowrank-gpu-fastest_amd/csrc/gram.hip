@@ -666,6 +666,297 @@ __global__ __launch_bounds__(64 * NTB * (NTB + 1) / 2 * limb_cols(NTB)) void k_g
   }
 }
 
+// ---------------------------------------------------------------------------
+// The int8-limb sliding Gram for 65..128 frames (NTB = 5..8): the walk, the
+// four-limb fixed point and the error bound of k_gram_limb, with the
+// NTB (NTB + 1) / 2 blocks of the upper triangle shared out over NW
+// wavefronts (block k to wavefront k mod NW, up to NBW each) and one x-y
+// column per workgroup: NTB = 6, 11 x 11 footprint: 21 blocks over 12
+// wavefronts, tiles of 2 x 4 limbs x 96 frames x 128 bytes (98 KB of LDS).
+// Each thread stages CPT 16-voxel chunks of one frame (the tile's 4 RH chunks
+// per frame over the NW / NTB * 4 threads of that frame). A wavefront forms its blocks
+// one after the other: the 13 RH MFMAs of a block, then its weights' FP64
+// combination into the block's prefix, so the int32 accumulators serve every
+// block in turn; the conversion of the next plane runs between the blocks.
+// ---------------------------------------------------------------------------
+// wavefronts per workgroup: two per frame block where three fit a SIMD
+// (12 at NTB = 6 with two 64-voxel halves, 10 at NTB = 5), else one
+__host__ __device__ constexpr int limbw_waves(int ntb, int rh) { return rh == 2 && ntb <= 6 ? 2 * ntb : ntb; }
+__host__ __device__ constexpr int limbw_nbw(int ntb, int rh) {
+  return (ntb * (ntb + 1) / 2 + limbw_waves(ntb, rh) - 1) / limbw_waves(ntb, rh);
+}
+__host__ __device__ constexpr size_t limbw_lds(int ntb, int rh) {
+  // two sets of kLimbs tiles, the footprint row offsets, two sets of
+  // per-wavefront plane maxima
+  return (size_t)2 * kLimbs * 16 * ntb * 64 * rh + 4 * (size_t)(64 * rh + 2 * limbw_waves(ntb, rh));
+}
+
+template <int NTB, int RH>
+__global__ __launch_bounds__(64 * limbw_waves(NTB, RH)) void k_gram_limbw(const float* __restrict__ xc,
+                                                        double* __restrict__ gram, Geom g, int ix0,
+                                                        int ncol, int nseg, int per_xcd) {
+  constexpr int NW = limbw_waves(NTB, RH), NT = 64 * NW, TS = 16 * NTB, R = 64 * RH;
+  constexpr int NBLK = NTB * (NTB + 1) / 2, NBW = limbw_nbw(NTB, RH);
+  constexpr int TPF = NT / TS;        // threads per frame (4 or 8)
+  constexpr int CPT = 4 * RH / TPF;   // 16-voxel chunks each thread stages
+  constexpr int NV = 16 * CPT;        // staged values per thread
+  constexpr int TB = TS * R;          // bytes per limb tile
+  constexpr int cmask = R / 16 - 1;
+  static_assert(TPF * TS == NT && CPT * TPF == 4 * RH, "the tile's chunks over the threads");
+  extern __shared__ __align__(16) unsigned char s_lb[];
+  const int T = g.T, F = g.px * g.py;
+  const int unit = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
+  const int colb = unit / nseg, seg = unit - colb * nseg;
+  if (colb >= ncol * g.nvol) return;  // the whole workgroup
+  const int w0 = (int)((long long)seg * g.ncz / nseg), w1 = (int)((long long)(seg + 1) * g.ncz / nseg);
+  const int z0 = corner_of(w0, g.nsz, g.Z, g.pz, g.stride);
+  const int z1 = corner_of(w1 - 1, g.nsz, g.Z, g.pz, g.stride) + g.pz;
+  const int vb = colb / ncol, col = colb - vb * ncol;  // volume of the batch, column in it
+  xc += (long long)vb * g.nvox * g.T;
+  gram += (long long)vb * g.P * g.T * g.T;
+  unsigned char* s_t = s_lb;                               // 2 x kLimbs tiles
+  unsigned* s_rowb = (unsigned*)(s_t + 2 * kLimbs * TB);   // byte offset of footprint row n
+  unsigned* s_wmax = s_rowb + R;                           // 2 x per-wavefront plane maxima
+  const int tid = threadIdx.x;
+  const int ix = ix0 + col / g.ncy, iy = col % g.ncy;
+  const int cx = corner_of(ix, g.nsx, g.X, g.px, g.stride);
+  const int cy = corner_of(iy, g.nsy, g.Y, g.py, g.stride);
+  for (int n = tid; n < R; n += NT) {  // rows past the footprint repeat its last
+    const int nn = n < F ? n : F - 1, dx = nn / g.py, dy = nn - dx * g.py;
+    s_rowb[n] = (unsigned)((dx * g.Y + dy) * g.Z) * (unsigned)T * 4u;
+  }
+
+  // staging role: thread -> frame st, chunks sc0 + TPF c (c < CPT) of 16
+  // voxels (value 16 c + k: voxel 16 (sc0 + TPF c) + k)
+  const int st = tid % TS, sc0 = tid / TS;
+  const bool st_ok = st < T;
+  const unsigned long long cbo = (unsigned long long)(((long long)cx * g.Y + cy) * g.Z) * T * 4u;
+  const char* cb = (const char*)xc +
+                   ((unsigned long long)(unsigned)__builtin_amdgcn_readfirstlane((int)(cbo >> 32)) << 32 |
+                    (unsigned)__builtin_amdgcn_readfirstlane((int)cbo));
+  const unsigned sto = 4u * (unsigned)(st_ok ? st : T - 1);
+  float sa[NV], sb[NV];  // plane z in sa when z - z0 is even, else in sb
+  unsigned vmask = 0;    // bit v: value v is a footprint voxel
+  auto fetch = [&](float(&stage)[NV], int z) {
+    typedef const __attribute__((address_space(1))) char gchar;
+    const unsigned long long zb = (unsigned long long)(cb + (size_t)z * 4u * (unsigned)T);
+    gchar* cbz = (gchar*)((unsigned long long)(unsigned)__builtin_amdgcn_readfirstlane(
+                              (int)(zb >> 32)) << 32 |
+                          (unsigned)__builtin_amdgcn_readfirstlane((int)zb));
+#pragma unroll
+    for (int c = 0; c < CPT; ++c) {
+      const intx4* rob = (const intx4*)(s_rowb + 16 * (sc0 + TPF * c));
+#pragma unroll
+      for (int k4 = 0; k4 < 4; ++k4) {
+        const intx4 o = rob[k4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          stage[16 * c + 4 * k4 + j] =
+              *(const __attribute__((address_space(1))) float*)(cbz + ((unsigned)o[j] + sto));
+      }
+    }
+  };
+
+  // compute role: blocks w + NW j of the upper triangle (tb1 <= tb2)
+  const int lane = tid & 63, i = lane & 15, kq = lane >> 4;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  int tb1[NBW], tb2[NBW];
+  bool live[NBW];
+#pragma unroll
+  for (int j = 0; j < NBW; ++j) {
+    int k = wave + NW * j;
+    live[j] = k < NBLK;
+    tb1[j] = 0;
+    tb2[j] = 0;
+    for (int t1 = 0; t1 < NTB && k >= 0; ++t1) {
+      if (k < NTB - t1) {
+        tb1[j] = t1;
+        tb2[j] = t1 + k;
+        k = -1;
+      } else {
+        k -= NTB - t1;
+      }
+    }
+  }
+  double* Gc = gram + (long long)col * g.ncz * T * T;
+  double P[NBW][4];
+#pragma unroll
+  for (int j = 0; j < NBW; ++j)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) P[j][r] = 0.0;
+  // entry (row tb1*16 + 4 kq + r, column tb2*16 + i) of window iz, clamped in
+  // bounds for the snapshot loads
+  auto eoff = [&](int j, int iz, int r, bool clamp) -> long long {
+    int ti = tb1[j] * 16 + 4 * kq + r, tj = tb2[j] * 16 + i;
+    if (clamp) {
+      iz = iz < w1 ? iz : w1 - 1;
+      ti = ti < T ? ti : T - 1;
+      tj = tj < T ? tj : T - 1;
+    }
+    return (long long)iz * T * T + (long long)ti * T + tj;
+  };
+  // At a window's first plane -P goes into its entry and the mirror entry;
+  // after its last plane an FP64 atomic add (no value returned) adds P to
+  // both: the same single rounding of P(end) - P(start) on both sides, with
+  // no load to wait for and no snapshot registers.
+  auto start = [&](int iz) {
+#pragma unroll
+    for (int j = 0; j < NBW; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int ti = tb1[j] * 16 + 4 * kq + r, tj = tb2[j] * 16 + i;
+        if (live[j] && ti < T && tj < T) {
+          Gc[eoff(j, iz, r, false)] = -P[j][r];
+          if (tb1[j] != tb2[j]) Gc[(long long)iz * T * T + (long long)tj * T + ti] = -P[j][r];
+        }
+      }
+  };
+  auto finish = [&](int iz) {
+#pragma unroll
+    for (int j = 0; j < NBW; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int ti = tb1[j] * 16 + 4 * kq + r, tj = tb2[j] * 16 + i;
+        if (live[j] && ti < T && tj < T) {
+          unsafeAtomicAdd(Gc + eoff(j, iz, r, false), P[j][r]);
+          if (tb1[j] != tb2[j])
+            unsafeAtomicAdd(Gc + (long long)iz * T * T + (long long)tj * T + ti, P[j][r]);
+        }
+      }
+  };
+  __syncthreads();  // offset table written
+#pragma unroll
+  for (int c = 0; c < CPT; ++c)
+#pragma unroll
+    for (int k = 0; k < 16; ++k) vmask |= (16 * (sc0 + TPF * c) + k < F) ? 1u << (16 * c + k) : 0u;
+  const bool vfull = vmask == (NV == 32 ? 0xffffffffu : (1u << NV) - 1u);
+
+  auto plane_max = [&](const float(&stage)[NV], int z) {
+    unsigned m = 0;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) m = max(m, __float_as_uint(stage[k]) & 0x7fffffffu);
+    m = max(m, (unsigned)__builtin_amdgcn_update_dpp((int)m, (int)m, 0xb1, 0xf, 0xf, false));
+    m = max(m, (unsigned)__builtin_amdgcn_update_dpp((int)m, (int)m, 0x4e, 0xf, 0xf, false));
+    m = max(m, (unsigned)__builtin_amdgcn_update_dpp((int)m, (int)m, 0x141, 0xf, 0xf, false));
+    m = max(m, (unsigned)__builtin_amdgcn_update_dpp((int)m, (int)m, 0x140, 0xf, 0xf, false));
+    m = max(max((unsigned)__builtin_amdgcn_readlane((int)m, 0), (unsigned)__builtin_amdgcn_readlane((int)m, 16)),
+            max((unsigned)__builtin_amdgcn_readlane((int)m, 32), (unsigned)__builtin_amdgcn_readlane((int)m, 48)));
+    if (lane == 0) s_wmax[(z & 1) * NW + wave] = m;
+  };
+  constexpr double kMagic = 0x1.8p52 + (double)0x80808080LL;
+
+  int is = w0, ie = w0;  // next window to start, next window to end
+  double sc2 = 0.0;      // 2^(8 kLimbW0 - 2s) of plane z - 1 (NaN: non-finite plane)
+  auto step = [&](int z, float(&cur)[NV], float(&nxt)[NV]) __attribute__((always_inline)) {
+    __syncthreads();
+    const int zm = z - 1;
+    bool act = false, ends = false;
+    if (z > z0) {
+      if (is < w1 && corner_of(is, g.nsz, g.Z, g.pz, g.stride) == zm) start(is++);
+      act = is > ie;
+      ends = ie < is && corner_of(ie, g.nsz, g.Z, g.pz, g.stride) + g.pz == zm + 1;
+    }
+    if (!vfull) {
+#pragma unroll
+      for (int k = 0; k < NV; ++k) cur[k] = ((vmask >> k) & 1u) ? cur[k] : 0.f;
+    }
+    unsigned M = s_wmax[(z & 1) * NW];
+#pragma unroll
+    for (int w = 1; w < NW; ++w) M = max(M, s_wmax[(z & 1) * NW + w]);
+    const bool bad = M >= 0x7f800000u;
+    const int e = bad ? 0 : __builtin_amdgcn_frexp_expf(__uint_as_float(M));
+    const double scl = st_ok ? __builtin_amdgcn_ldexp(1.0, kLimbBits - e) : 0.0;
+    const double sc2_next =
+        bad ? __builtin_nan("") : __builtin_amdgcn_ldexp(1.0, 8 * kLimbW0 - 2 * (kLimbBits - e));
+    const unsigned char* tset = s_t + (zm & 1) * kLimbs * TB;
+    unsigned L[kLimbs][NV / 4];  // limb a of values 4 d .. 4 d + 3 in word d
+    auto convert = [&](int d) __attribute__((always_inline)) {
+      unsigned lo[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        lo[j] = (unsigned)__double_as_longlong(fma((double)cur[4 * d + j], scl, kMagic));
+      const unsigned t01l = __builtin_amdgcn_perm(lo[1], lo[0], 0x05010400u);
+      const unsigned t01h = __builtin_amdgcn_perm(lo[1], lo[0], 0x07030602u);
+      const unsigned t23l = __builtin_amdgcn_perm(lo[3], lo[2], 0x05010400u);
+      const unsigned t23h = __builtin_amdgcn_perm(lo[3], lo[2], 0x07030602u);
+      L[0][d] = __builtin_amdgcn_perm(t23l, t01l, 0x05040100u) ^ 0x80808080u;
+      L[1][d] = __builtin_amdgcn_perm(t23l, t01l, 0x07060302u) ^ 0x80808080u;
+      L[2][d] = __builtin_amdgcn_perm(t23h, t01h, 0x05040100u) ^ 0x80808080u;
+      L[3][d] = __builtin_amdgcn_perm(t23h, t01h, 0x07060302u) ^ 0x80808080u;
+    };
+    // the blocks in turn, each block's weights combined into its prefix one
+    // block later (the MFMA results are in by then), a piece of the
+    // conversion after each
+    constexpr int ND = NV / 4;  // conversion pieces (4 values each)
+    int piece = 0;
+    intx4 acc[2][kLimbNW];
+    auto combine = [&](int j) __attribute__((always_inline)) {
+      if (act && live[j]) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          double t = (double)acc[j & 1][kLimbNW - 1][r];
+#pragma unroll
+          for (int w = kLimbNW - 2; w >= 0; --w) t = fma(t, 256.0, (double)acc[j & 1][w][r]);
+          P[j][r] = fma(t, sc2, P[j][r]);
+        }
+      }
+    };
+#pragma unroll
+    for (int j = 0; j < NBW; ++j) {
+      if (live[j]) {  // (wave-uniform)
+#pragma unroll
+        for (int hf = 0; hf < RH; ++hf) {
+          const int co = ((4 * hf + kq) ^ (i & cmask)) << 4;
+          intx4 av[kLimbs], bv[kLimbs];
+#pragma unroll
+          for (int l = 0; l < kLimbs; ++l) {
+            av[l] = *(const intx4*)(tset + l * TB + (tb1[j] * 16 + i) * R + co);
+            bv[l] = *(const intx4*)(tset + l * TB + (tb2[j] * 16 + i) * R + co);
+          }
+#pragma unroll
+          for (int a = 0; a < kLimbs; ++a)
+#pragma unroll
+            for (int b = (kLimbW0 - a > 0 ? kLimbW0 - a : 0); b < kLimbs; ++b) {
+              const int w = a + b - kLimbW0;
+              const bool zero = hf == 0 && (a == 0 || b == kLimbs - 1);
+              acc[j & 1][w] = __builtin_amdgcn_mfma_i32_16x16x64_i8(
+                  av[a], bv[b], zero ? intx4{0, 0, 0, 0} : acc[j & 1][w], 0, 0, 0);
+            }
+        }
+      }
+      if (j > 0) combine(j - 1);
+      // conversion pieces spread over the blocks (the rest after the last)
+#pragma unroll
+      for (; piece < ND && piece * NBW < ND * (j + 1); ++piece) convert(piece);
+    }
+    combine(NBW - 1);
+#pragma unroll
+    for (; piece < ND; ++piece) convert(piece);
+    if (z < z1) {
+#pragma unroll
+      for (int c = 0; c < CPT; ++c) {
+        unsigned char* dst =
+            s_t + (z & 1) * kLimbs * TB + st * R + (((sc0 + TPF * c) ^ (st & cmask)) << 4);
+#pragma unroll
+        for (int a = 0; a < kLimbs; ++a)
+          *(intx4*)(dst + a * TB) = intx4{(int)L[a][4 * c], (int)L[a][4 * c + 1],
+                                          (int)L[a][4 * c + 2], (int)L[a][4 * c + 3]};
+      }
+    }
+    fetch(cur, z + 2 < z1 ? z + 2 : z1 - 1);  // unconditional, as k_gram_limb's
+    if (ends) finish(ie++);
+    sc2 = sc2_next;
+    if (z + 1 < z1) plane_max(nxt, z + 1);
+  };
+  fetch(sa, z0);
+  fetch(sb, z0 + 1 < z1 ? z0 + 1 : z1 - 1);
+  plane_max(sa, z0);
+  for (int z = z0; z <= z1; z += 2) {
+    step(z, sa, sb);
+    if (z + 1 <= z1) step(z + 1, sb, sa);
+  }
+}
+
 constexpr int kMaxDevices = 64;
 
 // Workgroups of a sliding kernel (fn, nthr threads, lds bytes) the current
@@ -736,6 +1027,47 @@ static int launch_gram_limb_t(const float* xc, double* gram, const Geom& g, int 
   return launch_status("gram (int8 sliding) kernel");
 }
 
+template <int NTB, int RH>
+static int launch_gram_limbw_t(const float* xc, double* gram, const Geom& g, int ix0, int nix,
+                               hipStream_t st) {
+  constexpr int NTHR = 64 * limbw_waves(NTB, RH);
+  static std::atomic<long long> cache[kMaxDevices];
+  const size_t lds = limbw_lds(NTB, RH);
+  int rc = prepare_lds((const void*)k_gram_limbw<NTB, RH>, lds);
+  if (rc != MPPCA_OK) return rc;
+  const int ncol = nix * g.ncy;
+  int nblk = 0, per_xcd = 0;
+  const int nseg = slide_runs(g, ncol * g.nvol, 1,
+                              resident_slots((const void*)k_gram_limbw<NTB, RH>, NTHR, lds, cache),
+                              &nblk, &per_xcd);
+  if (nseg == 0) return set_err(MPPCA_EUNSUPPORTED, "gram (int8 sliding, wide): too many columns");
+  hipLaunchKernelGGL((k_gram_limbw<NTB, RH>), dim3(8u * (unsigned)per_xcd), dim3(NTHR), lds, st, xc,
+                     gram, g, ix0, ncol, nseg, per_xcd);
+  return launch_status("gram (int8 sliding, wide) kernel");
+}
+
+// The wide int8 kernel takes sliding launches with 65..128 frames and a
+// footprint of at most 128 voxels (rows of 64 or 128 bytes per frame).
+static bool limbw_fits(const Geom& g) {
+  return g.T > 64 && g.T <= 128 && g.px * g.py <= 128 &&
+         (long long)g.px * g.Y * g.Z * g.T * 4 < (1LL << 31);
+}
+
+static int launch_gram_limbw(const float* xc, double* gram, const Geom& g, int ix0, int nix,
+                             hipStream_t st) {
+  const bool two = g.px * g.py > 64;
+  switch (ntb_of(g.T)) {
+    case 5: return two ? launch_gram_limbw_t<5, 2>(xc, gram, g, ix0, nix, st)
+                       : launch_gram_limbw_t<5, 1>(xc, gram, g, ix0, nix, st);
+    case 6: return two ? launch_gram_limbw_t<6, 2>(xc, gram, g, ix0, nix, st)
+                       : launch_gram_limbw_t<6, 1>(xc, gram, g, ix0, nix, st);
+    case 7: return two ? launch_gram_limbw_t<7, 2>(xc, gram, g, ix0, nix, st)
+                       : launch_gram_limbw_t<7, 1>(xc, gram, g, ix0, nix, st);
+    default: return two ? launch_gram_limbw_t<8, 2>(xc, gram, g, ix0, nix, st)
+                        : launch_gram_limbw_t<8, 1>(xc, gram, g, ix0, nix, st);
+  }
+}
+
 // The sliding kernel applies to launches over whole x-corner columns whose
 // plane (footprint x frames) fits the staging registers and the LDS, and whose
 // footprint byte offsets fit in 31 bits.
@@ -772,8 +1104,10 @@ int launch_gram64(const float* xc, double* gram, const Geom& g, long long b0, lo
       want != MPPCA_GRAM_PATCH)
     return set_err(MPPCA_EINVAL, "gram64: unknown Gram kernel %d", want);
   const int lntb = cols && (want == MPPCA_GRAM_AUTO || want == MPPCA_GRAM_LIMB) ? limb_ntb(g) : 0;
-  if (want == MPPCA_GRAM_LIMB && lntb == 0)
+  const bool wide = cols && (want == MPPCA_GRAM_AUTO || want == MPPCA_GRAM_LIMB) && limbw_fits(g);
+  if (want == MPPCA_GRAM_LIMB && lntb == 0 && !wide)
     return set_err(MPPCA_EUNSUPPORTED, "gram64: the int8 sliding kernel does not take this launch");
+  if (wide) return launch_gram_limbw(xc, gram, g, (int)(b0 / per_ix), (int)(nb / per_ix), st);
   if (want == MPPCA_GRAM_SLIDE && !(cols && slide_fits(g)))
     return set_err(MPPCA_EUNSUPPORTED, "gram64: the FP64 sliding kernel does not take this launch");
   if (lntb > 0) {

@@ -368,12 +368,13 @@ def _limb_bound(xc, patch, stride):
 
 
 def test_gram64_exact(pkg, lib):
-    """FP64 Gram. T > 64 (per-patch FP64 MFMA sums: products of float32 values
-    are exact, sums carry 53 bits) is held to the patch-sum bound; T <= 64
-    (sliding window over each x-y column, on int8 limbs of each plane rounded
-    at 2^-30 of its largest value, or on FP64 MFMA) to the prefix-difference
-    bound plus the quantisation bound; a launch over part of the patches (not
-    whole columns) takes the per-patch kernel."""
+    """FP64 Gram. A launch over whole x-y columns slides a window along each
+    column (on int8 limbs of each plane rounded at 2^-30 of its largest value
+    up to 128 frames with a footprint of at most 128 voxels, or on FP64 MFMA
+    up to 64 frames) and is held to the prefix-difference bound plus the
+    quantisation bound; a launch over part of the patches (not whole columns)
+    takes the per-patch kernel (products of float32 values exact in FP64,
+    sums of 53 bits), held to the patch-sum bound."""
     torch = _torch()
     rng = np.random.default_rng(8)
     for shape, patch, stride in [((13, 13, 13, 48), (11, 11, 11), 2), ((9, 8, 8, 100), (4, 4, 4), 2),
@@ -383,7 +384,12 @@ def test_gram64_exact(pkg, lib):
                                  # T = 24 with an 81-voxel footprint: the 48-frame int8 kernel
                                  ((12, 12, 10, 24), (9, 9, 3), 2),
                                  # T = 56 with a 156-voxel footprint: the FP64 sliding kernel
-                                 ((15, 14, 10, 56), (13, 12, 3), 2)]:
+                                 ((15, 14, 10, 56), (13, 12, 3), 2),
+                                 # 65..128 frames: the wide int8 kernel (one and two
+                                 # 64-voxel halves per tile row, 5..8 frame blocks)
+                                 ((13, 13, 12, 96), (11, 11, 5), 3), ((11, 10, 9, 100), (9, 9, 3), 2),
+                                 ((10, 10, 8, 80), (4, 4, 4), 2), ((10, 10, 7, 128), (9, 8, 3), 2),
+                                 ((9, 9, 8, 72), (3, 5, 4), 3)]:
         print("gram64", shape, patch, stride, flush=True)
         vol = (rng.random(shape[:3] + (3,)) @ rng.standard_normal((3, shape[3]))
                + 0.05 * rng.standard_normal(shape)).astype(np.float32)
@@ -392,12 +398,15 @@ def test_gram64_exact(pkg, lib):
         X = _gather(xc, patch, stride).astype(np.float64)
         G64 = np.einsum("bnt,bns->bts", X, X)
         A = np.abs(X)
-        bound = np.einsum("bnt,bns->bts", A, A) * X.shape[1] * 2.0 ** -52
-        if t <= 64:
-            bound = np.maximum(bound, _slide_bound(xc, patch, stride) + _limb_bound(xc, patch, stride)
-                               + np.abs(G64) * 2.0 ** -52)
+        exact = np.einsum("bnt,bns->bts", A, A) * X.shape[1] * 2.0 ** -52
+        # whole columns: a sliding kernel (int8 limbs up to 128 frames with a
+        # footprint of at most 128 voxels, else FP64 MFMA up to 64 frames)
+        sliding = t <= 64 or patch[0] * patch[1] <= 128
+        slide = (np.maximum(exact, _slide_bound(xc, patch, stride) + _limb_bound(xc, patch, stride)
+                            + np.abs(G64) * 2.0 ** -52) if sliding else exact)
         dx = _dev(xc)
         for b0, nb in ((0, P), (1, P - 2)):  # whole volume; a range of partial columns
+            bound = slide if b0 == 0 else exact
             g = torch.full((nb, t, t), float("nan"), dtype=torch.float64, device="cuda")
             pkg._lib.check(lib.mppca_gram64(dx.data_ptr(), g.data_ptr(), *shape, *patch, stride,
                                             b0, nb, _stream()), "gram64")
@@ -413,6 +422,7 @@ def test_gram64_nonfinite(pkg, lib):
     torch = _torch()
     rng = np.random.default_rng(11)
     for shape, patch, stride, bad in [((13, 13, 16, 48), (11, 11, 5), 2, np.nan),
+                                      ((12, 12, 10, 96), (11, 11, 3), 2, np.nan),
                                       ((10, 9, 12, 17), (5, 3, 4), 2, np.inf),
                                       ((8, 8, 10, 64), (4, 3, 3), 3, np.nan)]:
         print("gram64 nonfinite", shape, patch, stride, bad, flush=True)
