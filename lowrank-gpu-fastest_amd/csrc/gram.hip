@@ -678,6 +678,13 @@ __global__ __launch_bounds__(64 * NTB * (NTB + 1) / 2 * limb_cols(NTB)) void k_g
 // one after the other: the 13 RH MFMAs of a block, then its weights' FP64
 // combination into the block's prefix, so the int32 accumulators serve every
 // block in turn; the conversion of the next plane runs between the blocks.
+// Only the upper triangle is formed here: a window's -P goes into its entries
+// at its first plane and an FP64 atomic add (no value returned) adds P after
+// its last, one rounding of P(end) - P(start) with no load to wait for and no
+// snapshot registers; k_gram_mirror then copies each matrix's upper triangle
+// to its lower one in whole rows. (Mirror entries written from the block's
+// registers are scattered, one row per lane: as stores they cost 0.2 ms and as
+// atomic adds 0.8 ms of the 1.5 ms the kernel took at 96x96x48x96.)
 // ---------------------------------------------------------------------------
 // wavefronts per workgroup: two per frame block where three fit a SIMD
 // (12 at NTB = 6 with two 64-voxel halves, 10 at NTB = 5), else one
@@ -795,34 +802,21 @@ __global__ __launch_bounds__(64 * limbw_waves(NTB, RH)) void k_gram_limbw(const 
     }
     return (long long)iz * T * T + (long long)ti * T + tj;
   };
-  // At a window's first plane -P goes into its entry and the mirror entry;
-  // after its last plane an FP64 atomic add (no value returned) adds P to
-  // both: the same single rounding of P(end) - P(start) on both sides, with
-  // no load to wait for and no snapshot registers.
   auto start = [&](int iz) {
 #pragma unroll
     for (int j = 0; j < NBW; ++j)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int ti = tb1[j] * 16 + 4 * kq + r, tj = tb2[j] * 16 + i;
-        if (live[j] && ti < T && tj < T) {
+      for (int r = 0; r < 4; ++r)
+        if (live[j] && tb1[j] * 16 + 4 * kq + r < T && tb2[j] * 16 + i < T)
           Gc[eoff(j, iz, r, false)] = -P[j][r];
-          if (tb1[j] != tb2[j]) Gc[(long long)iz * T * T + (long long)tj * T + ti] = -P[j][r];
-        }
-      }
   };
   auto finish = [&](int iz) {
 #pragma unroll
     for (int j = 0; j < NBW; ++j)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int ti = tb1[j] * 16 + 4 * kq + r, tj = tb2[j] * 16 + i;
-        if (live[j] && ti < T && tj < T) {
+      for (int r = 0; r < 4; ++r)
+        if (live[j] && tb1[j] * 16 + 4 * kq + r < T && tb2[j] * 16 + i < T)
           unsafeAtomicAdd(Gc + eoff(j, iz, r, false), P[j][r]);
-          if (tb1[j] != tb2[j])
-            unsafeAtomicAdd(Gc + (long long)iz * T * T + (long long)tj * T + ti, P[j][r]);
-        }
-      }
   };
   __syncthreads();  // offset table written
 #pragma unroll
@@ -1027,6 +1021,28 @@ static int launch_gram_limb_t(const float* xc, double* gram, const Geom& g, int 
   return launch_status("gram (int8 sliding) kernel");
 }
 
+// Lower triangle of each T x T matrix from its upper one (the wide int8
+// Gram forms the upper block triangle only): one workgroup per 16 x 16 tile
+// on or below the diagonal, read as whole rows of its mirror tile, transposed
+// through LDS and written as whole rows.
+__global__ __launch_bounds__(256) void k_gram_mirror(double* __restrict__ gram, int T, long long nb) {
+  __shared__ double tile[16][17];
+  const int nt = (T + 15) / 16, ntri = nt * (nt + 1) / 2;
+  const long long u = blockIdx.x;
+  const long long b = u / ntri;
+  if (b >= nb) return;
+  int k = (int)(u - b * ntri), bi = 0;
+  while (k > bi) k -= ++bi;  // tile (bi, bj), bj = k <= bi
+  const int bj = k;
+  double* G = gram + b * T * T;
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  const int ur = bj * 16 + ty, uc = bi * 16 + tx;  // tile (bj, bi) of the upper part
+  if (ur < T && uc < T) tile[ty][tx] = G[(long long)ur * T + uc];
+  __syncthreads();
+  const int lr = bi * 16 + ty, lc = bj * 16 + tx;
+  if (lr < T && lc < T && lr > lc) G[(long long)lr * T + lc] = tile[tx][ty];
+}
+
 template <int NTB, int RH>
 static int launch_gram_limbw_t(const float* xc, double* gram, const Geom& g, int ix0, int nix,
                                hipStream_t st) {
@@ -1043,7 +1059,14 @@ static int launch_gram_limbw_t(const float* xc, double* gram, const Geom& g, int
   if (nseg == 0) return set_err(MPPCA_EUNSUPPORTED, "gram (int8 sliding, wide): too many columns");
   hipLaunchKernelGGL((k_gram_limbw<NTB, RH>), dim3(8u * (unsigned)per_xcd), dim3(NTHR), lds, st, xc,
                      gram, g, ix0, ncol, nseg, per_xcd);
-  return launch_status("gram (int8 sliding, wide) kernel");
+  if ((rc = launch_status("gram (int8 sliding, wide) kernel"))) return rc;
+  const long long nmat = (long long)ncol * g.ncz * g.nvol;
+  if (nmat >= (1LL << 31)) return set_err(MPPCA_EUNSUPPORTED, "gram (mirror): too many patches");
+  const int nt = (g.T + 15) / 16;
+  const long long nwg = nmat * (nt * (nt + 1) / 2);
+  if (nwg >= (1LL << 31)) return set_err(MPPCA_EUNSUPPORTED, "gram (mirror): too many tiles");
+  hipLaunchKernelGGL(k_gram_mirror, dim3((unsigned)nwg), dim3(256), 0, st, gram, g.T, nmat);
+  return launch_status("gram mirror kernel");
 }
 
 // The wide int8 kernel takes sliding launches with 65..128 frames and a
