@@ -121,10 +121,12 @@ int mppca_gram(const float* xc, float* gram, int X, int Y, int Z, int T,
 
 /* As mppca_gram with FP64 accumulation; gram[b] is T x T doubles. A launch
  * over whole x-corner columns (b0 and nb multiples of ncy * ncz) with
- * T <= 64 slides a window along each x-y column: each plane's values are
- * rounded to integers at 2^-30 of the plane's largest |x| and their products
- * summed exactly in int8-limb MFMA (v_mfma_i32_16x16x64_i8) before the FP64
- * prefix. Other launches sum exact FP64 products (v_mfma_f64_16x16x4f64).
+ * T <= 64, or T in 65..128 with a footprint px * py <= 128, slides a window
+ * along each x-y column: each plane's values are rounded to integers at
+ * 2^-30 of the plane's largest |x| and their products summed exactly in
+ * int8-limb MFMA (v_mfma_i32_16x16x64_i8) before the FP64 prefix (for
+ * T > 64 the upper block triangle, then mirrored). Other launches sum exact
+ * FP64 products (v_mfma_f64_16x16x4f64).
  * Replaces gram_kernel (step8_tuned.py:32-56) in the PLAIN form. */
 int mppca_gram64(const float* xc, double* gram, int X, int Y, int Z, int T,
                  int px, int py, int pz, int stride, int64_t b0, int64_t nb,
@@ -221,7 +223,7 @@ int mppca_mixed_eigenvalues(const float* vol, int X, int Y, int Z, int T, int px
                             int pz, int stride, void* workspace, int64_t workspace_bytes,
                             float* lam_desc, void* stream);
 
-/* Floats of one patch's factor block for T frames (< 0 if T > 64). */
+/* Floats of one patch's factor block for T frames (< 0 if T < 2 or T > 128). */
 int64_t mppca_factor_floats(int T);
 
 /* As mppca_eig64_mp, writing the rank-p factors of every patch into
