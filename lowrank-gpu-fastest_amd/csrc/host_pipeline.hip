@@ -591,7 +591,10 @@ extern "C" int mppca_denoise_host(const float* vol, float* out, int X, int Y, in
     while (up_slabs.load(std::memory_order_acquire) <= i && !err.load(std::memory_order_acquire))
       relax();
     if (err.load(std::memory_order_acquire)) break;  // reported after the join
-    hipStream_t S = sc;
+    // (plain form with DMA uploads: the input stream is free, and the sets
+    // alternate between it and the compute stream, so that one set's
+    // eigensolves, bound by their latency, run beside the next set's Gram)
+    hipStream_t S = pipelined && !mixed && dma_up && si != sc && (set & 1) ? si : sc;
     if (dma_up) {  // the slab's uploads have landed (the host sees the engines' signal)
       const hsa_signal_value_t v = wait_signal(C.sig_up[i], &err);
       if (v > 0) break;
@@ -616,6 +619,8 @@ extern "C" int mppca_denoise_host(const float* vol, float* out, int X, int Y, in
       while (j + 2 < K && uploaded(j + 1)) ++j;
     }
     if (!dma_up && (rc = hip_ok(hipStreamWaitEvent(S, C.ev_in[j], 0), "wait"))) break;
+    // (this set's Gram reads planes the previous set centred, maybe on the other stream)
+    if (prev_gram && (rc = hip_ok(hipStreamWaitEvent(S, prev_gram, 0), "wait"))) break;
     const bool last = j + 1 == K;
     const int xlo = xa[i], xhi = xa[j + 1];
     i = j + 1;

@@ -90,12 +90,13 @@ class Denoiser:
         self.margin = torch.empty(self.n_patches, dtype=torch.float32, device=dev)
         self.sweeps = torch.empty(self.n_patches, dtype=torch.int32, device=dev)
         self._pin_in = None
-        # host -> host pipeline: x-slabs and host copy threads (a sweep of 4-16
-        # slabs and 6-16 threads on the default volume: 7 slabs and 6 threads
-        # best; more slabs are slower because every launch of a slab costs at
-        # least one pass of its longest workgroup, however few workgroups it has,
-        # and more threads take cores from the threads that drive the GPU)
-        self.nslabs = 7
+        # host -> host pipeline: x-slabs and host copy threads (a sweep of 5-20
+        # slabs on the default volume: 9-11 slabs best, 2.73-2.75 ms against
+        # 2.80 at 7 and 2.89 at 16; more slabs are slower because every launch
+        # of a slab costs at least one pass of its longest workgroup, however
+        # few workgroups it has; 6 threads: more take cores from the threads
+        # that drive the GPU)
+        self.nslabs = 9
         self.nthreads = max(1, min(6, os.cpu_count() or 1))
 
     def _geom_args(self):
