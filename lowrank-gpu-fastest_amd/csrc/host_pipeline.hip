@@ -225,7 +225,7 @@ struct PipeCtx {
   int device;
   int nstage;  // staging threads
   Team team;   // thread 0 uploads, thread 1 downloads, threads 2.. stage
-  std::vector<hipEvent_t> ev_in, ev_gram, ev_eig, ev_rec, ev_out;
+  std::vector<hipEvent_t> ev_in, ev_ctr, ev_gram, ev_eig, ev_rec, ev_out;
   std::vector<hsa_signal_t> sig_up, sig_dn;
   int32_t* h_info = nullptr;  // pinned: the per-patch status, downloaded with the result
   size_t h_info_n = 0;
@@ -383,6 +383,7 @@ extern "C" int mppca_denoise_host(const float* vol, float* out, int X, int Y, in
   CtxLease lease{acquire_ctx(dev, nthreads)};
   PipeCtx& C = *lease.c;
   if ((rc = C.events(C.ev_in, K, hipEventDisableTiming)) ||
+      (rc = C.events(C.ev_ctr, K, hipEventDisableTiming)) ||
       (rc = C.events(C.ev_gram, K, hipEventDisableTiming)) ||
       (rc = C.events(C.ev_eig, K + 1, hipEventDisableTiming)) ||
       (rc = C.events(C.ev_rec, NJ + 1, hipEventDisableTiming | hipEventReleaseToSystem)) ||
@@ -587,6 +588,7 @@ extern "C" int mppca_denoise_host(const float* vol, float* out, int X, int Y, in
     return hipEventQuery(C.ev_in[i]) == hipSuccess;
   };
   hipEvent_t prev_gram = nullptr;  // the last Gram launch of the plain form
+  hipEvent_t prev_ctr = nullptr;   // the last set's centring
   for (int i = 0, set = 0; i < K; ++set) {
     while (up_slabs.load(std::memory_order_acquire) <= i && !err.load(std::memory_order_acquire))
       relax();
@@ -619,8 +621,10 @@ extern "C" int mppca_denoise_host(const float* vol, float* out, int X, int Y, in
       while (j + 2 < K && uploaded(j + 1)) ++j;
     }
     if (!dma_up && (rc = hip_ok(hipStreamWaitEvent(S, C.ev_in[j], 0), "wait"))) break;
-    // (this set's Gram reads planes the previous set centred, maybe on the other stream)
-    if (prev_gram && (rc = hip_ok(hipStreamWaitEvent(S, prev_gram, 0), "wait"))) break;
+    // This set's Gram reads planes that the previous set centred, maybe on the
+    // other stream (a set that completes no corner runs its centring only);
+    // the sets before it centred theirs earlier on either stream.
+    if (prev_ctr && (rc = hip_ok(hipStreamWaitEvent(S, prev_ctr, 0), "wait"))) break;
     const bool last = j + 1 == K;
     const int xlo = xa[i], xhi = xa[j + 1];
     i = j + 1;
@@ -637,6 +641,8 @@ extern "C" int mppca_denoise_host(const float* vol, float* out, int X, int Y, in
                                       mixed ? w.amax + (size_t)xlo * kAmaxSpread : nullptr,
                                       (long long)Y * Z)))
       break;
+    if ((rc = hip_ok(hipEventRecord(C.ev_ctr[set], S), "event"))) break;
+    prev_ctr = C.ev_ctr[set];
     int ix_new = ix_done;
     while (ix_new < g.ncx && corner_of(ix_new, g.nsx, X, px, stride) + px <= xhi) ++ix_new;
     long long rb0 = 0, rnb = 0;  // this set's patches

@@ -119,3 +119,22 @@ def test_host_pipeline_alternating_volumes(pkg):
         got = d(vols[k % 3])
         _assert_same(got, wants[k % 3], "factors")
         assert int(d.info()["sweeps"].min()) >= 0
+
+
+def test_host_pipeline_slabs_without_corners(pkg):
+    """Slabs narrower than the stride complete no corner and only centre their
+    planes; the next set's Gram reads those planes, whichever compute stream
+    either set ran on. Alternating volumes (the workspace holds the other
+    volume's centred planes when a call starts) each give their own
+    device-path result and ranks, for several slab counts."""
+    shape, patch, stride = (23, 23, 19, 24), (11, 11, 11), 4
+    vols = [_volume(shape, s) for s in (31, 32)]
+    d = pkg.Denoiser(shape, patch, stride)
+    wants = [_device_result(d, v) for v in vols]
+    for nslabs in (9, 12, 23):
+        d.nslabs = nslabs
+        for k in range(6):
+            print("slabs without corners: nslabs", nslabs, "call", k, flush=True)
+            got = d(vols[k % 2])
+            _assert_same(got, wants[k % 2][0], "factors")
+            np.testing.assert_array_equal(d.info()["rank"], wants[k % 2][1])
