@@ -388,7 +388,7 @@ __host__ __device__ constexpr size_t limb_lds(int ntb, int F) {
   return (size_t)limb_cols(ntb) * limb_col_bytes(ntb, F);
 }
 
-template <int NTB, int RH>
+template <int NTB, int RH, bool PAST = false>
 __global__ __launch_bounds__(64 * NTB * (NTB + 1) / 2 * limb_cols(NTB)) void k_gram_limb(
     const float* __restrict__ xc, double* __restrict__ gram, Geom g, int ix0, int ncol,
     int nseg, int per_xcd) {
@@ -475,6 +475,10 @@ __global__ __launch_bounds__(64 * NTB * (NTB + 1) / 2 * limb_cols(NTB)) void k_g
     }
   }
   const bool diag = tb1 == tb2;
+  // PAST (T in 17..32 on the 48-frame kernel, the only launch with blocks
+  // wholly past T): the wavefront of such a block stages, converts and
+  // reduces plane maxima, and issues no MFMA
+  const bool dead = PAST && tb2 * 16 >= T;
   const int t2 = tb2 * 16 + i;
   const unsigned char* opA = s_t + (tb1 * 16 + i) * R;  // frame tb1*16 + i of limb tile 0
   const unsigned char* opB = s_t + (tb2 * 16 + i) * R;
@@ -603,6 +607,11 @@ __global__ __launch_bounds__(64 * NTB * (NTB + 1) / 2 * limb_cols(NTB)) void k_g
     // matrix-pipe slots.
     constexpr int NPAIR = kLimbPairs * RH;
     int pi = 0, piece = 0;
+    if (dead) {  // (wave-uniform)
+#pragma unroll
+      for (int d = 0; d < 4; ++d) convert(d);
+      piece = 4;
+    } else {
 #pragma unroll
     for (int hf = 0; hf < RH; ++hf) {
       const int co = ((4 * hf + kq) ^ sw) << 4;
@@ -630,6 +639,7 @@ __global__ __launch_bounds__(64 * NTB * (NTB + 1) / 2 * limb_cols(NTB)) void k_g
     }
 #pragma unroll
     for (; piece < 4; ++piece) convert(piece);
+    }
 #pragma unroll
     for (int a = 0; a < kLimbs; ++a)
       asm volatile("" ::"v"(L[a][0]), "v"(L[a][1]), "v"(L[a][2]), "v"(L[a][3]));
@@ -644,7 +654,7 @@ __global__ __launch_bounds__(64 * NTB * (NTB + 1) / 2 * limb_cols(NTB)) void k_g
     // z1 - 1, unused), so that the wait for the snapshot loads above, which
     // precede these in issue order, leaves them in flight
     fetch(cur, z + 2 < z1 ? z + 2 : z1 - 1);
-    if (act) {
+    if (act && !dead) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         double t = (double)acc[kLimbNW - 1][r];
@@ -1002,21 +1012,21 @@ static int launch_gram_slide_t(const float* xc, double* gram, const Geom& g, int
   return launch_status("gram (sliding) kernel");
 }
 
-template <int NTB, int RH>
+template <int NTB, int RH, bool PAST = false>
 static int launch_gram_limb_t(const float* xc, double* gram, const Geom& g, int ix0, int nix,
                               hipStream_t st) {
   constexpr int NTHR = 64 * NTB * (NTB + 1) / 2 * limb_cols(NTB);
   static std::atomic<long long> cache[kMaxDevices];
   const size_t lds = limb_lds(NTB, g.px * g.py);
-  int rc = prepare_lds((const void*)k_gram_limb<NTB, RH>, lds);
+  int rc = prepare_lds((const void*)k_gram_limb<NTB, RH, PAST>, lds);
   if (rc != MPPCA_OK) return rc;
   const int ncol = nix * g.ncy;
   int nblk = 0, per_xcd = 0;
   const int nseg = slide_runs(g, ncol * g.nvol, limb_cols(NTB),
-                              resident_slots((const void*)k_gram_limb<NTB, RH>, NTHR, lds, cache),
+                              resident_slots((const void*)k_gram_limb<NTB, RH, PAST>, NTHR, lds, cache),
                               &nblk, &per_xcd);
   if (nseg == 0) return set_err(MPPCA_EUNSUPPORTED, "gram (int8 sliding): too many columns");
-  hipLaunchKernelGGL((k_gram_limb<NTB, RH>), dim3(8u * (unsigned)per_xcd), dim3(NTHR), lds, st, xc,
+  hipLaunchKernelGGL((k_gram_limb<NTB, RH, PAST>), dim3(8u * (unsigned)per_xcd), dim3(NTHR), lds, st, xc,
                      gram, g, ix0, ncol, nseg, per_xcd);
   return launch_status("gram (int8 sliding) kernel");
 }
@@ -1140,6 +1150,9 @@ int launch_gram64(const float* xc, double* gram, const Geom& g, long long b0, lo
       case 1: return launch_gram_limb_t<1, 1>(xc, gram, g, ix0, nix, st);
       case 2: return launch_gram_limb_t<2, 1>(xc, gram, g, ix0, nix, st);
       case 3:
+        if (g.T <= 32)  // the 48-frame kernel for 17..32 frames: blocks past T
+          return two ? launch_gram_limb_t<3, 2, true>(xc, gram, g, ix0, nix, st)
+                     : launch_gram_limb_t<3, 1, true>(xc, gram, g, ix0, nix, st);
         return two ? launch_gram_limb_t<3, 2>(xc, gram, g, ix0, nix, st)
                    : launch_gram_limb_t<3, 1>(xc, gram, g, ix0, nix, st);
       default:

@@ -381,8 +381,11 @@ def test_gram64_exact(pkg, lib):
                                  ((10, 9, 8, 17), (5, 3, 4), 2), ((8, 8, 8, 128), (5, 5, 5), 3),
                                  ((9, 7, 30, 33), (3, 4, 5), 3), ((8, 9, 23, 64), (4, 3, 3), 5),
                                  ((6, 5, 40, 2), (2, 2, 7), 1),
-                                 # T = 24 with an 81-voxel footprint: the 48-frame int8 kernel
-                                 ((12, 12, 10, 24), (9, 9, 3), 2),
+                                 # T = 24, 32 and 20 with 81-, 121- and 100-voxel footprints:
+                                 # the 48-frame int8 kernel (wavefronts of blocks past T
+                                 # stage and convert but issue no MFMA)
+                                 ((12, 12, 10, 24), (9, 9, 3), 2), ((13, 13, 9, 32), (11, 11, 3), 2),
+                                 ((12, 12, 8, 20), (10, 10, 4), 3),
                                  # T = 56 with a 156-voxel footprint: the FP64 sliding kernel
                                  ((15, 14, 10, 56), (13, 12, 3), 2),
                                  # 65..128 frames: the wide int8 kernel (one and two
