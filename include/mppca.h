@@ -59,8 +59,11 @@ extern "C" {
  *          MPPCA_MIXED_RESOLVE_MARGIN are solved again from the exact FP64
  *          Gram with the FP64 eigensolver, up to MPPCA_MIXED_RESOLVE_FRAC of
  *          the patches (more are flagged -3 in the status output). Rank-p
- *          factors and FP32 MFMA reconstruction as PLAIN. The margin output
- *          holds the relative margin. Its reference is the FP64 oracle. */
+ *          factors as FP16 hi/lo pairs (each vector's largest entry
+ *          positive) and the reconstruction on FP16 MFMA, three products
+ *          with FP32 accumulation, the tile's centred rows scaled by a power
+ *          of two. The margin output holds the relative margin. Its
+ *          reference is the FP64 oracle. */
 #define MPPCA_FORM_PLAIN 0
 #define MPPCA_FORM_FP32 1
 #define MPPCA_FORM_MIXED 2

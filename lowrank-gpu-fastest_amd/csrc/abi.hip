@@ -474,7 +474,8 @@ int run_device_path(const float* vol, float* out, const Geom& g, int form, const
     return rc;
   if (form == MPPCA_FORM_MIXED) {
     if ((rc = run_mixed_solve(g, w, rk, mg, inf, st))) return rc;
-    if (w.fac) return launch_recon_factors(w.xc, w.fac, rk, w.weight, w.mean, out, g, 0, g.X, st);
+    if (w.fac)
+      return launch_recon_factors(w.xc, w.fac, rk, w.weight, w.mean, out, g, 0, g.X, st, true);
   } else if (form == MPPCA_FORM_PLAIN) {
     if ((rc = launch_gram64(w.xc, (double*)w.gram, g, 0, g.P, st))) return rc;
     if ((rc = launch_eig64_mp((double*)w.gram, P, T, g.N, w.proj, w.fac, rk, w.weight, mg,

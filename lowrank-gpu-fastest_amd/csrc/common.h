@@ -210,6 +210,49 @@ __device__ inline void zero_factor_slots(float* F, int p, int TF, int lane) {
     }
 }
 
+// The precision form's factor blocks: the same size, tiles and slots, each
+// float plane of a slot tile replaced by an FP16 hi plane and an FP16 lo plane
+// (v ~ hi + lo, lo the rounding of v - hi: 22 significant bits), scaled so
+// that the reconstruction's FP16 operands stay in the normal range. In slot
+// tile nc, as half indices from the block's start:
+//   Ut: 2*nc*16*TF + c*TF + t (hi), + 16*TF (lo), of 2^8 * the entry;
+//   Uw: 2*(TF*TF + nc*16*TF) + t*16 + c (hi), + 16*TF (lo), of 2^9 * w_b * the
+//   entry (w_b <= 1/2).
+constexpr double kH3UtScale = 256.0, kH3UwScale = 512.0;
+__device__ inline void h3_store(_Float16* H, long long hi_at, int lo_off, double v) {
+  const _Float16 h = (_Float16)v;
+  H[hi_at] = h;
+  H[hi_at + lo_off] = (_Float16)(v - (double)h);
+}
+__device__ inline void write_factors_h3(float* F, const double* z, int ldz, int g0, int gn, int p,
+                                        int T, int TF, double w, int lane) {
+  _Float16* H = (_Float16*)F;
+  for (int t = lane; t < TF; t += 64)
+    for (int g = 0; g < gn; ++g) {
+      const int G = g0 + g, nc = G >> 4;
+      const int q = p - 16 * nc < 16 ? p - 16 * nc : 16;
+      const int c = factor_slot(G & 15, q, p);
+      const double v = t < T ? z[g * ldz + t] : 0.0;
+      h3_store(H, 2LL * nc * 16 * TF + c * TF + t, 16 * TF, kH3UtScale * v);
+      h3_store(H, 2LL * (TF * TF + nc * 16 * TF) + t * 16 + c, 16 * TF, kH3UwScale * w * v);
+    }
+}
+__device__ inline void zero_factor_slots_h3(float* F, int p, int TF, int lane) {
+  _Float16* H = (_Float16*)F;
+  const int nct = p > 0 ? (p + 15) >> 4 : 1;
+  for (int t = lane; t < TF; t += 64)
+    for (int nc = 0; nc < nct; ++nc) {
+      const int q = p - 16 * nc < 16 ? p - 16 * nc : 16;
+      const int S2 = factor_groups(q, p);
+      for (int c = 0; c < 16; ++c) {
+        const int a = c >> 2, s = c & 3;
+        if (s < S2 && a * S2 + s < q) continue;  // slot of vector a*S2 + s
+        h3_store(H, 2LL * nc * 16 * TF + c * TF + t, 16 * TF, 0.0);
+        h3_store(H, 2LL * (TF * TF + nc * 16 * TF) + t * 16 + c, 16 * TF, 0.0);
+      }
+    }
+}
+
 // Error models of the two routes (see MpErr).
 constexpr MpErr kErrFp32Route = {16.0, 0.0};
 constexpr MpErr kErrFp64Route = {1.0, 32.0};
@@ -288,10 +331,11 @@ int run_mixed_solve(const Geom& g, const Workspace& w, int* rank, float* margin,
 int launch_recon(const float* xc, const float* proj, const float* weight, float* acc,
                  const Geom& g, long long b0, long long nb, hipStream_t st);
 // Output planes x in [x0, x1) (x0 a multiple of recon_factor_tile_x(g), x1 too
-// or X); reads the factors of every patch that meets them.
+// or X); reads the factors of every patch that meets them (h3: the precision
+// form's FP16 hi/lo factor blocks).
 int launch_recon_factors(const float* xc, const float* fac, const int* rank, const float* weight,
                          const float* mean, float* out, const Geom& g, int x0, int x1,
-                         hipStream_t st);
+                         hipStream_t st, bool h3 = false);
 int recon_factor_tile_x(const Geom& g);
 
 // Host copy of n bytes into pinned staging (host_copy.cpp): non-temporal

@@ -1453,7 +1453,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(TP > 64 ? 2 
   if (__any(!finite)) {  // non-finite input: no rank, no projector, flagged
     if (Pb)
       for (int idx = lane; idx < T * T; idx += 64) Pb[idx] = 0.f;
-    if (Fb) zero_factor_slots(Fb, 0, TP, lane);  // read by paired reconstruction
+    if (Fb) {  // read by paired reconstruction
+      if constexpr (RULE64)
+        zero_factor_slots_h3(Fb, 0, TP, lane);
+      else
+        zero_factor_slots(Fb, 0, TP, lane);
+    }
     if (lane == 0) {
       rank[b] = 0;
       weight[b] = 0.5f;
@@ -1753,7 +1758,39 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(TP > 64 ? 2 
       }
       __syncthreads();
     }
-    if (Fb) write_factors(Fb, s_z, LD, g0, gn, p, T, TP, 1.0 / (2.0 + (double)p), lane);
+    if (RULE64 && Fb) {
+      // The precision form fixes each vector's sign (its largest entry, the
+      // first of equals, positive): the sign the twisted factorisation gives
+      // can turn with the last bits of the eigenvalue, which depend on how a
+      // launch packs the eigenvalue search, and the FP16 MFMAs that
+      // reconstruct from the factors do not round a negated sum to the
+      // negated result, so the sign would reach the output's last bit.
+      __shared__ int s_neg[kRowGroup];
+      if (lane < gn) {
+        double big = -1.0;
+        int neg = 0;
+        for (int t = 0; t < T; ++t) {
+          const double x = s_z[lane * LD + t];
+          if (fabs(x) > big) {
+            big = fabs(x);
+            neg = x < 0.0;
+          }
+        }
+        s_neg[lane] = neg;
+      }
+      __syncthreads();
+      for (int q = lane; q < gn * T; q += 64) {
+        const int gg = q / T, t = q - gg * T;
+        if (s_neg[gg]) s_z[gg * LD + t] = -s_z[gg * LD + t];
+      }
+      __syncthreads();
+    }
+    if (Fb) {  // (the precision form: FP16 hi/lo factors)
+      if constexpr (RULE64)
+        write_factors_h3(Fb, s_z, LD, g0, gn, p, T, TP, 1.0 / (2.0 + (double)p), lane);
+      else
+        write_factors(Fb, s_z, LD, g0, gn, p, T, TP, 1.0 / (2.0 + (double)p), lane);
+    }
     // P[l][c] (+)= sum_g x_g[l] x_g[c]; rank > 8 accumulates over passes in
     // global memory (each lane owns its rows, so no other lane touches them)
     if (Pb)
@@ -1775,7 +1812,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(TP > 64 ? 2 
       }
     __syncthreads();
   }
-  if (Fb) zero_factor_slots(Fb, p, TP, lane);
+  if (Fb) {
+    if constexpr (RULE64)
+      zero_factor_slots_h3(Fb, p, TP, lane);
+    else
+      zero_factor_slots(Fb, p, TP, lane);
+  }
 }
 
 // Modes of the three-kernel solve: PLAIN (FP64 reduction, step 0's MP rule),

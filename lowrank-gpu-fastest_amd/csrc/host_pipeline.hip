@@ -563,7 +563,7 @@ extern "C" int mppca_denoise_host(const float* vol, float* out, int X, int Y, in
       hipStream_t sk = st2 && (k & 1) ? st2 : st;
       if (b > x_done) {
         if (pipelined && (r = launch_recon_factors(w.xc, w.fac, rk, w.weight, w.mean, d_out, g,
-                                                   x_done, b, sk)))
+                                                   x_done, b, sk, mixed)))
           return r;
         x_done = b;
       }

@@ -144,10 +144,56 @@ __device__ __forceinline__ void touch_range(int a0, int ext, int S, int p, int s
 // lane) live in registers; the wave writes out = mean + D / W once. Patches
 // are visited in increasing index, so W adds the weights in the order step 0
 // accumulates wsum (step0_numpy_loop.py:36-39).
-// 128 VGPRs at T <= 48 give 4 waves per SIMD.
+// 128 VGPRs at T <= 48 give 4 waves per SIMD (3 for H3 at T 33..48, whose
+// operand splits need ~150).
+//
+// H3 (the precision form): the factors arrive as FP16 hi/lo pairs (common.h)
+// and both products run on FP16 MFMA with FP32 accumulation, three products
+// each (hi*hi + hi*lo + lo*hi; the lo*lo term is below 2^-22 of the product):
+//   C^T: v_mfma_f32_16x16x32_f16 per 32 frames (16x16x16 for a last 16), the
+//        tile's centred rows scaled by one power of two per wavefront (its
+//        largest |x| to [4, 8), so |C| < 2^15 for T <= 128) and split once;
+//   D:   v_mfma_f32_16x16x16f16 over all 16 slots of a tile, C split after
+//        the mask.
+// Each 32-bit register of the FP32 form's operand arrays then holds two FP16
+// values: for 32 frames, words 8m..8m+3 the hi and 8m+4..8m+7 the lo of
+// frames 32m + 8kq .. +7; for a last 16, two words hi and two lo of frames
+// 32m + 4kq .. +3; per frame block tb of Uw, words 4tb, 4tb+1 hi and 4tb+2,
+// 4tb+3 lo of slots 4kq .. +3. The register counts are the FP32 form's.
 // ---------------------------------------------------------------------------
-template <int NTB, int SX>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NTB >= 4 ? 1 : (SX >= 4 ? 2 : 4))))
+typedef _Float16 halfx4 __attribute__((ext_vector_type(4)));
+typedef _Float16 halfx8 __attribute__((ext_vector_type(8)));
+typedef unsigned uintx2 __attribute__((ext_vector_type(2)));
+typedef unsigned uintx4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ halfx8 h3_w8(const float* w) {
+  const uintx4 u = {__float_as_uint(w[0]), __float_as_uint(w[1]), __float_as_uint(w[2]),
+                    __float_as_uint(w[3])};
+  return __builtin_bit_cast(halfx8, u);
+}
+__device__ __forceinline__ halfx4 h3_w4(const float* w) {
+  const uintx2 u = {__float_as_uint(w[0]), __float_as_uint(w[1])};
+  return __builtin_bit_cast(halfx4, u);
+}
+// n floats of v (n = 4 or 8) -> n/2 words of hi then n/2 words of lo, in place
+template <int N>
+__device__ __forceinline__ void h3_split_words(float* v) {
+  _Float16 h[N], l[N];
+#pragma unroll
+  for (int e = 0; e < N; ++e) {
+    h[e] = (_Float16)v[e];
+    l[e] = (_Float16)(v[e] - (float)h[e]);
+  }
+#pragma unroll
+  for (int e = 0; e < N / 2; ++e) {
+    v[e] = __uint_as_float((unsigned)__builtin_bit_cast(unsigned short, h[2 * e]) |
+                           (unsigned)__builtin_bit_cast(unsigned short, h[2 * e + 1]) << 16);
+    v[N / 2 + e] = __uint_as_float((unsigned)__builtin_bit_cast(unsigned short, l[2 * e]) |
+                                   (unsigned)__builtin_bit_cast(unsigned short, l[2 * e + 1]) << 16);
+  }
+}
+
+template <int NTB, int SX, bool H3 = false>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NTB >= 4 ? 1 : (SX >= 4 ? 2 : (H3 && NTB == 3 ? 3 : 4)))))
 void k_recon_lr(const float* __restrict__ xc,
                                                   const float* __restrict__ fac,
                                                   const int* __restrict__ rank,
@@ -185,8 +231,46 @@ void k_recon_lr(const float* __restrict__ xc,
   // (their slots and factor rows are zero, so they add nothing)
   const int yi = y0 + (i >> 2), zi = z0 + (i & 3);
   float xv[SX][KQ];
+  constexpr int N32 = NTB / 2, N16 = NTB & 1;  // H3: frame blocks of 32, and a last 16
+  float dscale = 1.f;  // H3: 2^-17 / the wavefront's scale of its rows
+  if constexpr (H3) {
+    // xv[j][8m + e] = frame 32m + 8kq + e; xv[j][8 N32 + e] = frame 32 N32 + 4kq + e
+    unsigned mx = 0;
 #pragma unroll
-  for (int j = 0; j < SX; ++j) {
+    for (int j = 0; j < SX; ++j) {
+      const int xj = min(x0 + j, g.X - 1);
+      const float* row = xc + (((long long)xj * g.Y + min(yi, g.Y - 1)) * g.Z + min(zi, g.Z - 1)) * T;
+#pragma unroll
+      for (int m = 0; m < N32 + N16; ++m) {
+        const int n = m < N32 ? 8 : 4, f0 = 32 * m + n * kq;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          if (e >= n) break;
+          const int f = f0 + e;
+          const float v = row[f < T ? f : T - 1];
+          xv[j][8 * m + e] = f < T ? v : 0.f;
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < KQ; ++k) mx = max(mx, __float_as_uint(xv[j][k]) & 0x7fffffffu);
+    }
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) mx = max(mx, (unsigned)__shfl_xor((int)mx, o));
+    // largest |x| < 2^e: rows scaled by 2^(3 - e); a non-finite or zero tile keeps scale 1
+    const int e = (mx == 0u || mx >= 0x7f800000u) ? 3 : __builtin_amdgcn_frexp_expf(__uint_as_float(mx));
+    const float sx = __builtin_amdgcn_ldexpf(1.f, 3 - e);
+    dscale = __builtin_amdgcn_ldexpf(1.f, e - 3 - 17);
+#pragma unroll
+    for (int j = 0; j < SX; ++j) {
+#pragma unroll
+      for (int k = 0; k < KQ; ++k) xv[j][k] *= sx;
+#pragma unroll
+      for (int m = 0; m < N32; ++m) h3_split_words<8>(&xv[j][8 * m]);
+      if (N16) h3_split_words<4>(&xv[j][8 * N32]);
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < SX && !H3; ++j) {
     const int xj = min(x0 + j, g.X - 1);
     const float* row = xc + (((long long)xj * g.Y + min(yi, g.Y - 1)) * g.Z + min(zi, g.Z - 1)) * T;
     if ((T & 3) == 0) {
@@ -301,6 +385,29 @@ void k_recon_lr(const float* __restrict__ xc,
     const unsigned oA = std_ || i < 8 ? 0u : S.offb, oB = std_ || kq < 2 ? 0u : S.offb;
     const int rowA = std_ ? i : (i & 7), colB = std_ ? 4 * kq : 4 * (kq & 1);
     const char* F = (const char*)S.fa;
+    if constexpr (H3) {  // byte offsets of the FP16 hi planes (lo: + 32 TF bytes)
+      const unsigned uA = oA + 2u * (unsigned)(2 * nc * 16 * TF + rowA * TF);
+#pragma unroll
+      for (int m = 0; m < N32; ++m) {
+        const unsigned a = uA + 2u * (unsigned)(32 * m + 8 * kq);
+        const float4 h = *(const float4*)(F + a), l = *(const float4*)(F + a + 32u * TF);
+        ut[8 * m] = h.x; ut[8 * m + 1] = h.y; ut[8 * m + 2] = h.z; ut[8 * m + 3] = h.w;
+        ut[8 * m + 4] = l.x; ut[8 * m + 5] = l.y; ut[8 * m + 6] = l.z; ut[8 * m + 7] = l.w;
+      }
+      if (N16) {
+        const unsigned a = uA + 2u * (unsigned)(32 * N32 + 4 * kq);
+        const float2 h = *(const float2*)(F + a), l = *(const float2*)(F + a + 32u * TF);
+        ut[8 * N32] = h.x; ut[8 * N32 + 1] = h.y; ut[8 * N32 + 2] = l.x; ut[8 * N32 + 3] = l.y;
+      }
+      const unsigned uB = oB + 2u * (unsigned)(2 * (TF * TF + nc * 16 * TF) + i * 16 + colB);
+#pragma unroll
+      for (int tb = 0; tb < NTB; ++tb) {
+        const unsigned b = uB + 512u * tb;  // 16 frames of 16 slots of 2 bytes
+        const float2 h = *(const float2*)(F + b), l = *(const float2*)(F + b + 32u * TF);
+        uw[4 * tb] = h.x; uw[4 * tb + 1] = h.y; uw[4 * tb + 2] = l.x; uw[4 * tb + 3] = l.y;
+      }
+      return;
+    }
     const unsigned uA = oA + 4u * (unsigned)(nc * 16 * TF + rowA * TF + kq * KQ);
 #pragma unroll
     for (int v = 0; v < KV; ++v) {
@@ -354,6 +461,55 @@ void k_recon_lr(const float* __restrict__ xc,
       if (nc > 0) load_fac(S, nc, ut, uw);  // rank above 16: further slot tiles
       const int S2 = S.kind == 0 ? (min(16, pa - 16 * nc) + 3) >> 2
                                  : ((S.kind == 2 ? max(pa, pb) : pa) + 1) >> 1;
+      if constexpr (H3) {
+#pragma unroll
+        for (int j = 0; j < SX; ++j) {
+          const bool inA = x0 + j >= cxa && x0 + j < cxa + g.px;
+          const bool inB = S.kind == 2 && x0 + j >= cxb && x0 + j < cxb + g.px;
+          if (!inA && !inB) continue;
+          floatx4 ct = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+          for (int m = 0; m < N32; ++m) {
+            const halfx8 uh = h3_w8(ut + 8 * m), ul = h3_w8(ut + 8 * m + 4);
+            const halfx8 xh = h3_w8(&xv[j][8 * m]), xl = h3_w8(&xv[j][8 * m + 4]);
+            ct = __builtin_amdgcn_mfma_f32_16x16x32_f16(ul, xh, ct, 0, 0, 0);
+            ct = __builtin_amdgcn_mfma_f32_16x16x32_f16(uh, xl, ct, 0, 0, 0);
+            ct = __builtin_amdgcn_mfma_f32_16x16x32_f16(uh, xh, ct, 0, 0, 0);
+          }
+          if (N16) {
+            // its own accumulator, added on the VALU: a 16x16x16 MFMA that
+            // takes a 16x16x32 MFMA's result as its C operand read it before
+            // it was written (measured: the 32 frames' sums were lost), so
+            // no chain mixes the two shapes
+            const halfx4 uh = h3_w4(ut + 8 * N32), ul = h3_w4(ut + 8 * N32 + 2);
+            const halfx4 xh = h3_w4(&xv[j][8 * N32]), xl = h3_w4(&xv[j][8 * N32 + 2]);
+            floatx4 c16 = floatx4{0.f, 0.f, 0.f, 0.f};
+            c16 = __builtin_amdgcn_mfma_f32_16x16x16f16(ul, xh, c16, 0, 0, 0);
+            c16 = __builtin_amdgcn_mfma_f32_16x16x16f16(uh, xl, c16, 0, 0, 0);
+            c16 = __builtin_amdgcn_mfma_f32_16x16x16f16(uh, xh, c16, 0, 0, 0);
+            if (N32 > 0)
+              ct += c16;
+            else
+              ct = c16;
+          }
+          const bool keep = myz & (laneB ? inB : inA);
+          halfx4 ch, cl;
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const float c = keep ? ct[r] : 0.f;
+            ch[r] = (_Float16)c;
+            cl[r] = (_Float16)(c - (float)ch[r]);
+          }
+#pragma unroll
+          for (int tb = 0; tb < NTB; ++tb) {
+            const halfx4 wh = h3_w4(uw + 4 * tb), wl = h3_w4(uw + 4 * tb + 2);
+            D[j][tb] = __builtin_amdgcn_mfma_f32_16x16x16f16(cl, wh, D[j][tb], 0, 0, 0);
+            D[j][tb] = __builtin_amdgcn_mfma_f32_16x16x16f16(ch, wl, D[j][tb], 0, 0, 0);
+            D[j][tb] = __builtin_amdgcn_mfma_f32_16x16x16f16(ch, wh, D[j][tb], 0, 0, 0);
+          }
+        }
+        continue;
+      }
 #pragma unroll
       for (int j = 0; j < SX; ++j) {
         const bool inA = x0 + j >= cxa && x0 + j < cxa + g.px;
@@ -418,6 +574,10 @@ void k_recon_lr(const float* __restrict__ xc,
       const long long vox = ((long long)(x0 + j) * g.Y + (y0 + kq)) * g.Z + (z0 + r);
       const float W = Wsum[j][r];
       const float m = mean[vox];
+      if constexpr (H3) {
+#pragma unroll
+        for (int tb = 0; tb < NTB; ++tb) D[j][tb][r] *= dscale;  // exact: a power of two
+      }
       float* orow = out + vox * T;
 #pragma unroll
       for (int tb = 0; tb < NTB; ++tb) {
@@ -428,7 +588,7 @@ void k_recon_lr(const float* __restrict__ xc,
   }
 }
 
-template <int NTB, int SX>
+template <int NTB, int SX, bool H3>
 static int launch_recon_lr_t(const float* xc, const float* fac, const int* rank,
                              const float* weight, const float* mean, float* out, const Geom& g,
                              int x0, int x1, hipStream_t st) {
@@ -441,26 +601,26 @@ static int launch_recon_lr_t(const float* xc, const float* fac, const int* rank,
   if (g.P >= (1LL << 31)) return set_err(MPPCA_EUNSUPPORTED, "recon: 2^31 patches or more");
   if (g.X > 65535 || g.Y > 65535 || g.Z > 65535)
     return set_err(MPPCA_EUNSUPPORTED, "recon: volume axis above 65535");
-  hipLaunchKernelGGL((k_recon_lr<NTB, SX>), dim3((unsigned)(8 * per_xcd)), dim3(256), 0, st, xc,
+  hipLaunchKernelGGL((k_recon_lr<NTB, SX, H3>), dim3((unsigned)(8 * per_xcd)), dim3(256), 0, st, xc,
                      fac, rank, weight, mean, out, g, bx0, nbx, nby, nbz, per_xcd);
   return launch_status("recon (factors) kernel");
 }
 
 int recon_factor_tile_x(const Geom& g) { return g.tune.recon_sx == 0 ? 2 : g.tune.recon_sx; }
 
-template <int SX>
+template <int SX, bool H3>
 static int launch_recon_lr_sx(const float* xc, const float* fac, const int* rank,
                               const float* weight, const float* mean, float* out, const Geom& g,
                               int x0, int x1, hipStream_t st) {
   switch (ntb_of(g.T)) {
-    case 1: return launch_recon_lr_t<1, SX>(xc, fac, rank, weight, mean, out, g, x0, x1, st);
-    case 2: return launch_recon_lr_t<2, SX>(xc, fac, rank, weight, mean, out, g, x0, x1, st);
-    case 3: return launch_recon_lr_t<3, SX>(xc, fac, rank, weight, mean, out, g, x0, x1, st);
-    case 4: return launch_recon_lr_t<4, SX>(xc, fac, rank, weight, mean, out, g, x0, x1, st);
-    case 5: return launch_recon_lr_t<5, SX>(xc, fac, rank, weight, mean, out, g, x0, x1, st);
-    case 6: return launch_recon_lr_t<6, SX>(xc, fac, rank, weight, mean, out, g, x0, x1, st);
-    case 7: return launch_recon_lr_t<7, SX>(xc, fac, rank, weight, mean, out, g, x0, x1, st);
-    case 8: return launch_recon_lr_t<8, SX>(xc, fac, rank, weight, mean, out, g, x0, x1, st);
+    case 1: return launch_recon_lr_t<1, SX, H3>(xc, fac, rank, weight, mean, out, g, x0, x1, st);
+    case 2: return launch_recon_lr_t<2, SX, H3>(xc, fac, rank, weight, mean, out, g, x0, x1, st);
+    case 3: return launch_recon_lr_t<3, SX, H3>(xc, fac, rank, weight, mean, out, g, x0, x1, st);
+    case 4: return launch_recon_lr_t<4, SX, H3>(xc, fac, rank, weight, mean, out, g, x0, x1, st);
+    case 5: return launch_recon_lr_t<5, SX, H3>(xc, fac, rank, weight, mean, out, g, x0, x1, st);
+    case 6: return launch_recon_lr_t<6, SX, H3>(xc, fac, rank, weight, mean, out, g, x0, x1, st);
+    case 7: return launch_recon_lr_t<7, SX, H3>(xc, fac, rank, weight, mean, out, g, x0, x1, st);
+    case 8: return launch_recon_lr_t<8, SX, H3>(xc, fac, rank, weight, mean, out, g, x0, x1, st);
     default:
       return set_err(MPPCA_EUNSUPPORTED, "recon (factors): T = %d above 128", g.T);
   }
@@ -468,16 +628,20 @@ static int launch_recon_lr_sx(const float* xc, const float* fac, const int* rank
 
 int launch_recon_factors(const float* xc, const float* fac, const int* rank, const float* weight,
                          const float* mean, float* out, const Geom& g, int x0, int x1,
-                         hipStream_t st) {
+                         hipStream_t st, bool h3) {
   // x-slabs per wavefront: 2 unless the tile schedule says otherwise
   const int sx = g.tune.recon_sx == 0 ? 2 : g.tune.recon_sx;
   if (sx != 1 && sx != 2 && sx != 4)
     return set_err(MPPCA_EINVAL, "recon (factors): %d x-slabs per wavefront (1, 2 or 4)", sx);
   if (x0 < 0 || x1 > g.X || x0 % sx != 0 || (x1 != g.X && x1 % sx != 0))
     return set_err(MPPCA_EINVAL, "recon (factors): x range [%d, %d) not tile aligned", x0, x1);
-  if (sx == 1) return launch_recon_lr_sx<1>(xc, fac, rank, weight, mean, out, g, x0, x1, st);
-  if (sx == 4) return launch_recon_lr_sx<4>(xc, fac, rank, weight, mean, out, g, x0, x1, st);
-  return launch_recon_lr_sx<2>(xc, fac, rank, weight, mean, out, g, x0, x1, st);
+  if (h3) {  // the precision form's FP16 hi/lo factors (4-aligned ranges run 2 x-slabs)
+    if (sx == 1) return launch_recon_lr_sx<1, true>(xc, fac, rank, weight, mean, out, g, x0, x1, st);
+    return launch_recon_lr_sx<2, true>(xc, fac, rank, weight, mean, out, g, x0, x1, st);
+  }
+  if (sx == 1) return launch_recon_lr_sx<1, false>(xc, fac, rank, weight, mean, out, g, x0, x1, st);
+  if (sx == 4) return launch_recon_lr_sx<4, false>(xc, fac, rank, weight, mean, out, g, x0, x1, st);
+  return launch_recon_lr_sx<2, false>(xc, fac, rank, weight, mean, out, g, x0, x1, st);
 }
 
 }  // namespace mppca
