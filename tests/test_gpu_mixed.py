@@ -60,13 +60,35 @@ def _check(out, info, vol, patch, stride, parallel=False):
     ((24, 20, 18, 48), (11, 11, 11), 3, 6, 0.03),   # the default geometry, fewer patches
     ((14, 13, 12, 17), (3, 5, 4), 2, 3, 0.05),      # odd T, anisotropic patch
     ((16, 16, 14, 56), (9, 9, 9), 3, 5, 0.04),      # T in 49..64
-    ((20, 20, 16, 96), (7, 7, 7), 3, 8, 0.03),      # projector route, T = 96
+    ((20, 20, 16, 96), (7, 7, 7), 3, 8, 0.03),      # T = 96
     ((14, 14, 12, 128), (5, 5, 5), 2, 6, 0.05),     # T = 128, N < T - 1
+    # FP16 reconstruction: 32- and 16-frame blocks together (T = 80, 112)
+    ((14, 14, 12, 80), (5, 5, 5), 2, 6, 0.02),
+    ((12, 12, 10, 112), (5, 5, 5), 2, 6, 0.04),
 ])
 def test_configs(pkg, shape, patch, stride, rank, noise):
     vol, _ = pkg.make_lowrank_volume(shape, rank, 2.0, noise, 11)
     out, info = pkg.denoise(vol, patch, stride, form="mixed", return_info=True)
     _check(out, info, vol, patch, stride)
+
+
+@pytest.mark.parametrize("shape,patch,stride", [((14, 14, 12, 80), (5, 5, 5), 2),
+                                                 ((14, 14, 12, 48), (5, 5, 5), 2)])
+def test_high_rank_patches(pkg, shape, patch, stride):
+    """A rank-20 signal: retained ranks 8 to 11, patches on the standard
+    (not paired) factor layout. The FP32 reduction's subspace error grows
+    with the rank kept below the MP edge (measured 1.4e-6 at T = 80 and
+    2.2e-6 at T = 48, the same with an FP32 reconstruction), so the bar here
+    is 3e-6; the ranks still match the FP64 oracle's."""
+    global REL_TOL
+    vol, _ = pkg.make_lowrank_volume(shape, 20, 2.0, 0.02, 11)
+    out, info = pkg.denoise(vol, patch, stride, form="mixed", return_info=True)
+    assert int(info["rank"].max()) > 8
+    saved, REL_TOL = REL_TOL, 3e-6
+    try:
+        _check(out, info, vol, patch, stride)
+    finally:
+        REL_TOL = saved
 
 
 @pytest.mark.parametrize("scale", [1e-6, 1e5])
