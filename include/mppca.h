@@ -169,7 +169,9 @@ int mppca_eigh_mp(const float* gram, int64_t nb, int T, int N, int max_sweeps,
  * matrices in, proj[b] = V_p V_p^T as float32, rank / weight / margin /
  * lam_desc (float32, descending; may be NULL) / info (0 ok, -2 non-finite
  * input; may be NULL) out. Only the top-p eigenvectors are computed. gram is
- * used as scratch and clobbered (it is consumed before being overwritten).
+ * used as scratch and clobbered (it is consumed before being overwritten);
+ * each matrix is taken as symmetric, and the default (grid) layout reads its
+ * upper triangle only.
  * Replaces eigh + mp_keep + the projector bmm (step8_tuned.py:134-137) in the
  * PLAIN form. */
 int mppca_eig64_mp(double* gram, int64_t nb, int T, int N, float* proj,

@@ -477,7 +477,10 @@ int run_device_path(const float* vol, float* out, const Geom& g, int form, const
     if (w.fac)
       return launch_recon_factors(w.xc, w.fac, rk, w.weight, w.mean, out, g, 0, g.X, st, true);
   } else if (form == MPPCA_FORM_PLAIN) {
-    if ((rc = launch_gram64(w.xc, (double*)w.gram, g, 0, g.P, st))) return rc;
+    // (the grid eigensolver reads the upper triangle only; the LDS layout
+    // reads whole matrices)
+    if ((rc = launch_gram64(w.xc, (double*)w.gram, g, 0, g.P, st, g.tune.eig == MPPCA_EIG_LDS)))
+      return rc;
     if ((rc = launch_eig64_mp((double*)w.gram, P, T, g.N, w.proj, w.fac, rk, w.weight, mg,
                               nullptr, inf, st, g.tune.eig)))
       return rc;

@@ -302,8 +302,10 @@ int launch_normalize(const float* acc, const float* mean, const float* weight, f
                      const Geom& g, hipStream_t st);
 int launch_gram(const float* xc, float* gram, const Geom& g, long long b0, long long nb,
                 hipStream_t st);
+// (full = false: the strictly lower blocks may be left unwritten, for a
+// consumer that reads the upper triangle only, as the grid eigensolver does)
 int launch_gram64(const float* xc, double* gram, const Geom& g, long long b0, long long nb,
-                  hipStream_t st);
+                  hipStream_t st, bool full = true);
 int launch_eigh_jacobi(const float* gram, float* lam, float* vec, int* info, long long nb, int T,
                        int max_sweeps, hipStream_t st);
 int launch_eigh_mp(const float* gram, long long nb, int T, int N, int max_sweeps, float* proj,

@@ -789,7 +789,8 @@ __global__ __launch_bounds__(64) void k_tri_grid(double* __restrict__ gram, int 
     for (int n = 0; n < R; ++n) {
       const int i = gr + 8 * m, j = gc + 8 * n;
       const int si = rev ? T - 1 - i : i, sj = rev ? T - 1 - j : j;
-      a[m][n] = (i < T && j < T) ? (S)Gin[si * T + sj] : S(0);
+      // (the upper triangle: the int8 Gram writes no other)
+      a[m][n] = (i < T && j < T) ? (S)Gin[min(si, sj) * T + max(si, sj)] : S(0);
       finite = finite && isfinite(a[m][n]);
     }
   if (__any(!finite)) {  // the whole matrix is in registers: flag it
@@ -972,7 +973,8 @@ __global__ __launch_bounds__(256) void k_tri_grid4(double* __restrict__ gram, in
     for (int n = 0; n < R; ++n) {
       const int i = gr + 16 * m, j = gc + 16 * n;
       const int si = rev ? T - 1 - i : i, sj = rev ? T - 1 - j : j;
-      a[m][n] = (i < T && j < T) ? (S)Gin[si * T + sj] : S(0);
+      // (the upper triangle: the int8 Gram writes no other)
+      a[m][n] = (i < T && j < T) ? (S)Gin[min(si, sj) * T + max(si, sj)] : S(0);
       finite = finite && isfinite(a[m][n]);
     }
   if (__syncthreads_or(!finite)) {  // the whole matrix is in registers: flag it

@@ -391,7 +391,7 @@ __host__ __device__ constexpr size_t limb_lds(int ntb, int F) {
 template <int NTB, int RH, bool PAST = false>
 __global__ __launch_bounds__(64 * NTB * (NTB + 1) / 2 * limb_cols(NTB)) void k_gram_limb(
     const float* __restrict__ xc, double* __restrict__ gram, Geom g, int ix0, int ncol,
-    int nseg, int per_xcd) {
+    int nseg, int per_xcd, int mirror) {
   constexpr int NBLK = NTB * (NTB + 1) / 2, NCOL = 64 * NBLK;  // threads per column
   constexpr int TS = 16 * NTB, QS = slide_rows_per_pass(NTB);
   static_assert(QS * TS == NCOL, "staging grid must cover a column's threads");
@@ -518,7 +518,8 @@ __global__ __launch_bounds__(64 * NTB * (NTB + 1) / 2 * limb_cols(NTB)) void k_g
       if (ent_ok && ti < T) {
         const double v = P[r] + snap[r];
         *entry(iz, r) = v;
-        if (!diag) Gc[(long long)iz * T * T + (long long)t2 * T + ti] = v;
+        // (mirror == 0: the consumer reads the upper triangle only)
+        if (mirror && !diag) Gc[(long long)iz * T * T + (long long)t2 * T + ti] = v;
       }
     }
   };
@@ -1014,7 +1015,7 @@ static int launch_gram_slide_t(const float* xc, double* gram, const Geom& g, int
 
 template <int NTB, int RH, bool PAST = false>
 static int launch_gram_limb_t(const float* xc, double* gram, const Geom& g, int ix0, int nix,
-                              hipStream_t st) {
+                              hipStream_t st, bool full) {
   constexpr int NTHR = 64 * NTB * (NTB + 1) / 2 * limb_cols(NTB);
   static std::atomic<long long> cache[kMaxDevices];
   const size_t lds = limb_lds(NTB, g.px * g.py);
@@ -1027,7 +1028,7 @@ static int launch_gram_limb_t(const float* xc, double* gram, const Geom& g, int 
                               &nblk, &per_xcd);
   if (nseg == 0) return set_err(MPPCA_EUNSUPPORTED, "gram (int8 sliding): too many columns");
   hipLaunchKernelGGL((k_gram_limb<NTB, RH, PAST>), dim3(8u * (unsigned)per_xcd), dim3(NTHR), lds, st, xc,
-                     gram, g, ix0, ncol, nseg, per_xcd);
+                     gram, g, ix0, ncol, nseg, per_xcd, full ? 1 : 0);
   return launch_status("gram (int8 sliding) kernel");
 }
 
@@ -1055,7 +1056,7 @@ __global__ __launch_bounds__(256) void k_gram_mirror(double* __restrict__ gram, 
 
 template <int NTB, int RH>
 static int launch_gram_limbw_t(const float* xc, double* gram, const Geom& g, int ix0, int nix,
-                               hipStream_t st) {
+                               hipStream_t st, bool full) {
   constexpr int NTHR = 64 * limbw_waves(NTB, RH);
   static std::atomic<long long> cache[kMaxDevices];
   const size_t lds = limbw_lds(NTB, RH);
@@ -1070,6 +1071,7 @@ static int launch_gram_limbw_t(const float* xc, double* gram, const Geom& g, int
   hipLaunchKernelGGL((k_gram_limbw<NTB, RH>), dim3(8u * (unsigned)per_xcd), dim3(NTHR), lds, st, xc,
                      gram, g, ix0, ncol, nseg, per_xcd);
   if ((rc = launch_status("gram (int8 sliding, wide) kernel"))) return rc;
+  if (!full) return MPPCA_OK;  // (the consumer reads the upper triangle only)
   const long long nmat = (long long)ncol * g.ncz * g.nvol;
   if (nmat >= (1LL << 31)) return set_err(MPPCA_EUNSUPPORTED, "gram (mirror): too many patches");
   const int nt = (g.T + 15) / 16;
@@ -1087,17 +1089,17 @@ static bool limbw_fits(const Geom& g) {
 }
 
 static int launch_gram_limbw(const float* xc, double* gram, const Geom& g, int ix0, int nix,
-                             hipStream_t st) {
+                             hipStream_t st, bool full) {
   const bool two = g.px * g.py > 64;
   switch (ntb_of(g.T)) {
-    case 5: return two ? launch_gram_limbw_t<5, 2>(xc, gram, g, ix0, nix, st)
-                       : launch_gram_limbw_t<5, 1>(xc, gram, g, ix0, nix, st);
-    case 6: return two ? launch_gram_limbw_t<6, 2>(xc, gram, g, ix0, nix, st)
-                       : launch_gram_limbw_t<6, 1>(xc, gram, g, ix0, nix, st);
-    case 7: return two ? launch_gram_limbw_t<7, 2>(xc, gram, g, ix0, nix, st)
-                       : launch_gram_limbw_t<7, 1>(xc, gram, g, ix0, nix, st);
-    default: return two ? launch_gram_limbw_t<8, 2>(xc, gram, g, ix0, nix, st)
-                        : launch_gram_limbw_t<8, 1>(xc, gram, g, ix0, nix, st);
+    case 5: return two ? launch_gram_limbw_t<5, 2>(xc, gram, g, ix0, nix, st, full)
+                       : launch_gram_limbw_t<5, 1>(xc, gram, g, ix0, nix, st, full);
+    case 6: return two ? launch_gram_limbw_t<6, 2>(xc, gram, g, ix0, nix, st, full)
+                       : launch_gram_limbw_t<6, 1>(xc, gram, g, ix0, nix, st, full);
+    case 7: return two ? launch_gram_limbw_t<7, 2>(xc, gram, g, ix0, nix, st, full)
+                       : launch_gram_limbw_t<7, 1>(xc, gram, g, ix0, nix, st, full);
+    default: return two ? launch_gram_limbw_t<8, 2>(xc, gram, g, ix0, nix, st, full)
+                        : launch_gram_limbw_t<8, 1>(xc, gram, g, ix0, nix, st, full);
   }
 }
 
@@ -1128,7 +1130,7 @@ static int limb_ntb(const Geom& g) {
 }
 
 int launch_gram64(const float* xc, double* gram, const Geom& g, long long b0, long long nb,
-                  hipStream_t st) {
+                  hipStream_t st, bool full) {
   // whole x-corner columns: a sliding-window kernel
   const long long per_ix = (long long)g.ncy * g.ncz;
   const bool cols = nb > 0 && b0 % per_ix == 0 && nb % per_ix == 0;
@@ -1140,24 +1142,24 @@ int launch_gram64(const float* xc, double* gram, const Geom& g, long long b0, lo
   const bool wide = cols && (want == MPPCA_GRAM_AUTO || want == MPPCA_GRAM_LIMB) && limbw_fits(g);
   if (want == MPPCA_GRAM_LIMB && lntb == 0 && !wide)
     return set_err(MPPCA_EUNSUPPORTED, "gram64: the int8 sliding kernel does not take this launch");
-  if (wide) return launch_gram_limbw(xc, gram, g, (int)(b0 / per_ix), (int)(nb / per_ix), st);
+  if (wide) return launch_gram_limbw(xc, gram, g, (int)(b0 / per_ix), (int)(nb / per_ix), st, full);
   if (want == MPPCA_GRAM_SLIDE && !(cols && slide_fits(g)))
     return set_err(MPPCA_EUNSUPPORTED, "gram64: the FP64 sliding kernel does not take this launch");
   if (lntb > 0) {
     const int ix0 = (int)(b0 / per_ix), nix = (int)(nb / per_ix);
     const bool two = limb_rows(g.px * g.py) > 64;  // tile rows of 128 voxels
     switch (lntb) {
-      case 1: return launch_gram_limb_t<1, 1>(xc, gram, g, ix0, nix, st);
-      case 2: return launch_gram_limb_t<2, 1>(xc, gram, g, ix0, nix, st);
+      case 1: return launch_gram_limb_t<1, 1>(xc, gram, g, ix0, nix, st, full);
+      case 2: return launch_gram_limb_t<2, 1>(xc, gram, g, ix0, nix, st, full);
       case 3:
         if (g.T <= 32)  // the 48-frame kernel for 17..32 frames: blocks past T
-          return two ? launch_gram_limb_t<3, 2, true>(xc, gram, g, ix0, nix, st)
-                     : launch_gram_limb_t<3, 1, true>(xc, gram, g, ix0, nix, st);
-        return two ? launch_gram_limb_t<3, 2>(xc, gram, g, ix0, nix, st)
-                   : launch_gram_limb_t<3, 1>(xc, gram, g, ix0, nix, st);
+          return two ? launch_gram_limb_t<3, 2, true>(xc, gram, g, ix0, nix, st, full)
+                     : launch_gram_limb_t<3, 1, true>(xc, gram, g, ix0, nix, st, full);
+        return two ? launch_gram_limb_t<3, 2>(xc, gram, g, ix0, nix, st, full)
+                   : launch_gram_limb_t<3, 1>(xc, gram, g, ix0, nix, st, full);
       default:
-        return two ? launch_gram_limb_t<4, 2>(xc, gram, g, ix0, nix, st)
-                   : launch_gram_limb_t<4, 1>(xc, gram, g, ix0, nix, st);
+        return two ? launch_gram_limb_t<4, 2>(xc, gram, g, ix0, nix, st, full)
+                   : launch_gram_limb_t<4, 1>(xc, gram, g, ix0, nix, st, full);
     }
   }
   if (cols && slide_fits(g) && want != MPPCA_GRAM_PATCH) {

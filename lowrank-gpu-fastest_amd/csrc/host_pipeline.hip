@@ -651,7 +651,8 @@ extern "C" int mppca_denoise_host(const float* vol, float* out, int X, int Y, in
       if (mixed) {
         if ((rc = mixed_first_pass(g, w, b0, nb, rk, mg, inf, S))) break;
       } else {
-        if ((rc = launch_gram64(w.xc, (double*)w.gram + b0 * T * T, g, b0, nb, S)) ||
+        if ((rc = launch_gram64(w.xc, (double*)w.gram + b0 * T * T, g, b0, nb, S,
+                                g.tune.eig == MPPCA_EIG_LDS)) ||
             (rc = hip_ok(hipEventRecord(C.ev_gram[set], S), "event")))
           break;
         prev_gram = C.ev_gram[set];
