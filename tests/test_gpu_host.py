@@ -138,3 +138,25 @@ def test_host_pipeline_slabs_without_corners(pkg):
             got = d(vols[k % 2])
             _assert_same(got, wants[k % 2][0], "factors")
             np.testing.assert_array_equal(d.info()["rank"], wants[k % 2][1])
+
+
+def test_result_buffers_outlive_later_calls(pkg):
+    """Results are pinned buffers that a later call reuses only once the
+    caller has dropped the array (or every view of it)."""
+    import gc
+    shape, patch, stride = (30, 14, 14, 16), (5, 5, 5), 2
+    v1, v2 = _volume(shape, 41), _volume(shape, 42)
+    d = pkg.Denoiser(shape, patch, stride)
+    a1 = d(v1)
+    c1 = a1.copy()
+    view = a1[3:]
+    del a1
+    a2 = d(v2)
+    c2 = a2.copy()
+    np.testing.assert_array_equal(view, c1[3:])  # still v1's result
+    assert not np.array_equal(c1, c2)
+    del view, a2
+    gc.collect()
+    for _ in range(4):
+        np.testing.assert_array_equal(d(v1), c1)  # each result dropped at once
+    assert len(d._outs) == 2
