@@ -388,7 +388,7 @@ __host__ __device__ constexpr size_t limb_lds(int ntb, int F) {
   return (size_t)limb_cols(ntb) * limb_col_bytes(ntb, F);
 }
 
-template <int NTB, int RH, bool PAST = false>
+template <int NTB, int RH, bool PAST = false, int KR = 0>
 __global__ __launch_bounds__(64 * NTB * (NTB + 1) / 2 * limb_cols(NTB)) void k_gram_limb(
     const float* __restrict__ xc, double* __restrict__ gram, Geom g, int ix0, int ncol,
     int nseg, int per_xcd, int mirror) {
@@ -490,10 +490,24 @@ __global__ __launch_bounds__(64 * NTB * (NTB + 1) / 2 * limb_cols(NTB)) void k_g
     return Gc + (long long)iz * T * T + (long long)(tb1 * 16 + 4 * kq + r) * T + t2;
   };
   double P[4] = {0.0, 0.0, 0.0, 0.0};
+  // KR > 0: -P of the open windows in registers, window w in slot w mod KR
+  // (the launcher checks that no more than KR windows of a column are open
+  // at once); the slot index is wave-uniform, so each slot is a scalar branch
+  double ring[KR > 0 ? KR : 1][4];
   auto start = [&](int iz) {
+    if constexpr (KR > 0) {
+      const int slot = iz % KR;
 #pragma unroll
-    for (int r = 0; r < 4; ++r)
-      if (ent_ok && tb1 * 16 + 4 * kq + r < T) *entry(iz, r) = -P[r];
+      for (int k = 0; k < KR; ++k)
+        if (slot == k) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) ring[k][r] = -P[r];
+        }
+    } else {
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        if (ent_ok && tb1 * 16 + 4 * kq + r < T) *entry(iz, r) = -P[r];
+    }
   };
   // -P of the window that ends at a step's plane, loaded at the step's start
   // (after that step's start store, which it may read back when pz = 1) so
@@ -503,6 +517,7 @@ __global__ __launch_bounds__(64 * NTB * (NTB + 1) / 2 * limb_cols(NTB)) void k_g
   // no branch merges its wait with the plane loads' (those stay in flight).
   double snap[4];
   auto load_snap = [&](int iz) {
+    if constexpr (KR > 0) return;
     const int izc = iz < w1 ? iz : w1 - 1, tc = t2 < T ? t2 : T - 1;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
@@ -512,6 +527,15 @@ __global__ __launch_bounds__(64 * NTB * (NTB + 1) / 2 * limb_cols(NTB)) void k_g
   };
   auto finish = [&](int iz) {
     // the stores of -P precede the loads of snap in this wave's program order
+    if constexpr (KR > 0) {
+      const int slot = iz % KR;
+#pragma unroll
+      for (int k = 0; k < KR; ++k)
+        if (slot == k) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) snap[r] = ring[k][r];
+        }
+    }
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int ti = tb1 * 16 + 4 * kq + r;
@@ -1013,23 +1037,51 @@ static int launch_gram_slide_t(const float* xc, double* gram, const Geom& g, int
   return launch_status("gram (sliding) kernel");
 }
 
-template <int NTB, int RH, bool PAST = false>
-static int launch_gram_limb_t(const float* xc, double* gram, const Geom& g, int ix0, int nix,
+template <int NTB, int RH, bool PAST, int KR>
+static int launch_gram_limb_k(const float* xc, double* gram, const Geom& g, int ix0, int nix,
                               hipStream_t st, bool full) {
   constexpr int NTHR = 64 * NTB * (NTB + 1) / 2 * limb_cols(NTB);
   static std::atomic<long long> cache[kMaxDevices];
+  const void* fn = (const void*)k_gram_limb<NTB, RH, PAST, KR>;
   const size_t lds = limb_lds(NTB, g.px * g.py);
-  int rc = prepare_lds((const void*)k_gram_limb<NTB, RH, PAST>, lds);
+  int rc = prepare_lds(fn, lds);
   if (rc != MPPCA_OK) return rc;
   const int ncol = nix * g.ncy;
   int nblk = 0, per_xcd = 0;
-  const int nseg = slide_runs(g, ncol * g.nvol, limb_cols(NTB),
-                              resident_slots((const void*)k_gram_limb<NTB, RH, PAST>, NTHR, lds, cache),
+  const int nseg = slide_runs(g, ncol * g.nvol, limb_cols(NTB), resident_slots(fn, NTHR, lds, cache),
                               &nblk, &per_xcd);
   if (nseg == 0) return set_err(MPPCA_EUNSUPPORTED, "gram (int8 sliding): too many columns");
-  hipLaunchKernelGGL((k_gram_limb<NTB, RH, PAST>), dim3(8u * (unsigned)per_xcd), dim3(NTHR), lds, st, xc,
-                     gram, g, ix0, ncol, nseg, per_xcd, full ? 1 : 0);
+  hipLaunchKernelGGL((k_gram_limb<NTB, RH, PAST, KR>), dim3(8u * (unsigned)per_xcd), dim3(NTHR), lds,
+                     st, xc, gram, g, ix0, ncol, nseg, per_xcd, full ? 1 : 0);
   return launch_status("gram (int8 sliding) kernel");
+}
+
+// The most windows of one column open at once: window w is open from its
+// corner to its corner + pz - 1, and a window starting at another's last
+// plane starts before that one ends.
+static int open_windows(const Geom& g) {
+  int most = 0;
+  for (int w = 0, v = 0; w < g.ncz; ++w) {
+    const int last = corner_of(w, g.nsz, g.Z, g.pz, g.stride) + g.pz - 1;
+    if (v < w + 1) v = w + 1;
+    while (v < g.ncz && corner_of(v, g.nsz, g.Z, g.pz, g.stride) <= last) ++v;
+    most = v - w > most ? v - w : most;
+  }
+  return most;
+}
+
+// The window snapshots live in a ring of 5 register slots when a column
+// never holds more than 5 open windows (11^3 patches at stride >= 3: four
+// from the stride, one more from a last corner clamped to Z - pz), and in
+// the Gram buffer otherwise (stored at a window's first plane, read back
+// after its last).
+constexpr int kLimbRing = 5;
+template <int NTB, int RH, bool PAST = false>
+static int launch_gram_limb_t(const float* xc, double* gram, const Geom& g, int ix0, int nix,
+                              hipStream_t st, bool full) {
+  return open_windows(g) <= kLimbRing
+             ? launch_gram_limb_k<NTB, RH, PAST, kLimbRing>(xc, gram, g, ix0, nix, st, full)
+             : launch_gram_limb_k<NTB, RH, PAST, 0>(xc, gram, g, ix0, nix, st, full);
 }
 
 // Lower triangle of each T x T matrix from its upper one (the wide int8
