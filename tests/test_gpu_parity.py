@@ -64,6 +64,9 @@ def test_golden_volumes(pkg, golden):
     ((32, 32, 24, 16), (5, 5, 5), 2, 4, 0.05),      # small workload, 2475 patches
     ((20, 18, 16, 30), (7, 7, 7), 2, 5, 0.04),
     ((24, 20, 18, 48), (11, 11, 11), 3, 6, 0.03),
+    # Z - 11 = 1 mod 3: the clamped last corner opens a fifth window beside
+    # four strided ones (as at Z = 48), every slot of the Gram's register ring
+    ((22, 20, 24, 48), (11, 11, 11), 3, 6, 0.03),
     ((14, 13, 12, 17), (3, 5, 4), 2, 3, 0.05),      # odd T, anisotropic patch
     ((12, 12, 12, 64), (7, 7, 7), 2, 6, 0.03),
     ((11, 11, 11, 20), (11, 11, 11), 3, 4, 0.05),   # one patch = the whole volume
