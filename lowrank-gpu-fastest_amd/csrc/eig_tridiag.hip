@@ -1844,6 +1844,10 @@ static int launch_rows(double* gram, long long nb, int T, int N, float* proj, fl
   // the precision form's solves reduce from the low-energy end of the frames
   if (mode == kRowsMixed)
     launch_tri<TP, float, float, true>(gram, nb, T, st);
+  else if (mode == kRowsResolve && (TP == 48 || TP == 64))
+    // the re-solve runs a few matrices: the four-wavefront grid's latency
+    hipLaunchKernelGGL((k_tri_grid4<TP == 48 || TP == 64 ? TP : 80, double, double, true>),
+                       dim3((unsigned)nb), dim3(256), 0, st, gram, T, nb);
   else if (mode == kRowsResolve)
     launch_tri<TP, double, double, true>(gram, nb, T, st);
   else
