@@ -99,8 +99,10 @@ class Denoiser:
         # 2.80 at 7 and 2.89 at 16; more slabs are slower because every launch
         # of a slab costs at least one pass of its longest workgroup, however
         # few workgroups it has; 6 threads: more take cores from the threads
-        # that drive the GPU)
-        self.nslabs = 9
+        # that drive the GPU). The precision form re-solves the close calls
+        # of every slab (a chain of small launches), so fewer slabs suit it:
+        # 7 slabs 2.90 ms against 3.06 at 9 (5-9 swept)
+        self.nslabs = 7 if self.form == "mixed" else 9
         self.nthreads = max(1, min(6, os.cpu_count() or 1))
 
     def _geom_args(self):
