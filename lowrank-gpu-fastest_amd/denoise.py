@@ -100,9 +100,10 @@ class Denoiser:
         # of a slab costs at least one pass of its longest workgroup, however
         # few workgroups it has; 6 threads: more take cores from the threads
         # that drive the GPU). The precision form re-solves the close calls
-        # of every slab (a chain of small launches), so fewer slabs suit it:
-        # 7 slabs 2.90 ms against 3.06 at 9 (5-9 swept)
-        self.nslabs = 7 if self.form == "mixed" else 9
+        # of every slab (a chain of small launches), so fewer slabs suit it
+        # at 48 frames: 7 slabs 2.90 ms against 3.06 at 9 (5-9 swept); at 96
+        # frames 9 stay best (7.09 ms against 7.25 at 7 and 7.61 at 11)
+        self.nslabs = 7 if self.form == "mixed" and self.shape[3] <= 64 else 9
         self.nthreads = max(1, min(6, os.cpu_count() or 1))
 
     def _geom_args(self):
